@@ -1,0 +1,295 @@
+/*
+ * mipx_pdlp.h -- C-ABI of the MI355X PDLP library (libmipx_pdlp.so).
+ *
+ * This is the drop-in boundary for mipx's GPU PDLP path.  In the reference
+ * the GPU path is `PdlpSolver::solveGpu()` (src/lp/pdlp.cpp:1739-2792) plus
+ * the kernels of src/lp/pdlp_kernels.cu / .cuh and cuSPARSE SpMV calls
+ * (src/lp/pdlp.cpp:1899-1978).  Everything here uses plain C types: no HIP,
+ * CUDA or C++ types cross the boundary and every function returns an int
+ * error code (MPX_OK == 0); the last error text of the calling thread is
+ * available from mpx_last_error().
+ *
+ * Numbers: values are IEEE double (the reference's `Real`, include/mipx/core.h:7),
+ * indices are int32 (`Index`, include/mipx/core.h:9).  Status codes follow the
+ * order of `mipx::Status` (include/mipx/core.h:11-19).
+ */
+#ifndef MIPX_PDLP_H
+#define MIPX_PDLP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define MPX_OK 0
+#define MPX_ERR_INVALID 1      /* bad argument (sizes, null pointers, ...) */
+#define MPX_ERR_DEVICE 2       /* a HIP runtime call failed */
+#define MPX_ERR_OUT_OF_MEMORY 3
+#define MPX_ERR_UNSUPPORTED 4  /* option combination not implemented */
+#define MPX_ERR_BUSY 5         /* serving loop: no slot / budget free */
+
+/* ---- solver status (same order as mipx::Status, core.h:11-19) ---------- */
+#define MPX_STATUS_OPTIMAL 0
+#define MPX_STATUS_INFEASIBLE 1
+#define MPX_STATUS_UNBOUNDED 2
+#define MPX_STATUS_ITER_LIMIT 3
+#define MPX_STATUS_TIME_LIMIT 4
+#define MPX_STATUS_NODE_LIMIT 5
+#define MPX_STATUS_ERROR 6
+
+/* ---- precision modes (MI355X extension; FP64 is the reference's) ------- */
+#define MPX_PREC_FP64 0   /* every iteration in FP64 (the reference's GPU path) */
+#define MPX_PREC_MIXED 1  /* FP32 iterations with FP32 accumulation; FP64 eval,
+                             termination and a final FP64 refinement */
+
+/*
+ * An LP in the reference's ranged form (include/mipx/lp_problem.h:14-64):
+ *   min/max  obj'x + obj_offset   s.t.  row_lower <= A x <= row_upper,
+ *                                       col_lower <= x <= col_upper.
+ * A is CSR with column indices sorted within each row
+ * (include/mipx/sparse_matrix.h:36-42).  Infinite bounds are +-INFINITY.
+ */
+typedef struct mpx_lp {
+    int32_t num_rows;
+    int32_t num_cols;
+    int32_t nnz;
+    int32_t sense;                /* 0 = Minimize, 1 = Maximize (core.h:21-24) */
+    const int32_t* row_starts;    /* [num_rows + 1] */
+    const int32_t* col_indices;   /* [nnz] */
+    const double* values;         /* [nnz] */
+    const double* obj;            /* [num_cols] */
+    const double* col_lower;      /* [num_cols] */
+    const double* col_upper;      /* [num_cols] */
+    const double* row_lower;      /* [num_rows] */
+    const double* row_upper;      /* [num_rows] */
+    double obj_offset;
+} mpx_lp;
+
+/* Cooperative cancellation: polled between iteration blocks, like the
+ * reference's `stop_flag` (include/mipx/pdlp.h:86, src/lp/pdlp.cpp:2136-2140).
+ * Return nonzero to stop; the solve then reports MPX_STATUS_ITER_LIMIT. */
+typedef int (*mpx_stop_fn)(void* user);
+
+/*
+ * Mirrors `mipx::PdlpOptions` (include/mipx/pdlp.h:19-87) field by field,
+ * same names and defaults (mpx_pdlp_default_options fills them).  Booleans
+ * are int32 0/1.  The fields after `stop_user` are MI355X extensions.
+ */
+typedef struct mpx_pdlp_options {
+    int32_t max_iter;                 /* 4000000 */
+    double optimality_tol;            /* 5e-5 */
+    double primal_tol;                /* 5e-5 */
+    double dual_tol;                  /* 5e-5 */
+    double max_solve_seconds;         /* -1 = no limit */
+    int32_t optimality_norm;          /* 0 = L2, 1 = LInf */
+    int32_t sv_max_iter;              /* 5000 */
+    double sv_tol;                    /* 1e-4 */
+    int32_t adaptive_step;            /* 0 */
+    double adaptive_step_down;        /* 0.7 */
+    double adaptive_step_up;          /* 1.0 */
+    double adaptive_step_min;         /* 1e-6 */
+    double adaptive_step_max;         /* 1e3 */
+    double primal_weight;             /* 1.0 */
+    int32_t update_primal_weight;     /* 1 */
+    double pid_kp;                    /* 0.99 */
+    double pid_ki;                    /* 0.01 */
+    double pid_kd;                    /* 0.0 */
+    double pid_i_smooth;              /* 0.3 */
+    double restart_sufficient_decay;  /* 0.2 */
+    double restart_necessary_decay;   /* 0.5 */
+    double restart_artificial_fraction; /* 0.36 */
+    int32_t termination_eval_frequency; /* 200 */
+    double matrix_zero_tol;           /* 0.0 */
+    int32_t enable_polishing;         /* 1 */
+    double polish_primal_tol;         /* 1e-6 */
+    double polish_dual_tol;           /* 1e-6 */
+    int32_t polish_max_iter;          /* 500000 */
+    int32_t do_ruiz_scaling;          /* 1 */
+    int32_t ruiz_iterations;          /* 10 */
+    int32_t do_pock_chambolle_scaling; /* 1 */
+    int32_t do_bound_obj_rescaling;   /* 1 */
+    int32_t preconditioner_refresh;   /* 0 */
+    double preconditioner_refresh_ratio; /* 10.0 */
+    int32_t anderson_acceleration;    /* 0 */
+    int32_t anderson_window_size;     /* 5 */
+    double anderson_regularization;   /* 1e-10 */
+    int32_t use_gpu;                  /* 1 (this library always runs on the GPU) */
+    int32_t gpu_min_rows;             /* 512 (gate evaluated by the caller) */
+    int32_t gpu_min_nnz;              /* 10000 (gate evaluated by the caller) */
+    int32_t verbose;                  /* 1 */
+    mpx_stop_fn stop_fn;              /* NULL */
+    void* stop_user;                  /* NULL */
+    /* ---- MI355X extensions ---- */
+    int32_t precision;                /* MPX_PREC_FP64 */
+    double refine_tol;                /* mixed precision: FP64 refinement target
+                                         for primal/dual/gap; <=0 = the solve's
+                                         own tolerances */
+    int32_t spmv_group;               /* lanes per row/column in the SpMV
+                                         kernels, 0 = autotuned.  1 gives the
+                                         sequential per-row summation order of
+                                         the reference's CPU SpMV. */
+    int32_t device;                   /* HIP device ordinal, default 0 */
+} mpx_pdlp_options;
+
+/* Mirrors `mipx::LpResult` (include/mipx/lp_solver.h:11-16) plus the final
+ * termination quantities (reference definitions: src/lp/pdlp.cpp:1105-1202). */
+typedef struct mpx_pdlp_result {
+    int32_t status;          /* MPX_STATUS_* */
+    int32_t iterations;      /* total PDHG iterations incl. polish */
+    double objective;        /* offset + obj'x on the recovered x (pdlp.cpp:1698-1703) */
+    double work_units;       /* iterations * (4 nnz + 2 (m + n)) (pdlp.cpp:1716-1719) */
+    double rel_primal;       /* last evaluated relative primal residual */
+    double rel_dual;         /* last evaluated relative dual residual */
+    double rel_gap;          /* last evaluated relative gap */
+    double primal_obj;       /* last evaluated primal objective (original units) */
+    double dual_obj;         /* last evaluated dual objective (original units) */
+    double step;             /* 0.998 / ||A_scaled||_2 */
+    double primal_weight;    /* final primal weight */
+    int32_t restarts;
+    int32_t iterations_fp32; /* MPX_PREC_MIXED: iterations run in FP32 */
+    double seconds;          /* wall time inside the library */
+    double setup_seconds;    /* upload + scaling + power iteration */
+} mpx_pdlp_result;
+
+void mpx_pdlp_default_options(mpx_pdlp_options* opts);
+const char* mpx_last_error(void);
+const char* mpx_version(void);
+int mpx_device_count(int32_t* count);
+
+/*
+ * Solve one LP on the GPU.  Replaces `PdlpSolver::solveGpu()`
+ * (src/lp/pdlp.cpp:1739-2792): upload, on-device Ruiz / Pock-Chambolle /
+ * bound-objective scaling (reference host code pdlp.cpp:469-669), on-device
+ * power iteration (pdlp.cpp:1988-2041), the restarted Halpern PDHG loop in
+ * hipGraph-captured blocks of `termination_eval_frequency` iterations, the
+ * polish phase and recovery to original space (pdlp.cpp:2678-2712).
+ *
+ * warm_x[num_cols] / warm_y[num_rows] may be NULL (reference setWarmStart,
+ * pdlp.cpp:457-467,1812-1827).  x_out[num_cols] and y_out[num_rows] receive
+ * the recovered primal and dual (on Infeasible: the Farkas ray in y_out; on
+ * Unbounded: the primal ray in x_out; pdlp.cpp:2318-2340); either may be NULL.
+ * The trivial n == 0 and m == 0 cases are handled in closed form
+ * (pdlp.cpp:812-860).  Returns MPX_OK unless the arguments are invalid; a
+ * device failure is reported as MPX_STATUS_ERROR in `res`.
+ */
+int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts,
+                   const double* warm_x, const double* warm_y,
+                   double* x_out, double* y_out, mpx_pdlp_result* res);
+
+/*
+ * Batched solve of `count` LPs that share one constraint matrix, objective
+ * and row bounds and differ only in column bounds -- branch-and-bound node
+ * relaxations (the reference's caller: mip_solver.cpp node LPs via
+ * setColBounds, pdlp.cpp:2890-2898).  col_lower/col_upper are [count][num_cols]
+ * row-major (NULL = use base->col_lower / col_upper for every instance).
+ * All instances run in one persistent kernel with per-instance restart,
+ * primal-weight and termination logic on the device.  x_out is
+ * [count][num_cols], y_out [count][num_rows] (either may be NULL); results[count].
+ */
+int mpx_pdlp_solve_batch_shared(const mpx_lp* base, int32_t count,
+                                const double* col_lower, const double* col_upper,
+                                const mpx_pdlp_options* opts,
+                                double* x_out, double* y_out,
+                                mpx_pdlp_result* results);
+
+/*
+ * Batched solve of `count` independent LPs of uneven sizes in one persistent
+ * launch.  x_out[i] / y_out[i] are per-LP output arrays (or NULL).
+ */
+int mpx_pdlp_solve_batch(const mpx_lp* lps, int32_t count,
+                         const mpx_pdlp_options* opts,
+                         double* const* x_out, double* const* y_out,
+                         mpx_pdlp_result* results);
+
+/* ---- serving loop (continuous batching) ---------------------------------
+ * A server owns a device context and an HBM budget.  Submitted requests are
+ * admitted into device slots as memory and slots free up; every admitted LP
+ * iterates in the same persistent launch as the others; finished slots are
+ * recycled while others keep running.  Results are identical to a solo
+ * mpx_pdlp_solve_batch() of the same LP with the same options.
+ */
+typedef struct mpx_server mpx_server;
+int mpx_server_create(const mpx_pdlp_options* opts, double hbm_budget_bytes,
+                      int32_t max_slots, mpx_server** out);
+/* Copies the LP; returns a ticket >= 0 in *ticket. */
+int mpx_server_submit(mpx_server* srv, const mpx_lp* lp, int64_t* ticket);
+/* Runs one scheduling round: admit, iterate one block, retire.  Returns the
+ * number of requests still pending or running in *active. */
+int mpx_server_step(mpx_server* srv, int32_t* active);
+/* Fetches a finished result (x_out/y_out may be NULL).  *done = 0 if the
+ * ticket is not finished yet. */
+int mpx_server_result(mpx_server* srv, int64_t ticket, int32_t* done,
+                      double* x_out, double* y_out, mpx_pdlp_result* res);
+int mpx_server_destroy(mpx_server* srv);
+
+/* ---- device-pointer kernels (lower-level entry points) -------------------
+ * These operate on device buffers the caller owns (e.g. from hipMalloc or a
+ * framework allocator) on the caller's HIP stream (`stream` is a hipStream_t
+ * passed as void*, NULL = default stream).  They are the building blocks of
+ * mpx_pdlp_solve and are used by the parity tests.
+ */
+
+/* y = A x (CSR, FP64).  Reference: SparseMatrix::multiply
+ * (src/lp/sparse_matrix.cpp:181-193) / cusparseSpMV (pdlp.cpp:1969-1973).
+ * group = lanes per row (1, 2, 4, ..., 64; 0 = auto).  group == 1 sums each
+ * row sequentially in storage order, as the reference's CPU loop does. */
+int mpx_spmv_csr_f64(int32_t rows, int32_t cols, const int32_t* row_starts,
+                     const int32_t* col_indices, const double* values,
+                     const double* x, double* y, int32_t group, void* stream);
+int mpx_spmv_csr_f32(int32_t rows, int32_t cols, const int32_t* row_starts,
+                     const int32_t* col_indices, const float* values,
+                     const float* x, float* y, int32_t group, void* stream);
+
+/*
+ * Scaled problem, as produced on the device by the reference's
+ * buildScaledProblem + buildTransposeCSR (src/lp/pdlp.cpp:469-707) and the
+ * power iteration (pdlp.cpp:748-795).  Host buffers, filled by
+ * mpx_pdlp_scale_problem (any pointer may be NULL to skip it).
+ */
+typedef struct mpx_scaled_problem {
+    double* values;        /* [nnz] scaled A, CSR order */
+    double* at_values;     /* [nnz] scaled A' (CSC of A, rows increasing) */
+    int32_t* at_row_starts; /* [num_cols + 1] */
+    int32_t* at_col_indices; /* [nnz] */
+    double* cscaled;       /* [n] */
+    double* col_lower;     /* [n] scaled, +-inf kept */
+    double* col_upper;     /* [n] */
+    double* row_lower;     /* [m] */
+    double* row_upper;     /* [m] */
+    double* row_scale;     /* [m] */
+    double* col_scale;     /* [n] */
+    double* sigma_base;    /* [m] */
+    double* tau_base;      /* [n] */
+    double objective_scale;
+    double constraint_scale;
+    double spectral_norm;  /* sqrt(max(sigma^2, 1e-12)) from the power iteration */
+    int32_t power_iterations;
+} mpx_scaled_problem;
+
+int mpx_pdlp_scale_problem(const mpx_lp* lp, const mpx_pdlp_options* opts,
+                           mpx_scaled_problem* out);
+
+/*
+ * Runs `iters` Halpern PDHG iterations (reference loop body,
+ * src/lp/pdlp.cpp:980-1072; GPU kernels pdlp_kernels.cu:46-157) on the
+ * device from the given scaled state, with inner-iteration offset
+ * `inner_count` (lambda = (inner+k)/(inner+k+1)), fixed step and primal
+ * weight.  State arrays are host buffers updated in place:
+ * current_x, initial_x, pdhg_x [n]; current_y, initial_y, pdhg_y [m].
+ * `sp` provides values/at_* structure, cscaled, bounds, sigma/tau bases.
+ * precision selects FP64 or FP32 iterations.
+ */
+int mpx_pdhg_iterate(const mpx_lp* lp, const mpx_scaled_problem* sp,
+                     int32_t iters, int32_t inner_count, double step,
+                     double primal_weight, int32_t precision, int32_t group,
+                     double* current_x, double* initial_x, double* pdhg_x,
+                     double* current_y, double* initial_y, double* pdhg_y);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* MIPX_PDLP_H */

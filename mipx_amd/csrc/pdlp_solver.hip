@@ -1,0 +1,1388 @@
+// pdlp_solver.hip -- single-LP PDLP on one MI355X, and the C-ABI entry points.
+//
+// mpx_pdlp_solve() replaces the reference's PdlpSolver::solveGpu()
+// (src/lp/pdlp.cpp:1739-2792).  Flow:
+//   1. upload the raw CSR problem once; the CSC structure (A' in CSR) and the
+//      CSR->CSC permutation are built on the host (index work only);
+//   2. scaling on the device (pdlp_setup.h), bit-identical per element to the
+//      reference host code;
+//   3. power iteration on the device with one host check per captured group
+//      of steps (the reference syncs three times per step, pdlp.cpp:1988-2041);
+//   4. Halpern PDHG in hipGraph-captured blocks of N = termination_eval_frequency
+//      iterations (2 fused kernels per iteration); step, primal weight and the
+//      inner count live in device memory so one graph survives restarts
+//      (pdlp.cpp:2043-2051,2145-2190);
+//   5. after every block a deterministic evaluation (2 fused SpMV + reduction
+//      kernels, no atomics) and one 80-byte device->host copy; the restart,
+//      PI primal-weight, termination and certificate logic then runs on the
+//      host exactly as in the reference (pdlp.cpp:2228-2469);
+//   6. polish phase (pdlp.cpp:2477-2676) and recovery (2678-2712).
+// MPX_PREC_MIXED runs phase 4 in FP32 (FP32 matrix, iterates and
+// accumulation) while every evaluation stays FP64, then refines in FP64.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <random>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/mipx_pdlp.h"
+#include "device_common.h"
+#include "pdlp_kernels.h"
+#include "pdlp_setup.h"
+
+namespace mpx {
+
+namespace {
+thread_local std::string g_error;
+}
+void setError(const std::string& msg) { g_error = msg; }
+const char* lastError() { return g_error.c_str(); }
+
+template <typename F>
+void withGroup(int g, F&& f) {
+    switch (g) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        case 16: f(std::integral_constant<int, 16>{}); break;
+        case 32: f(std::integral_constant<int, 32>{}); break;
+        default: f(std::integral_constant<int, 64>{}); break;
+    }
+}
+
+// Lanes per row from the mean entries per row (autotuned choice when the
+// caller leaves spmv_group at 0): about two entries per lane.
+int pickGroup(long long nnz, int rows, int forced) {
+    if (forced > 0) {
+        int g = 1;
+        while (g < forced && g < 64) g <<= 1;
+        return g;
+    }
+    if (rows <= 0) return 1;
+    const double avg = static_cast<double>(nnz) / rows;
+    int g = 1;
+    while (g < 64 && 2.0 * g < avg) g <<= 1;
+    return g;
+}
+
+inline bool isFin(double v) { return std::isfinite(v); }
+
+// ---- device arena ----------------------------------------------------------
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    void reserve(size_t bytes) {
+        cap = bytes;
+        if (bytes) MPX_HIP(hipMalloc(&base, bytes));
+    }
+    template <typename T>
+    T* take(size_t count) {
+        const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        if (used + bytes > cap) {
+            setError("internal: device arena overflow");
+            throw DeviceError();
+        }
+        T* p = reinterpret_cast<T*>(base + used);
+        used += bytes;
+        return p;
+    }
+    ~Arena() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+inline size_t padBytes(size_t count, size_t elt) { return (count * elt + 255) & ~size_t(255); }
+
+struct PinnedHost {
+    void* p = nullptr;
+    explicit PinnedHost(size_t bytes) { MPX_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault)); }
+    ~PinnedHost() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { MPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    void reset() {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+        e = nullptr;
+        g = nullptr;
+    }
+    ~Graph() { reset(); }
+};
+
+// Host CSC structure of A plus the CSR->CSC permutation: rows visited in
+// increasing order, so each column lists its rows increasing (the order of
+// the reference's buildTransposeCSR, pdlp.cpp:671-707).
+void buildTransposeStructure(int m, int n, const int* rs, const int* ci, std::vector<int>& cs,
+                             std::vector<int>& ri, std::vector<int>& perm) {
+    const int nnz = rs[m];
+    cs.assign(static_cast<size_t>(n) + 1, 0);
+    ri.resize(static_cast<size_t>(nnz));
+    perm.resize(static_cast<size_t>(nnz));
+    for (int k = 0; k < nnz; ++k) ++cs[ci[k] + 1];
+    for (int j = 0; j < n; ++j) cs[j + 1] += cs[j];
+    std::vector<int> next(cs.begin(), cs.end() - 1);
+    for (int i = 0; i < m; ++i)
+        for (int k = rs[i]; k < rs[i + 1]; ++k) {
+            const int dst = next[ci[k]]++;
+            ri[dst] = i;
+            perm[dst] = k;
+        }
+}
+
+// ---- original-space certificate checks (host) ------------------------------
+// Same tests as the reference's checkInfeasibilityCertificate /
+// checkUnboundednessCertificate (src/lp/pdlp.cpp:119-284), on the original LP.
+double l2(const std::vector<double>& v) {
+    double s = 0.0;
+    for (double e : v) s += e * e;
+    return std::sqrt(s);
+}
+
+bool farkasHolds(const mpx_lp& lp, const std::vector<double>& y, double tol) {
+    const int m = lp.num_rows, n = lp.num_cols;
+    if (m == 0) return false;
+    const double ny = l2(y);
+    if (ny < 1e-15) return false;
+    std::vector<double> r(y);
+    for (double& v : r) v /= ny;
+    for (int i = 0; i < m; ++i) {
+        const bool lo = isFin(lp.row_lower[i]), hi = isFin(lp.row_upper[i]);
+        if (hi && !lo && r[i] < -tol) return false;
+        if (lo && !hi && r[i] > tol) return false;
+    }
+    std::vector<double> atr(static_cast<size_t>(n), 0.0);
+    for (int i = 0; i < m; ++i) {
+        if (r[i] == 0.0) continue;
+        for (int k = lp.row_starts[i]; k < lp.row_starts[i + 1]; ++k)
+            atr[lp.col_indices[k]] += lp.values[k] * r[i];
+    }
+    for (int j = 0; j < n; ++j) {
+        if (isFin(lp.col_lower[j]) && atr[j] < -tol) return false;
+        if (isFin(lp.col_upper[j]) && atr[j] > tol) return false;
+    }
+    double rowpart = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double lo = lp.row_lower[i], hi = lp.row_upper[i];
+        if (lo == hi) rowpart += r[i] * lo;
+        else if (r[i] > 0.0 && isFin(hi)) rowpart += r[i] * hi;
+        else if (r[i] < 0.0 && isFin(lo)) rowpart += r[i] * lo;
+    }
+    double colpart = 0.0;
+    for (int j = 0; j < n; ++j) {
+        if (atr[j] > 0.0 && isFin(lp.col_lower[j])) colpart += atr[j] * lp.col_lower[j];
+        else if (atr[j] < 0.0 && isFin(lp.col_upper[j])) colpart += atr[j] * lp.col_upper[j];
+    }
+    return rowpart - colpart < -tol;
+}
+
+bool rayHolds(const mpx_lp& lp, const std::vector<double>& x, double tol) {
+    const int m = lp.num_rows, n = lp.num_cols;
+    if (n == 0) return false;
+    const double nx = l2(x);
+    if (nx < 1e-15) return false;
+    std::vector<double> r(x);
+    for (double& v : r) v /= nx;
+    for (int j = 0; j < n; ++j) {
+        if (isFin(lp.col_lower[j]) && r[j] < -tol) return false;
+        if (isFin(lp.col_upper[j]) && r[j] > tol) return false;
+    }
+    for (int i = 0; i < m; ++i) {
+        double ar = 0.0;
+        for (int k = lp.row_starts[i]; k < lp.row_starts[i + 1]; ++k)
+            ar += lp.values[k] * r[lp.col_indices[k]];
+        const bool lo = isFin(lp.row_lower[i]), hi = isFin(lp.row_upper[i]);
+        if (lo && hi) {
+            if (std::abs(ar) > tol) return false;
+        } else if (hi) {
+            if (ar > tol) return false;
+        } else if (lo) {
+            if (ar < -tol) return false;
+        }
+    }
+    const double sgn = lp.sense ? -1.0 : 1.0;
+    double cr = 0.0;
+    for (int j = 0; j < n; ++j) cr += lp.obj[j] * r[j];
+    return sgn * cr < -tol;
+}
+
+// ---- the solver ------------------------------------------------------------
+struct EvalOut {
+    double fpe, rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj, dx2, dy2;
+};
+
+template <typename T>
+struct IterVecs {
+    T *x = nullptr, *x0 = nullptr, *xr = nullptr, *xh = nullptr;
+    T *y = nullptr, *y0 = nullptr, *yh = nullptr;
+    T *val = nullptr, *valt = nullptr;
+    T *c = nullptr, *tau = nullptr, *l = nullptr, *u = nullptr;
+    T *sig = nullptr, *rl = nullptr, *ru = nullptr;
+};
+
+class SingleSolver {
+public:
+    SingleSolver(const mpx_lp& lp, const mpx_pdlp_options& o) : lp_(lp), o_(o) {
+        m_ = lp.num_rows;
+        n_ = lp.num_cols;
+        nnz_ = lp.row_starts[m_];
+        sgn_ = lp.sense ? -1.0 : 1.0;
+        mixed_ = o.precision == MPX_PREC_MIXED;
+        gr_ = pickGroup(nnz_, m_, o.spmv_group);
+        gc_ = pickGroup(nnz_, n_, o.spmv_group);
+    }
+
+    // Device setup: arena, uploads, scaling, power iteration.
+    void setup(const double* warm_x, const double* warm_y);
+    // Scaled problem download (mpx_pdlp_scale_problem).
+    void download(mpx_scaled_problem* out);
+    // Full solve.
+    void solve(const std::chrono::steady_clock::time_point t0, double* x_out, double* y_out,
+               mpx_pdlp_result* res);
+    // Plain iterations from a given state (mpx_pdhg_iterate).
+    void iterateFrom(const mpx_scaled_problem* sp, int iters, int inner, double step, double pw,
+                     int precision, double* cx, double* ix, double* px, double* cy, double* iy,
+                     double* py);
+
+    double step() const { return step_; }
+    int powerIters() const { return power_iters_; }
+
+private:
+    template <typename T>
+    IterVecs<T>& vecs();
+    template <typename T>
+    void launchIteration(int k, bool storeHat);
+    template <typename T>
+    void runBlock(int batch, Graph& graph, bool allowGraph);
+    template <typename T>
+    EvalOut evaluate();
+    template <typename T>
+    void restartCopy();
+    template <typename T>
+    void recover(double* x_out, double* y_out, double* objective);
+    template <typename T>
+    void downloadHats(std::vector<double>& xh, std::vector<double>& yh);
+    void pushScalars();
+    void powerIteration();
+    void convertIterates64to32();
+    void convertIterates32to64();
+
+    const mpx_lp& lp_;
+    const mpx_pdlp_options& o_;
+    int m_, n_, nnz_;
+    double sgn_;
+    bool mixed_;
+    int gr_, gc_;
+    Stream stream_;
+    Arena arena_;
+    // structure and original data
+    int *rs_ = nullptr, *ci_ = nullptr, *cs_ = nullptr, *ri_ = nullptr, *perm_ = nullptr;
+    double *obj_o_ = nullptr, *l_o_ = nullptr, *u_o_ = nullptr, *rl_o_ = nullptr, *ru_o_ = nullptr;
+    double *row_scale_ = nullptr, *col_scale_ = nullptr, *cmax_ = nullptr;
+    double* scales_ = nullptr;  // [os, cs]
+    double* sums_ = nullptr;
+    double *pcol_ = nullptr, *prow_ = nullptr, *metrics_ = nullptr, *red_ = nullptr;
+    int nbc_ = 1, nbr_ = 1, nbred_ = 1;
+    IterVecs<double> v64_;
+    IterVecs<float> v32_;
+    IterScalars* sc_dev_ = nullptr;
+    IterScalars* sc_host_ = nullptr;
+    double* metrics_host_ = nullptr;
+    std::unique_ptr<PinnedHost> pinned_;
+    double os_ = 1.0, cs_host_ = 1.0;
+    double step_ = 1.0;
+    int power_iters_ = 0;
+    double pw_ = 1.0;
+    int inner_ = 0;
+    double cnorm_ = 0.0, bnorm_ = 0.0;
+};
+
+template <>
+IterVecs<double>& SingleSolver::vecs<double>() { return v64_; }
+template <>
+IterVecs<float>& SingleSolver::vecs<float>() { return v32_; }
+
+void SingleSolver::pushScalars() {
+    sc_host_->step = step_;
+    sc_host_->primal_weight = pw_;
+    sc_host_->inner = inner_;
+    MPX_HIP(hipMemcpyAsync(sc_dev_, sc_host_, sizeof(IterScalars), hipMemcpyHostToDevice,
+                           stream_.s));
+}
+
+void SingleSolver::setup(const double* warm_x, const double* warm_y) {
+    const size_t m = static_cast<size_t>(m_), n = static_cast<size_t>(n_),
+                 nnz = static_cast<size_t>(nnz_);
+    nbc_ = std::max(1, ceilDiv(static_cast<long long>(n_) * gc_, kBlock));
+    nbr_ = std::max(1, ceilDiv(static_cast<long long>(m_) * gr_, kBlock));
+    nbred_ = std::max(1, std::min(1024, ceilDiv(std::max(n_, m_), kBlock)));
+    const int nbAt = std::max(1, ceilDiv(static_cast<long long>(n_) * gc_, kBlock));
+    // Arena size.
+    size_t bytes = 0;
+    bytes += padBytes(m + 1, 4) + 3 * padBytes(nnz, 4) + padBytes(n + 1, 4) + padBytes(nnz, 4);
+    bytes += 2 * padBytes(nnz, 8);                            // val, valt (fp64)
+    bytes += 6 * padBytes(n, 8) + 6 * padBytes(m, 8);         // originals, scales, cmax
+    bytes += 7 * padBytes(n, 8) + 6 * padBytes(m, 8);         // fp64 iterate + constants
+    bytes += padBytes(16, 8) * 4 + padBytes(2048, 8);         // scales, sums, metrics, red
+    bytes += padBytes(static_cast<size_t>(nbc_) * kNumColFields, 8);
+    bytes += padBytes(static_cast<size_t>(nbr_) * kNumRowFields, 8);
+    bytes += padBytes(static_cast<size_t>(std::max(nbAt, 2 * nbred_)), 8);
+    bytes += 3 * padBytes(std::max(n, m), 8);                 // power iteration vectors
+    bytes += padBytes(1, sizeof(IterScalars)) + padBytes(1, sizeof(PowerState));
+    if (mixed_) {
+        bytes += 2 * padBytes(nnz, 4);
+        bytes += 8 * padBytes(n, 4) + 6 * padBytes(m, 4);
+    }
+    bytes += 4096;
+    arena_.reserve(bytes);
+
+    rs_ = arena_.take<int>(m + 1);
+    ci_ = arena_.take<int>(nnz);
+    cs_ = arena_.take<int>(n + 1);
+    ri_ = arena_.take<int>(nnz);
+    perm_ = arena_.take<int>(nnz);
+    v64_.val = arena_.take<double>(nnz);
+    v64_.valt = arena_.take<double>(nnz);
+    obj_o_ = arena_.take<double>(n);
+    l_o_ = arena_.take<double>(n);
+    u_o_ = arena_.take<double>(n);
+    rl_o_ = arena_.take<double>(m);
+    ru_o_ = arena_.take<double>(m);
+    row_scale_ = arena_.take<double>(m);
+    col_scale_ = arena_.take<double>(n);
+    cmax_ = arena_.take<double>(n);
+    v64_.x = arena_.take<double>(n);
+    v64_.x0 = arena_.take<double>(n);
+    v64_.xr = arena_.take<double>(n);
+    v64_.xh = arena_.take<double>(n);
+    v64_.c = arena_.take<double>(n);
+    v64_.tau = arena_.take<double>(n);
+    v64_.l = arena_.take<double>(n);
+    v64_.u = arena_.take<double>(n);
+    v64_.y = arena_.take<double>(m);
+    v64_.y0 = arena_.take<double>(m);
+    v64_.yh = arena_.take<double>(m);
+    v64_.sig = arena_.take<double>(m);
+    v64_.rl = arena_.take<double>(m);
+    v64_.ru = arena_.take<double>(m);
+    scales_ = arena_.take<double>(16);
+    sums_ = arena_.take<double>(16);
+    metrics_ = arena_.take<double>(16);
+    red_ = arena_.take<double>(2 * 1024);
+    pcol_ = arena_.take<double>(static_cast<size_t>(nbc_) * kNumColFields);
+    prow_ = arena_.take<double>(static_cast<size_t>(nbr_) * kNumRowFields);
+    sc_dev_ = arena_.take<IterScalars>(1);
+    if (mixed_) {
+        v32_.val = arena_.take<float>(nnz);
+        v32_.valt = arena_.take<float>(nnz);
+        v32_.x = arena_.take<float>(n);
+        v32_.x0 = arena_.take<float>(n);
+        v32_.xr = arena_.take<float>(n);
+        v32_.xh = arena_.take<float>(n);
+        v32_.c = arena_.take<float>(n);
+        v32_.tau = arena_.take<float>(n);
+        v32_.l = arena_.take<float>(n);
+        v32_.u = arena_.take<float>(n);
+        v32_.y = arena_.take<float>(m);
+        v32_.y0 = arena_.take<float>(m);
+        v32_.yh = arena_.take<float>(m);
+        v32_.sig = arena_.take<float>(m);
+        v32_.rl = arena_.take<float>(m);
+        v32_.ru = arena_.take<float>(m);
+    }
+    pinned_ = std::make_unique<PinnedHost>(sizeof(IterScalars) + 32 * sizeof(double));
+    sc_host_ = static_cast<IterScalars*>(pinned_->p);
+    metrics_host_ = reinterpret_cast<double*>(static_cast<char*>(pinned_->p) + sizeof(IterScalars));
+
+    hipStream_t st = stream_.s;
+    // Host transpose structure.
+    std::vector<int> cs, ri, perm;
+    buildTransposeStructure(m_, n_, lp_.row_starts, lp_.col_indices, cs, ri, perm);
+    MPX_HIP(hipMemcpyAsync(rs_, lp_.row_starts, (m + 1) * 4, hipMemcpyHostToDevice, st));
+    if (nnz) {
+        MPX_HIP(hipMemcpyAsync(ci_, lp_.col_indices, nnz * 4, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(ri_, ri.data(), nnz * 4, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(perm_, perm.data(), nnz * 4, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(v64_.val, lp_.values, nnz * 8, hipMemcpyHostToDevice, st));
+    }
+    MPX_HIP(hipMemcpyAsync(cs_, cs.data(), (n + 1) * 4, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(obj_o_, lp_.obj, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(l_o_, lp_.col_lower, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(u_o_, lp_.col_upper, n * 8, hipMemcpyHostToDevice, st));
+    if (m) {
+        MPX_HIP(hipMemcpyAsync(rl_o_, lp_.row_lower, m * 8, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(ru_o_, lp_.row_upper, m * 8, hipMemcpyHostToDevice, st));
+    }
+    // Scales start at 1 (filled by a conversion from a host vector of ones is
+    // wasteful; a kernel would do, but hipMemcpy of a constant buffer is
+    // simplest and happens once).
+    {
+        std::vector<double> ones(std::max(n, m), 1.0);
+        MPX_HIP(hipMemcpyAsync(row_scale_, ones.data(), m * 8, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(col_scale_, ones.data(), n * 8, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(v64_.sig, ones.data(), m * 8, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(v64_.tau, ones.data(), n * 8, hipMemcpyHostToDevice, st));
+        const double one2[2] = {1.0, 1.0};
+        MPX_HIP(hipMemcpyAsync(scales_, one2, 16, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipStreamSynchronize(st));
+    }
+    const int gm = std::max(1, ceilDiv(m_, kBlock)), gn = std::max(1, ceilDiv(n_, kBlock));
+    const int gz = std::max(1, ceilDiv(nnz_, kBlock));
+    // Ruiz (pdlp.cpp:520-558).
+    if (o_.do_ruiz_scaling && m_ > 0 && n_ > 0 && nnz_ > 0) {
+        for (int it = 0; it < o_.ruiz_iterations; ++it) {
+            kRuizRows<<<gm, kBlock, 0, st>>>(m_, rs_, v64_.val, row_scale_);
+            kRuizColMax<<<gn, kBlock, 0, st>>>(n_, cs_, perm_, v64_.val, cmax_, col_scale_);
+            kRuizColApply<<<gz, kBlock, 0, st>>>(nnz_, ci_, cmax_, v64_.val);
+        }
+    }
+    kScaleColumns<<<gn, kBlock, 0, st>>>(n_, sgn_, obj_o_, col_scale_, l_o_, u_o_, v64_.c, v64_.l,
+                                          v64_.u);
+    if (m_) kScaleRows<<<gm, kBlock, 0, st>>>(m_, row_scale_, rl_o_, ru_o_, v64_.rl, v64_.ru);
+    if (nnz_) kGather<<<gz, kBlock, 0, st>>>(nnz_, perm_, v64_.val, v64_.valt);
+    // Pock-Chambolle (pdlp.cpp:581-604).
+    if (o_.do_pock_chambolle_scaling && m_ > 0 && n_ > 0) {
+        kPockChambolle<<<gm, kBlock, 0, st>>>(m_, rs_, v64_.val, v64_.sig);
+        kPockChambolle<<<gn, kBlock, 0, st>>>(n_, cs_, v64_.valt, v64_.tau);
+    }
+    // Bound / objective rescaling (pdlp.cpp:606-640).
+    if (o_.do_bound_obj_rescaling && m_ > 0 && n_ > 0) {
+        double* part = red_;
+        kRescaleNorms<<<nbred_, kBlock, 0, st>>>(n_, m_, v64_.c, v64_.rl, v64_.ru, part, nbred_);
+        kFinalize<<<1, kBlock, 0, st>>>(part, nbred_, 2, 0u, sums_);
+        kRescaleFactors<<<1, 1, 0, st>>>(sums_, scales_);
+        kApplyRescale<<<std::max(gn, gm), kBlock, 0, st>>>(n_, m_, scales_, v64_.c, v64_.l,
+                                                             v64_.u, v64_.rl, v64_.ru);
+    }
+    MPX_HIP(hipGetLastError());
+    double sc[2];
+    MPX_HIP(hipMemcpyAsync(sc, scales_, 16, hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipStreamSynchronize(st));
+    os_ = sc[0];
+    cs_host_ = sc[1];
+
+    // Termination norms of the original problem (pdlp.cpp:53-81), host.
+    if (o_.optimality_norm == 1) {
+        double cn = 0.0, bn = 0.0;
+        for (int j = 0; j < n_; ++j) cn = std::max(cn, std::abs(lp_.obj[j]));
+        for (int i = 0; i < m_; ++i) {
+            const double lo = lp_.row_lower[i], hi = lp_.row_upper[i];
+            if (isFin(hi)) bn = std::max(bn, std::abs(hi));
+            if (isFin(lo) && lo != hi) bn = std::max(bn, std::abs(lo));
+        }
+        cnorm_ = cn;
+        bnorm_ = bn;
+    } else {
+        double cq = 0.0, bq = 0.0;
+        for (int j = 0; j < n_; ++j) cq += lp_.obj[j] * lp_.obj[j];
+        for (int i = 0; i < m_; ++i) {
+            const double lo = lp_.row_lower[i], hi = lp_.row_upper[i];
+            if (isFin(hi)) bq += hi * hi;
+            if (isFin(lo) && lo != hi) bq += lo * lo;
+        }
+        cnorm_ = std::sqrt(cq);
+        bnorm_ = std::sqrt(bq);
+    }
+
+    // Iterates start at zero or at the scaled warm start.
+    MPX_HIP(hipMemsetAsync(v64_.x, 0, n * 8, st));
+    MPX_HIP(hipMemsetAsync(v64_.x0, 0, n * 8, st));
+    MPX_HIP(hipMemsetAsync(v64_.xh, 0, n * 8, st));
+    MPX_HIP(hipMemsetAsync(v64_.xr, 0, n * 8, st));
+    if (m) {
+        MPX_HIP(hipMemsetAsync(v64_.y, 0, m * 8, st));
+        MPX_HIP(hipMemsetAsync(v64_.y0, 0, m * 8, st));
+        MPX_HIP(hipMemsetAsync(v64_.yh, 0, m * 8, st));
+    }
+    if (warm_x && warm_y) {
+        double* wx = cmax_;   // n scratch
+        double* wy = prow_;   // needs m doubles: use y-hat as staging instead
+        (void)wy;
+        MPX_HIP(hipMemcpyAsync(wx, warm_x, n * 8, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(v64_.yh, warm_y, m * 8, hipMemcpyHostToDevice, st));
+        kWarmStart<<<std::max(gn, gm), kBlock, 0, st>>>(n_, m_, sgn_, wx, v64_.yh, col_scale_,
+                                                         row_scale_, scales_, v64_.x, v64_.x0,
+                                                         v64_.y, v64_.y0);
+        MPX_HIP(hipMemsetAsync(v64_.yh, 0, m * 8, st));
+    }
+    MPX_HIP(hipGetLastError());
+    pw_ = o_.primal_weight;
+    powerIteration();
+}
+
+void SingleSolver::powerIteration() {
+    hipStream_t st = stream_.s;
+    if (n_ == 0 || m_ == 0) {
+        step_ = 0.998;
+        return;
+    }
+    // Start vector exactly as the reference (pdlp.cpp:756-764).
+    std::mt19937 rng(42);
+    std::normal_distribution<double> dist(0.0, 1.0);
+    std::vector<double> xs(static_cast<size_t>(n_));
+    for (double& v : xs) v = dist(rng);
+    double xn = l2(xs);
+    if (xn < 1e-15) xn = 1.0;
+    for (double& v : xs) v /= xn;
+    // Scratch: x in v64_.xr, xn in v64_.xh, y in v64_.yh (overwritten later).
+    double* x = v64_.xr;
+    double* xnew = v64_.xh;
+    double* y = v64_.yh;
+    MPX_HIP(hipMemcpyAsync(x, xs.data(), static_cast<size_t>(n_) * 8, hipMemcpyHostToDevice, st));
+    PowerState* ps = reinterpret_cast<PowerState*>(sums_ + 8);
+    MPX_HIP(hipMemsetAsync(ps, 0, sizeof(PowerState), st));
+    double* s2 = sums_ + 4;
+    double* rn = sums_ + 5;  // rn[0], rn[1]
+    const int nbAt = nbc_;
+    double* part = pcol_;
+    const int gm = std::max(1, ceilDiv(static_cast<long long>(m_) * gr_, kBlock));
+    const int gnAt = std::max(1, ceilDiv(static_cast<long long>(n_) * gc_, kBlock));
+    const int gn = std::max(1, ceilDiv(n_, kBlock));
+    auto oneStep = [&]() {
+        withGroup(gr_, [&](auto GI) {
+            constexpr int G = decltype(GI)::value;
+            kPowerSpmvA<G><<<gm, kBlock, 0, st>>>(m_, rs_, ci_, v64_.val, x, y, ps);
+        });
+        withGroup(gc_, [&](auto GI) {
+            constexpr int G = decltype(GI)::value;
+            kPowerSpmvAt<G><<<gnAt, kBlock, 0, st>>>(n_, cs_, ri_, v64_.valt, y, x, xnew, part,
+                                                      nbAt, ps);
+        });
+        kFinalize<<<1, kBlock, 0, st>>>(part, nbAt, 1, 0u, s2);
+        kPowerResid<<<nbred_, kBlock, 0, st>>>(n_, x, xnew, s2, red_, nbred_, ps);
+        kFinalize<<<1, kBlock, 0, st>>>(red_, nbred_, 2, 0u, rn);
+        kPowerUpdate<<<1, 1, 0, st>>>(s2, rn, o_.sv_tol, o_.sv_max_iter, ps);
+        kPowerNormalize<<<gn, kBlock, 0, st>>>(n_, xnew, x, ps);
+        kPowerCheck<<<1, 1, 0, st>>>(o_.sv_tol, o_.sv_max_iter, ps);
+    };
+    // Steps run in captured groups; one host check per group.
+    const int per = 16;
+    Graph g;
+    bool useGraph = true;
+    PowerState hs{};
+    int launched = 0;
+    while (launched < o_.sv_max_iter) {
+        if (useGraph && g.e == nullptr) {
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                for (int s = 0; s < per; ++s) oneStep();
+                if (hipStreamEndCapture(st, &g.g) != hipSuccess || g.g == nullptr ||
+                    hipGraphInstantiate(&g.e, g.g, nullptr, nullptr, 0) != hipSuccess) {
+                    g.reset();
+                    useGraph = false;
+                    (void)hipGetLastError();
+                }
+            } else {
+                useGraph = false;
+                (void)hipGetLastError();
+            }
+        }
+        if (useGraph) {
+            MPX_HIP(hipGraphLaunch(g.e, st));
+        } else {
+            for (int s = 0; s < per; ++s) oneStep();
+        }
+        launched += per;
+        MPX_HIP(hipMemcpyAsync(&hs, ps, sizeof(PowerState), hipMemcpyDeviceToHost, st));
+        MPX_HIP(hipStreamSynchronize(st));
+        if (hs.done) break;
+    }
+    MPX_HIP(hipGetLastError());
+    power_iters_ = hs.iters;
+    step_ = 0.998 / std::sqrt(std::max(hs.sigma_sq, 1e-12));
+}
+
+template <typename T>
+void SingleSolver::launchIteration(int k, bool storeHat) {
+    IterVecs<T>& v = vecs<T>();
+    hipStream_t st = stream_.s;
+    const int gridC = std::max(1, ceilDiv(static_cast<long long>(n_) * gc_, kBlock));
+    const int gridR = std::max(1, ceilDiv(static_cast<long long>(m_) * gr_, kBlock));
+    withGroup(gc_, [&](auto GI) {
+        constexpr int G = decltype(GI)::value;
+        if (storeHat)
+            kPrimalStep<G, T, true><<<gridC, kBlock, 0, st>>>(n_, cs_, ri_, v.valt, v.y, v.x, v.x0,
+                                                             v.xr, v.xh, v.c, v.tau, v.l, v.u,
+                                                             sc_dev_, k);
+        else
+            kPrimalStep<G, T, false><<<gridC, kBlock, 0, st>>>(n_, cs_, ri_, v.valt, v.y, v.x,
+                                                              v.x0, v.xr, v.xh, v.c, v.tau, v.l,
+                                                              v.u, sc_dev_, k);
+    });
+    withGroup(gr_, [&](auto GI) {
+        constexpr int G = decltype(GI)::value;
+        if (storeHat)
+            kDualStep<G, T, true><<<gridR, kBlock, 0, st>>>(m_, rs_, ci_, v.val, v.xr, v.y, v.y0,
+                                                           v.yh, v.sig, v.rl, v.ru, sc_dev_, k);
+        else
+            kDualStep<G, T, false><<<gridR, kBlock, 0, st>>>(m_, rs_, ci_, v.val, v.xr, v.y,
+                                                            v.y0, v.yh, v.sig, v.rl, v.ru,
+                                                            sc_dev_, k);
+    });
+}
+
+template <typename T>
+void SingleSolver::runBlock(int batch, Graph& graph, bool allowGraph) {
+    hipStream_t st = stream_.s;
+    pushScalars();
+    const int N = o_.termination_eval_frequency;
+    if (allowGraph && batch == N) {
+        if (graph.e == nullptr) {
+            bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                for (int k = 1; k <= batch; ++k) launchIteration<T>(k, k == batch);
+                ok = hipStreamEndCapture(st, &graph.g) == hipSuccess && graph.g != nullptr &&
+                     hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0) == hipSuccess;
+            }
+            if (!ok) {
+                // Permanent fallback to direct launches (reference pdlp.cpp:2159-2168).
+                (void)hipGetLastError();
+                graph.reset();
+                for (int k = 1; k <= batch; ++k) launchIteration<T>(k, k == batch);
+                MPX_HIP(hipGetLastError());
+                return;
+            }
+        }
+        MPX_HIP(hipGraphLaunch(graph.e, st));
+        return;
+    }
+    for (int k = 1; k <= batch; ++k) launchIteration<T>(k, k == batch);
+    MPX_HIP(hipGetLastError());
+}
+
+template <typename T>
+EvalOut SingleSolver::evaluate() {
+    IterVecs<T>& v = vecs<T>();
+    hipStream_t st = stream_.s;
+    withGroup(gc_, [&](auto GI) {
+        constexpr int G = decltype(GI)::value;
+        kEvalCol<G, T><<<nbc_, kBlock, 0, st>>>(n_, cs_, ri_, v64_.valt, v.yh, v.y0, v.xh, v.x0,
+                                                v64_.c, v64_.l, v64_.u, col_scale_, os_, pcol_,
+                                                nbc_);
+    });
+    withGroup(gr_, [&](auto GI) {
+        constexpr int G = decltype(GI)::value;
+        kEvalRow<G, T><<<nbr_, kBlock, 0, st>>>(m_, rs_, ci_, v64_.val, v.xh, v.yh, v.y0, v64_.rl,
+                                                v64_.ru, row_scale_, cs_host_, prow_, nbr_);
+    });
+    kFinalize<<<1, kBlock, 0, st>>>(pcol_, nbc_, kNumColFields, 1u << kDmax, metrics_);
+    kFinalize<<<1, kBlock, 0, st>>>(prow_, nbr_, kNumRowFields, 1u << kPmax,
+                                    metrics_ + kNumColFields);
+    MPX_HIP(hipGetLastError());
+    MPX_HIP(hipMemcpyAsync(metrics_host_, metrics_, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipStreamSynchronize(st));
+    const double* mc = metrics_host_;
+    const double* mr = metrics_host_ + kNumColFields;
+    EvalOut e{};
+    e.dx2 = mc[kDx2];
+    e.dy2 = mr[kDy2];
+    const double movement = e.dx2 * pw_ + e.dy2 / pw_;
+    const double interaction = mc[kInter] * (2.0 * step_);
+    e.fpe = std::sqrt(std::max(movement + interaction, 0.0));
+    e.rel_p = std::sqrt(mr[kPsq]) / (1.0 + bnorm_);
+    e.rel_p_inf = mr[kPmax] / (1.0 + bnorm_);
+    e.rel_d = std::sqrt(mc[kDsq]) / (1.0 + cnorm_);
+    e.rel_d_inf = mc[kDmax] / (1.0 + cnorm_);
+    const double unscale = 1.0 / (os_ * cs_host_);
+    e.pobj = lp_.obj_offset + sgn_ * mc[kPobj] * unscale;
+    e.dobj = lp_.obj_offset + sgn_ * (mc[kDcol] + mr[kDrow]) * unscale;
+    e.gap = std::abs(e.pobj - e.dobj) / (1.0 + std::abs(e.pobj) + std::abs(e.dobj));
+    return e;
+}
+
+template <typename T>
+void SingleSolver::restartCopy() {
+    IterVecs<T>& v = vecs<T>();
+    hipStream_t st = stream_.s;
+    kRestartCopy<T><<<std::max(1, ceilDiv(n_, kBlock)), kBlock, 0, st>>>(n_, v.xh, v.x, v.x0);
+    if (m_) kRestartCopy<T><<<ceilDiv(m_, kBlock), kBlock, 0, st>>>(m_, v.yh, v.y, v.y0);
+    MPX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void SingleSolver::downloadHats(std::vector<double>& xh, std::vector<double>& yh) {
+    IterVecs<T>& v = vecs<T>();
+    hipStream_t st = stream_.s;
+    xh.resize(static_cast<size_t>(n_));
+    yh.resize(static_cast<size_t>(m_));
+    if constexpr (std::is_same_v<T, double>) {
+        MPX_HIP(hipMemcpyAsync(xh.data(), v.xh, xh.size() * 8, hipMemcpyDeviceToHost, st));
+        if (m_) MPX_HIP(hipMemcpyAsync(yh.data(), v.yh, yh.size() * 8, hipMemcpyDeviceToHost, st));
+        MPX_HIP(hipStreamSynchronize(st));
+    } else {
+        std::vector<float> a(xh.size()), b(yh.size());
+        MPX_HIP(hipMemcpyAsync(a.data(), v.xh, a.size() * 4, hipMemcpyDeviceToHost, st));
+        if (m_) MPX_HIP(hipMemcpyAsync(b.data(), v.yh, b.size() * 4, hipMemcpyDeviceToHost, st));
+        MPX_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < a.size(); ++i) xh[i] = a[i];
+        for (size_t i = 0; i < b.size(); ++i) yh[i] = b[i];
+    }
+}
+
+template <typename T>
+void SingleSolver::recover(double* x_out, double* y_out, double* objective) {
+    IterVecs<T>& v = vecs<T>();
+    hipStream_t st = stream_.s;
+    double* xo = cmax_;  // n-sized scratch
+    std::vector<double> hx(static_cast<size_t>(n_)), hy(static_cast<size_t>(m_));
+    kRecover<T><<<std::max(1, ceilDiv(std::max(n_, m_), kBlock)), kBlock, 0, st>>>(
+        n_, 0, sgn_, v.xh, v.yh, col_scale_, row_scale_, scales_, l_o_, u_o_, 1, xo, nullptr);
+    MPX_HIP(hipMemcpyAsync(hx.data(), xo, hx.size() * 8, hipMemcpyDeviceToHost, st));
+    std::vector<double> xh_unused, yh;
+    MPX_HIP(hipStreamSynchronize(st));
+    downloadHats<T>(xh_unused, yh);
+    std::vector<double> rsc(static_cast<size_t>(m_));
+    if (m_) {
+        MPX_HIP(hipMemcpyAsync(rsc.data(), row_scale_, rsc.size() * 8, hipMemcpyDeviceToHost, st));
+        MPX_HIP(hipStreamSynchronize(st));
+    }
+    const double inv_os = 1.0 / os_;
+    for (int i = 0; i < m_; ++i) hy[i] = sgn_ * yh[i] * rsc[i] * inv_os;
+    double objv = lp_.obj_offset;
+    for (int j = 0; j < n_; ++j) objv += lp_.obj[j] * hx[j];
+    *objective = objv;
+    if (x_out) std::memcpy(x_out, hx.data(), hx.size() * 8);
+    if (y_out) std::memcpy(y_out, hy.data(), hy.size() * 8);
+}
+
+void SingleSolver::convertIterates64to32() {
+    hipStream_t st = stream_.s;
+    auto cv = [&](float* d, const double* s, int count) {
+        if (count > 0)
+            kConvert<float, double><<<ceilDiv(count, kBlock), kBlock, 0, st>>>(count, s, d);
+    };
+    cv(v32_.x, v64_.x, n_);
+    cv(v32_.x0, v64_.x0, n_);
+    cv(v32_.xh, v64_.xh, n_);
+    cv(v32_.y, v64_.y, m_);
+    cv(v32_.y0, v64_.y0, m_);
+    cv(v32_.yh, v64_.yh, m_);
+    MPX_HIP(hipGetLastError());
+}
+
+void SingleSolver::convertIterates32to64() {
+    hipStream_t st = stream_.s;
+    auto cv = [&](double* d, const float* s, int count) {
+        if (count > 0)
+            kConvert<double, float><<<ceilDiv(count, kBlock), kBlock, 0, st>>>(count, s, d);
+    };
+    cv(v64_.xh, v32_.xh, n_);
+    cv(v64_.yh, v32_.yh, m_);
+    cv(v64_.x0, v32_.x0, n_);
+    cv(v64_.y0, v32_.y0, m_);
+    MPX_HIP(hipGetLastError());
+}
+
+void SingleSolver::download(mpx_scaled_problem* out) {
+    hipStream_t st = stream_.s;
+    const size_t m = static_cast<size_t>(m_), n = static_cast<size_t>(n_),
+                 nnz = static_cast<size_t>(nnz_);
+    auto get = [&](double* dst, const double* src, size_t count) {
+        if (dst && count) MPX_HIP(hipMemcpyAsync(dst, src, count * 8, hipMemcpyDeviceToHost, st));
+    };
+    auto geti = [&](int32_t* dst, const int* src, size_t count) {
+        if (dst && count) MPX_HIP(hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToHost, st));
+    };
+    get(out->values, v64_.val, nnz);
+    get(out->at_values, v64_.valt, nnz);
+    geti(out->at_row_starts, cs_, n + 1);
+    geti(out->at_col_indices, ri_, nnz);
+    get(out->cscaled, v64_.c, n);
+    get(out->col_lower, v64_.l, n);
+    get(out->col_upper, v64_.u, n);
+    get(out->row_lower, v64_.rl, m);
+    get(out->row_upper, v64_.ru, m);
+    get(out->row_scale, row_scale_, m);
+    get(out->col_scale, col_scale_, n);
+    get(out->sigma_base, v64_.sig, m);
+    get(out->tau_base, v64_.tau, n);
+    MPX_HIP(hipStreamSynchronize(st));
+    out->objective_scale = os_;
+    out->constraint_scale = cs_host_;
+    out->spectral_norm = 0.998 / step_;
+    out->power_iterations = power_iters_;
+}
+
+// Converts the fp64 scaled problem into the fp32 iteration copies.
+template <typename T>
+void convertProblem(hipStream_t st, IterVecs<T>& d, const IterVecs<double>& s, int m, int n,
+                    int nnz) {
+    auto cv = [&](T* dst, const double* src, long long count) {
+        if (count > 0)
+            kConvert<T, double><<<ceilDiv(count, kBlock), kBlock, 0, st>>>(count, src, dst);
+    };
+    cv(d.val, s.val, nnz);
+    cv(d.valt, s.valt, nnz);
+    cv(d.c, s.c, n);
+    cv(d.tau, s.tau, n);
+    cv(d.l, s.l, n);
+    cv(d.u, s.u, n);
+    cv(d.sig, s.sig, m);
+    cv(d.rl, s.rl, m);
+    cv(d.ru, s.ru, m);
+}
+
+void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double* x_out,
+                         double* y_out, mpx_pdlp_result* res) {
+    const mpx_pdlp_options& o = o_;
+    const int N = o.termination_eval_frequency;
+    auto timeUp = [&]() {
+        if (o.max_solve_seconds < 0.0) return false;
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >=
+               o.max_solve_seconds;
+    };
+    auto stopRequested = [&]() { return o.stop_fn != nullptr && o.stop_fn(o.stop_user) != 0; };
+    const double work_per_iter =
+        4.0 * static_cast<double>(nnz_) + 2.0 * static_cast<double>(m_ + n_);
+    res->step = step_;
+
+    int total = 0, restarts = 0, fp32_iters = 0;
+    double fpe0 = -1.0, last_fpe = std::numeric_limits<double>::infinity();
+    double pw_err_sum = 0.0, pw_last_err = 0.0, best_pw = pw_, best_score =
+                                                                 std::numeric_limits<double>::infinity();
+    double precond_pw = o.primal_weight;
+    bool converged = false;
+    EvalOut e{};
+    inner_ = 0;
+    int status = MPX_STATUS_ITER_LIMIT;
+    bool done = false;
+    Graph g64, g32;
+    // Mixed precision: FP32 blocks until the FP64-evaluated KKT reaches the
+    // target or stops improving, then FP64 blocks from the FP32 iterate.
+    bool fp32_phase = mixed_;
+    int stall = 0;
+    double best_kkt = std::numeric_limits<double>::infinity();
+    if (fp32_phase) {
+        convertProblem<float>(stream_.s, v32_, v64_, m_, n_, nnz_);
+        convertIterates64to32();
+    }
+    auto kktOf = [&](const EvalOut& ev) {
+        const double tp = o.optimality_norm == 1 ? ev.rel_p_inf : ev.rel_p;
+        const double td = o.optimality_norm == 1 ? ev.rel_d_inf : ev.rel_d;
+        return std::max(std::max(tp / o.primal_tol, td / o.dual_tol), ev.gap / o.optimality_tol);
+    };
+    auto restartNow = [&](bool polish) {
+        if (fp32_phase) restartCopy<float>();
+        else restartCopy<double>();
+        (void)polish;
+        inner_ = 0;
+        fpe0 = -1.0;
+        last_fpe = std::numeric_limits<double>::infinity();
+    };
+    auto updatePw = [&](const EvalOut& ev, bool polish) {
+        const double pd = std::sqrt(ev.dx2), dd = std::sqrt(ev.dy2);
+        const double ratio = (ev.rel_p > 1e-15) ? ev.rel_d / ev.rel_p : 0.0;
+        if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+            const double err = std::log(dd) - std::log(pd) - std::log(pw_);
+            pw_err_sum = o.pid_i_smooth * pw_err_sum + err;
+            const double derr = err - pw_last_err;
+            pw_ *= std::exp(o.pid_kp * err + o.pid_ki * pw_err_sum + o.pid_kd * derr);
+            pw_ = clampRef(pw_, 1e-4, 1e8);
+            pw_last_err = err;
+            if (!polish) {
+                const double score = std::abs(std::log10(std::max(ev.rel_d, 1e-15)) -
+                                              std::log10(std::max(ev.rel_p, 1e-15)));
+                if (score < best_score) {
+                    best_score = score;
+                    best_pw = pw_;
+                }
+            }
+        } else if (!polish) {
+            pw_ = best_pw;
+            pw_err_sum = 0.0;
+            pw_last_err = 0.0;
+        }
+    };
+
+    while (total < o.max_iter) {
+        if (timeUp()) {
+            status = MPX_STATUS_TIME_LIMIT;
+            done = true;
+            break;
+        }
+        if (stopRequested()) {
+            status = MPX_STATUS_ITER_LIMIT;
+            done = true;
+            break;
+        }
+        const int batch = std::min(N, o.max_iter - total);
+        if (fp32_phase) runBlock<float>(batch, g32, true);
+        else runBlock<double>(batch, g64, true);
+        inner_ += batch;
+        total += batch;
+        if (fp32_phase) fp32_iters += batch;
+        e = fp32_phase ? evaluate<float>() : evaluate<double>();
+        if (fpe0 < 0.0) fpe0 = e.fpe;
+        const double tp = o.optimality_norm == 1 ? e.rel_p_inf : e.rel_p;
+        const double td = o.optimality_norm == 1 ? e.rel_d_inf : e.rel_d;
+        if (o.verbose && (total <= N || total % (10 * N) == 0))
+            std::printf("PDLP %7d  pobj=% .8e  pinf=% .2e  dinf=% .2e  gap=% .2e  fpe=% .2e  "
+                        "pw=% .2e  step=% .2e  [MI355X%s]\n",
+                        total, e.pobj, tp, td, e.gap, e.fpe, pw_, step_, fp32_phase ? " fp32" : "");
+        if (tp <= o.primal_tol && td <= o.dual_tol && e.gap <= o.optimality_tol) {
+            if (fp32_phase) {
+                // Converged on FP32 iterates judged in FP64: hand over to FP64.
+                convertIterates32to64();
+                fp32_phase = false;
+            }
+            converged = true;
+            break;
+        }
+        if (fp32_phase) {
+            const double k = kktOf(e);
+            if (k < 0.5 * best_kkt) {
+                best_kkt = k;
+                stall = 0;
+            } else if (++stall >= 10) {
+                // FP32 stagnated: refine in FP64 from the FP32 point
+                // (restart anchor = current point).
+                convertIterates32to64();
+                fp32_phase = false;
+                restartNow(false);
+                ++restarts;
+                continue;
+            }
+        }
+        // Certificates every 10 N (pdlp.cpp:2309-2341), on the host.
+        if (total >= 10 * N && total % (10 * N) == 0) {
+            std::vector<double> xh, yh;
+            if (fp32_phase) downloadHats<float>(xh, yh);
+            else downloadHats<double>(xh, yh);
+            std::vector<double> rsc(static_cast<size_t>(m_)), csc(static_cast<size_t>(n_));
+            MPX_HIP(hipMemcpy(rsc.data(), row_scale_, rsc.size() * 8, hipMemcpyDeviceToHost));
+            MPX_HIP(hipMemcpy(csc.data(), col_scale_, csc.size() * 8, hipMemcpyDeviceToHost));
+            std::vector<double> yo(static_cast<size_t>(m_)), xo(static_cast<size_t>(n_));
+            const double inv_os = 1.0 / os_;
+            for (int i = 0; i < m_; ++i) yo[i] = sgn_ * yh[i] * rsc[i] * inv_os;
+            if (farkasHolds(lp_, yo, 1e-6)) {
+                status = MPX_STATUS_INFEASIBLE;
+                if (y_out) std::memcpy(y_out, yo.data(), yo.size() * 8);
+                if (x_out) std::memset(x_out, 0, static_cast<size_t>(n_) * 8);
+                done = true;
+                break;
+            }
+            const double inv_cs = 1.0 / cs_host_;
+            for (int j = 0; j < n_; ++j) xo[j] = xh[j] * csc[j] * inv_cs;
+            if (rayHolds(lp_, xo, 1e-6)) {
+                status = MPX_STATUS_UNBOUNDED;
+                if (x_out) std::memcpy(x_out, xo.data(), xo.size() * 8);
+                if (y_out) std::memset(y_out, 0, static_cast<size_t>(m_) * 8);
+                done = true;
+                break;
+            }
+        }
+        bool restart;
+        if (total == N) restart = true;
+        else if (fpe0 > 0.0 && e.fpe <= o.restart_sufficient_decay * fpe0) restart = true;
+        else if (fpe0 > 0.0 && e.fpe <= o.restart_necessary_decay * fpe0 && e.fpe > last_fpe)
+            restart = true;
+        else
+            restart = inner_ >= static_cast<int>(o.restart_artificial_fraction *
+                                                  static_cast<double>(total));
+        last_fpe = e.fpe;
+        if (restart) {
+            ++restarts;
+            if (o.update_primal_weight) updatePw(e, false);
+            if (o.preconditioner_refresh && precond_pw > 0.0) {
+                // The refresh recomputes sigma/tau from the unchanged scaled
+                // matrix (pdlp.cpp:713-742): only the recorded weight moves.
+                const double drift = pw_ / precond_pw;
+                if ((drift > o.preconditioner_refresh_ratio ||
+                     drift < 1.0 / o.preconditioner_refresh_ratio) &&
+                    o.do_pock_chambolle_scaling)
+                    precond_pw = pw_;
+            }
+            restartNow(false);
+        }
+    }
+
+    if (!done && converged && o.enable_polishing && !timeUp()) {
+        const double rp = o.optimality_norm == 1 ? e.rel_p_inf : e.rel_p;
+        const double rd = o.optimality_norm == 1 ? e.rel_d_inf : e.rel_d;
+        if (rp > o.polish_primal_tol || rd > o.polish_dual_tol) {
+            restartNow(true);
+            int pcount = 0;
+            while (pcount < o.polish_max_iter) {
+                if (timeUp() || stopRequested()) break;
+                const int batch = std::min(N, o.polish_max_iter - pcount);
+                runBlock<double>(batch, g64, false);
+                inner_ += batch;
+                pcount += batch;
+                total += batch;
+                e = evaluate<double>();
+                if (fpe0 < 0.0) fpe0 = e.fpe;
+                const double rpn = o.optimality_norm == 1 ? e.rel_p_inf : e.rel_p;
+                const double rdn = o.optimality_norm == 1 ? e.rel_d_inf : e.rel_d;
+                if (o.verbose && pcount % (10 * N) == 0)
+                    std::printf("PDLP polish %7d  pinf=% .2e  dinf=% .2e  fpe=% .2e  [MI355X]\n",
+                                pcount, rpn, rdn, e.fpe);
+                if (rpn <= o.polish_primal_tol && rdn <= o.polish_dual_tol) break;
+                bool restart;
+                if (pcount == batch) restart = true;
+                else if (fpe0 > 0.0 && e.fpe <= o.restart_sufficient_decay * fpe0) restart = true;
+                else if (fpe0 > 0.0 && e.fpe <= o.restart_necessary_decay * fpe0 &&
+                         e.fpe > last_fpe)
+                    restart = true;
+                else
+                    restart = inner_ >= static_cast<int>(o.restart_artificial_fraction *
+                                                          static_cast<double>(pcount));
+                last_fpe = e.fpe;
+                if (restart) {
+                    ++restarts;
+                    if (o.update_primal_weight) updatePw(e, true);
+                    restartNow(true);
+                }
+            }
+        }
+    }
+
+    double objv = 0.0;
+    if (!done) {
+        if (converged) {
+            recover<double>(x_out, y_out, &objv);
+            status = MPX_STATUS_OPTIMAL;
+        } else {
+            status = timeUp() ? MPX_STATUS_TIME_LIMIT : MPX_STATUS_ITER_LIMIT;
+        }
+    }
+    res->status = status;
+    res->iterations = total;
+    res->objective = status == MPX_STATUS_OPTIMAL ? objv : 0.0;
+    res->work_units = static_cast<double>(total) * work_per_iter;
+    res->rel_primal = o.optimality_norm == 1 ? e.rel_p_inf : e.rel_p;
+    res->rel_dual = o.optimality_norm == 1 ? e.rel_d_inf : e.rel_d;
+    res->rel_gap = e.gap;
+    res->primal_obj = e.pobj;
+    res->dual_obj = e.dobj;
+    res->primal_weight = pw_;
+    res->restarts = restarts;
+    res->iterations_fp32 = fp32_iters;
+}
+
+void SingleSolver::iterateFrom(const mpx_scaled_problem* sp, int iters, int inner, double step,
+                               double pw, int precision, double* cx, double* ix, double* px,
+                               double* cy, double* iy, double* py) {
+    hipStream_t st = stream_.s;
+    const size_t m = static_cast<size_t>(m_), n = static_cast<size_t>(n_),
+                 nnz = static_cast<size_t>(nnz_);
+    // Overwrite the scaled data with the caller's.
+    MPX_HIP(hipMemcpyAsync(v64_.val, sp->values, nnz * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.valt, sp->at_values, nnz * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.c, sp->cscaled, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.l, sp->col_lower, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.u, sp->col_upper, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.rl, sp->row_lower, m * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.ru, sp->row_upper, m * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.sig, sp->sigma_base, m * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.tau, sp->tau_base, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.x, cx, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.x0, ix, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.xh, px, n * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.y, cy, m * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.y0, iy, m * 8, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(v64_.yh, py, m * 8, hipMemcpyHostToDevice, st));
+    step_ = step;
+    pw_ = pw;
+    inner_ = inner;
+    Graph none;
+    if (precision == MPX_PREC_MIXED) {
+        if (!mixed_) {
+            setError("mpx_pdhg_iterate: fp32 buffers not allocated");
+            throw DeviceError();
+        }
+        convertProblem<float>(st, v32_, v64_, m_, n_, nnz_);
+        auto cv = [&](float* d, const double* s, int count) {
+            if (count > 0)
+                kConvert<float, double><<<ceilDiv(count, kBlock), kBlock, 0, st>>>(count, s, d);
+        };
+        cv(v32_.x, v64_.x, n_);
+        cv(v32_.x0, v64_.x0, n_);
+        cv(v32_.xh, v64_.xh, n_);
+        cv(v32_.y, v64_.y, m_);
+        cv(v32_.y0, v64_.y0, m_);
+        cv(v32_.yh, v64_.yh, m_);
+        runBlock<float>(iters, none, false);
+        auto cb = [&](double* d, const float* s, int count) {
+            if (count > 0)
+                kConvert<double, float><<<ceilDiv(count, kBlock), kBlock, 0, st>>>(count, s, d);
+        };
+        cb(v64_.x, v32_.x, n_);
+        cb(v64_.xh, v32_.xh, n_);
+        cb(v64_.y, v32_.y, m_);
+        cb(v64_.yh, v32_.yh, m_);
+    } else {
+        runBlock<double>(iters, none, false);
+    }
+    MPX_HIP(hipMemcpyAsync(cx, v64_.x, n * 8, hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipMemcpyAsync(px, v64_.xh, n * 8, hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipMemcpyAsync(cy, v64_.y, m * 8, hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipMemcpyAsync(py, v64_.yh, m * 8, hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipStreamSynchronize(st));
+}
+
+bool validLp(const mpx_lp* lp) {
+    if (!lp) return false;
+    if (lp->num_rows < 0 || lp->num_cols < 0) return false;
+    if (!lp->row_starts && lp->num_rows > 0) return false;
+    if (lp->num_cols > 0 && (!lp->obj || !lp->col_lower || !lp->col_upper)) return false;
+    if (lp->num_rows > 0 && (!lp->row_lower || !lp->row_upper)) return false;
+    if (lp->num_rows > 0) {
+        if (lp->row_starts[0] != 0) return false;
+        const int nnz = lp->row_starts[lp->num_rows];
+        if (nnz < 0 || (nnz > 0 && (!lp->col_indices || !lp->values))) return false;
+        for (int i = 0; i < lp->num_rows; ++i)
+            if (lp->row_starts[i + 1] < lp->row_starts[i]) return false;
+        for (int k = 0; k < nnz; ++k)
+            if (lp->col_indices[k] < 0 || lp->col_indices[k] >= lp->num_cols) return false;
+    }
+    return true;
+}
+
+}  // namespace mpx
+
+using namespace mpx;
+
+extern "C" {
+
+void mpx_pdlp_default_options(mpx_pdlp_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->max_iter = 4000000;
+    o->optimality_tol = 5e-5;
+    o->primal_tol = 5e-5;
+    o->dual_tol = 5e-5;
+    o->max_solve_seconds = -1.0;
+    o->optimality_norm = 0;
+    o->sv_max_iter = 5000;
+    o->sv_tol = 1e-4;
+    o->adaptive_step = 0;
+    o->adaptive_step_down = 0.7;
+    o->adaptive_step_up = 1.0;
+    o->adaptive_step_min = 1e-6;
+    o->adaptive_step_max = 1e3;
+    o->primal_weight = 1.0;
+    o->update_primal_weight = 1;
+    o->pid_kp = 0.99;
+    o->pid_ki = 0.01;
+    o->pid_kd = 0.0;
+    o->pid_i_smooth = 0.3;
+    o->restart_sufficient_decay = 0.2;
+    o->restart_necessary_decay = 0.5;
+    o->restart_artificial_fraction = 0.36;
+    o->termination_eval_frequency = 200;
+    o->matrix_zero_tol = 0.0;
+    o->enable_polishing = 1;
+    o->polish_primal_tol = 1e-6;
+    o->polish_dual_tol = 1e-6;
+    o->polish_max_iter = 500000;
+    o->do_ruiz_scaling = 1;
+    o->ruiz_iterations = 10;
+    o->do_pock_chambolle_scaling = 1;
+    o->do_bound_obj_rescaling = 1;
+    o->preconditioner_refresh = 0;
+    o->preconditioner_refresh_ratio = 10.0;
+    o->anderson_acceleration = 0;
+    o->anderson_window_size = 5;
+    o->anderson_regularization = 1e-10;
+    o->use_gpu = 1;
+    o->gpu_min_rows = 512;
+    o->gpu_min_nnz = 10000;
+    o->verbose = 1;
+    o->stop_fn = nullptr;
+    o->stop_user = nullptr;
+    o->precision = MPX_PREC_FP64;
+    o->refine_tol = -1.0;
+    o->spmv_group = 0;
+    o->device = 0;
+}
+
+const char* mpx_last_error(void) { return mpx::lastError(); }
+const char* mpx_version(void) { return "mipx_amd 0.1 (gfx950)"; }
+
+int mpx_device_count(int32_t* count) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    if (count) *count = c;
+    return MPX_OK;
+}
+
+int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double* warm_x,
+                   const double* warm_y, double* x_out, double* y_out, mpx_pdlp_result* res) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!validLp(lp) || !opts || !res) {
+        setError("mpx_pdlp_solve: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    if (opts->termination_eval_frequency <= 0) {
+        setError("mpx_pdlp_solve: termination_eval_frequency must be positive");
+        return MPX_ERR_INVALID;
+    }
+    if (opts->adaptive_step || (opts->anderson_acceleration && opts->anderson_window_size >= 2) ||
+        opts->matrix_zero_tol > 0.0) {
+        setError("mpx_pdlp_solve: adaptive_step, anderson_acceleration and matrix_zero_tol are "
+                 "not supported by this library");
+        return MPX_ERR_UNSUPPORTED;
+    }
+    std::memset(res, 0, sizeof(*res));
+    const int m = lp->num_rows, n = lp->num_cols;
+    const double sgn = lp->sense ? -1.0 : 1.0;
+    auto finish = [&]() {
+        res->seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return MPX_OK;
+    };
+    // Closed-form trivial problems (pdlp.cpp:812-860): no device work needed.
+    if (n == 0) {
+        res->status = MPX_STATUS_OPTIMAL;
+        res->objective = lp->obj_offset;
+        if (y_out) std::memset(y_out, 0, sizeof(double) * static_cast<size_t>(m));
+        return finish();
+    }
+    if (m == 0) {
+        // Without rows every scale is 1 (Ruiz and rescaling need m > 0).
+        if (x_out) std::memset(x_out, 0, sizeof(double) * static_cast<size_t>(n));
+        double objv = lp->obj_offset;
+        for (int j = 0; j < n; ++j) {
+            const double c = sgn * lp->obj[j], lb = lp->col_lower[j], ub = lp->col_upper[j];
+            double xj;
+            if (c > 0.0) {
+                if (!isFin(lb)) {
+                    res->status = MPX_STATUS_UNBOUNDED;
+                    return finish();
+                }
+                xj = lb;
+            } else if (c < 0.0) {
+                if (!isFin(ub)) {
+                    res->status = MPX_STATUS_UNBOUNDED;
+                    return finish();
+                }
+                xj = ub;
+            } else {
+                xj = isFin(lb) ? lb : (isFin(ub) ? ub : 0.0);
+            }
+            if (x_out) x_out[j] = xj;
+            objv += lp->obj[j] * xj;
+        }
+        res->status = MPX_STATUS_OPTIMAL;
+        res->objective = objv;
+        return finish();
+    }
+    try {
+        MPX_HIP(hipSetDevice(opts->device));
+        SingleSolver s(*lp, *opts);
+        s.setup(warm_x, warm_y);
+        res->setup_seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        s.solve(t0, x_out, y_out, res);
+    } catch (const DeviceError&) {
+        res->status = MPX_STATUS_ERROR;
+        res->objective = 0.0;
+    } catch (const std::bad_alloc&) {
+        setError("host out of memory");
+        res->status = MPX_STATUS_ERROR;
+    }
+    return finish();
+}
+
+int mpx_pdlp_scale_problem(const mpx_lp* lp, const mpx_pdlp_options* opts,
+                           mpx_scaled_problem* out) {
+    if (!validLp(lp) || !opts || !out || lp->num_rows == 0 || lp->num_cols == 0) {
+        setError("mpx_pdlp_scale_problem: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    try {
+        MPX_HIP(hipSetDevice(opts->device));
+        SingleSolver s(*lp, *opts);
+        s.setup(nullptr, nullptr);
+        s.download(out);
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
+int mpx_pdhg_iterate(const mpx_lp* lp, const mpx_scaled_problem* sp, int32_t iters,
+                     int32_t inner_count, double step, double primal_weight, int32_t precision,
+                     int32_t group, double* current_x, double* initial_x, double* pdhg_x,
+                     double* current_y, double* initial_y, double* pdhg_y) {
+    if (!validLp(lp) || !sp || iters < 0 || lp->num_rows == 0 || lp->num_cols == 0) {
+        setError("mpx_pdhg_iterate: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    try {
+        mpx_pdlp_options o;
+        mpx_pdlp_default_options(&o);
+        o.spmv_group = group;
+        o.precision = precision;
+        o.do_ruiz_scaling = 0;
+        o.do_pock_chambolle_scaling = 0;
+        o.do_bound_obj_rescaling = 0;
+        o.sv_max_iter = 1;
+        o.termination_eval_frequency = std::max(1, iters);
+        MPX_HIP(hipSetDevice(0));
+        SingleSolver s(*lp, o);
+        s.setup(nullptr, nullptr);
+        s.iterateFrom(sp, iters, inner_count, step, primal_weight, precision, current_x, initial_x,
+                      pdhg_x, current_y, initial_y, pdhg_y);
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static int spmvImpl(int32_t rows, const int32_t* rs, const int32_t* ci, const T* val, const T* x,
+                    T* y, int32_t group, void* stream) {
+    if (rows < 0 || !rs || !y) return MPX_ERR_INVALID;
+    if (rows == 0) return MPX_OK;
+    const int g = group > 0 ? pickGroup(0, rows, group) : 1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int grid = std::max(1, ceilDiv(static_cast<long long>(rows) * g, kBlock));
+    withGroup(g, [&](auto GI) {
+        constexpr int G = decltype(GI)::value;
+        kSpmv<G, T><<<grid, kBlock, 0, st>>>(rows, rs, ci, val, x, y);
+    });
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        setError(std::string("spmv launch: ") + hipGetErrorString(e));
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
+extern "C" {
+
+int mpx_spmv_csr_f64(int32_t rows, int32_t cols, const int32_t* rs, const int32_t* ci,
+                     const double* val, const double* x, double* y, int32_t group, void* stream) {
+    (void)cols;
+    return spmvImpl<double>(rows, rs, ci, val, x, y, group, stream);
+}
+
+int mpx_spmv_csr_f32(int32_t rows, int32_t cols, const int32_t* rs, const int32_t* ci,
+                     const float* val, const float* x, float* y, int32_t group, void* stream) {
+    (void)cols;
+    return spmvImpl<float>(rows, rs, ci, val, x, y, group, stream);
+}
+
+}  // extern "C"

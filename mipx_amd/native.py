@@ -1,0 +1,161 @@
+"""Python access to libmipx_pdlp.so through its C-ABI (include/mipx_pdlp.h).
+
+This is plumbing for tests, smoke and bench: every call goes straight to the
+HIP library; there is no CPU fallback.  A missing library raises ImportError.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._abi import (MpxLp, MpxPdlpOptions, MpxPdlpResult, MpxScaledProblem, alloc_scaled,
+                   make_options, scaled_from_arrays, c_double_p)
+from ._loader import load_library
+
+_lib = None
+
+EXPORTED = [
+    "mpx_pdlp_default_options", "mpx_last_error", "mpx_version", "mpx_device_count",
+    "mpx_pdlp_solve", "mpx_pdlp_solve_batch_shared", "mpx_pdlp_solve_batch",
+    "mpx_server_create", "mpx_server_submit", "mpx_server_step", "mpx_server_result",
+    "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
+    "mpx_pdhg_iterate",
+]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    L = load_library("libmipx_pdlp.so")
+    L.mpx_last_error.restype = C.c_char_p
+    L.mpx_version.restype = C.c_char_p
+    L.mpx_pdlp_default_options.argtypes = [C.POINTER(MpxPdlpOptions)]
+    L.mpx_device_count.argtypes = [C.POINTER(C.c_int32)]
+    L.mpx_pdlp_solve.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions), C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
+    L.mpx_pdlp_solve_batch_shared.argtypes = [C.POINTER(MpxLp), C.c_int32, C.c_void_p,
+                                              C.c_void_p, C.POINTER(MpxPdlpOptions),
+                                              C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
+    L.mpx_pdlp_solve_batch.argtypes = [C.POINTER(MpxLp), C.c_int32, C.POINTER(MpxPdlpOptions),
+                                       C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
+    L.mpx_server_create.argtypes = [C.POINTER(MpxPdlpOptions), C.c_double, C.c_int32,
+                                    C.POINTER(C.c_void_p)]
+    L.mpx_server_submit.argtypes = [C.c_void_p, C.POINTER(MpxLp), C.POINTER(C.c_int64)]
+    L.mpx_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    L.mpx_server_result.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.c_void_p,
+                                    C.c_void_p, C.POINTER(MpxPdlpResult)]
+    L.mpx_server_destroy.argtypes = [C.c_void_p]
+    for nm in ("mpx_spmv_csr_f64", "mpx_spmv_csr_f32"):
+        getattr(L, nm).argtypes = [C.c_int32, C.c_int32] + [C.c_void_p] * 5 + [C.c_int32,
+                                                                            C.c_void_p]
+    L.mpx_pdlp_scale_problem.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions),
+                                         C.POINTER(MpxScaledProblem)]
+    L.mpx_pdhg_iterate.argtypes = ([C.POINTER(MpxLp), C.POINTER(MpxScaledProblem), C.c_int32,
+                                    C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32]
+                                   + [C.c_void_p] * 6)
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().mpx_last_error().decode()
+
+
+def device_count():
+    c = C.c_int32(0)
+    lib().mpx_device_count(C.byref(c))
+    return c.value
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed (code {rc}): {last_error()}")
+
+
+def solve(lp, warm=None, **opts):
+    """One LP on the GPU.  Returns (result dict, x, y) in original space."""
+    o = opts.pop("_options", None) or make_options(**opts)
+    clp = lp.as_c()
+    x = np.zeros(lp.num_cols)
+    y = np.zeros(lp.num_rows)
+    res = MpxPdlpResult()
+    wx = wy = None
+    if warm is not None:
+        wx = np.ascontiguousarray(warm[0], dtype=np.float64)
+        wy = np.ascontiguousarray(warm[1], dtype=np.float64)
+    rc = lib().mpx_pdlp_solve(C.byref(clp), C.byref(o),
+                              None if wx is None else wx.ctypes.data,
+                              None if wy is None else wy.ctypes.data,
+                              x.ctypes.data, y.ctypes.data, C.byref(res))
+    _check(rc, "mpx_pdlp_solve")
+    d = res.as_dict()
+    if d["status"] == 6:
+        d["error"] = last_error()
+    return d, x, y
+
+
+def solve_batch_shared(lp, col_lower=None, col_upper=None, **opts):
+    """count = rows of col_lower; returns (list of result dicts, X[count,n], Y[count,m])."""
+    o = opts.pop("_options", None) or make_options(**opts)
+    count = opts.pop("count", None)
+    if col_lower is not None:
+        col_lower = np.ascontiguousarray(col_lower, dtype=np.float64)
+        col_upper = np.ascontiguousarray(col_upper, dtype=np.float64)
+        count = col_lower.shape[0]
+    clp = lp.as_c()
+    X = np.zeros((count, lp.num_cols))
+    Y = np.zeros((count, lp.num_rows))
+    res = (MpxPdlpResult * count)()
+    rc = lib().mpx_pdlp_solve_batch_shared(
+        C.byref(clp), count, None if col_lower is None else col_lower.ctypes.data,
+        None if col_upper is None else col_upper.ctypes.data, C.byref(o), X.ctypes.data,
+        Y.ctypes.data, res)
+    _check(rc, "mpx_pdlp_solve_batch_shared")
+    return [r.as_dict() for r in res], X, Y
+
+
+def solve_batch(lps, **opts):
+    o = opts.pop("_options", None) or make_options(**opts)
+    count = len(lps)
+    arr = (MpxLp * count)(*[lp.as_c() for lp in lps])
+    xs = [np.zeros(lp.num_cols) for lp in lps]
+    ys = [np.zeros(lp.num_rows) for lp in lps]
+    xp = (C.c_void_p * count)(*[x.ctypes.data for x in xs])
+    yp = (C.c_void_p * count)(*[y.ctypes.data for y in ys])
+    res = (MpxPdlpResult * count)()
+    rc = lib().mpx_pdlp_solve_batch(arr, count, C.byref(o), C.cast(xp, C.c_void_p),
+                                    C.cast(yp, C.c_void_p), res)
+    _check(rc, "mpx_pdlp_solve_batch")
+    return [r.as_dict() for r in res], xs, ys
+
+
+def scale_problem(lp, **opts):
+    o = make_options(**opts)
+    sp, arrs = alloc_scaled(lp.num_rows, lp.num_cols, lp.nnz)
+    clp = lp.as_c()
+    _check(lib().mpx_pdlp_scale_problem(C.byref(clp), C.byref(o), C.byref(sp)),
+           "mpx_pdlp_scale_problem")
+    arrs["objective_scale"] = sp.objective_scale
+    arrs["constraint_scale"] = sp.constraint_scale
+    arrs["spectral_norm"] = sp.spectral_norm
+    arrs["power_iterations"] = sp.power_iterations
+    return arrs
+
+
+def pdhg_iterate(lp, scaled, iters, inner, step, pw, state, precision=0, group=1):
+    sp = scaled_from_arrays(scaled)
+    clp = lp.as_c()
+    _check(lib().mpx_pdhg_iterate(C.byref(clp), C.byref(sp), iters, inner, step, pw, precision,
+                                  group, *[state[k].ctypes.data for k in
+                                           ("cx", "ix", "px", "cy", "iy", "py")]),
+           "mpx_pdhg_iterate")
+    return state
+
+
+def spmv_device(rows, cols, rs, ci, val, x, y, group=0, stream=None):
+    """y = A x on device tensors (torch tensors or raw pointers)."""
+    ptr = lambda t: t if isinstance(t, int) else t.data_ptr()
+    f = lib().mpx_spmv_csr_f64 if str(getattr(val, "dtype", "")) == "torch.float64" else \
+        lib().mpx_spmv_csr_f32
+    _check(f(rows, cols, ptr(rs), ptr(ci), ptr(val), ptr(x), ptr(y), group, stream),
+           "mpx_spmv_csr")

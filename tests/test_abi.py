@@ -1,0 +1,45 @@
+"""CPU checks of the C-ABI library: it loads without a GPU and exports every
+function include/mipx_pdlp.h declares; the ctypes structs match the header."""
+import ctypes as C
+import os
+import re
+
+from mipx_amd import _abi, native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "mipx_pdlp.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpx_[a-z0-9_]+)\s*\(", text)) - {"mpx_stop_fn"})
+
+
+def test_exports_every_declared_function():
+    lib = native.lib()
+    names = header_functions()
+    assert len(names) >= 16
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    assert set(names) <= set(native.EXPORTED)
+
+
+def test_default_options_match_reference_defaults():
+    o = _abi.MpxPdlpOptions()
+    native.lib().mpx_pdlp_default_options(C.byref(o))
+    for k, v in _abi.DEFAULT_OPTIONS.items():
+        assert getattr(o, k) == v, k
+
+
+def test_struct_sizes():
+    # Field order mirrors the header; sizes follow the x86-64 C ABI.
+    assert C.sizeof(_abi.MpxLp) == 4 * 4 + 8 * 8 + 8
+    assert C.sizeof(_abi.MpxPdlpResult) == 2 * 4 + 9 * 8 + 2 * 4 + 2 * 8
+
+
+def test_invalid_arguments_rejected_without_gpu():
+    lib = native.lib()
+    res = _abi.MpxPdlpResult()
+    o = _abi.make_options()
+    assert lib.mpx_pdlp_solve(None, C.byref(o), None, None, None, None, C.byref(res)) == 1
+    assert "invalid" in native.last_error()

@@ -1,0 +1,178 @@
+"""GPU parity tests of the single-LP path (mpx_pdlp_solve and its kernels)
+against the oracle, through the C-ABI."""
+import numpy as np
+import pytest
+
+from conftest import TIGHT
+from lp_cases import STATUS, closed_form_cases, medium_lp, spmv_cases
+from mipx_amd import lpgen, native
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+@pytest.mark.parametrize("group", [1, 2, 4, 8, 16, 64])
+def test_spmv_device_parity(group):
+    torch = _torch()
+    lp = lpgen.c1_large(seed=9, m=20000, n=40000).lp
+    x = np.random.default_rng(0).standard_normal(lp.num_cols)
+    want = ob.spmv(lp, x)
+    dev = lambda a: torch.from_numpy(a).cuda()
+    y = torch.zeros(lp.num_rows, dtype=torch.float64, device="cuda")
+    native.spmv_device(lp.num_rows, lp.num_cols, dev(lp.row_starts), dev(lp.col_indices),
+                       dev(lp.values), dev(x), y, group)
+    torch.cuda.synchronize()
+    got = y.cpu().numpy()
+    if group == 1:
+        assert np.array_equal(got, want)      # reference summation order
+    else:
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("case", spmv_cases(), ids=lambda c: c[0]["name"])
+def test_spmv_known_answers_on_device(case):
+    torch = _torch()
+    c, lp = case
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    y = torch.zeros(lp.num_rows, dtype=torch.float64, device="cuda")
+    native.spmv_device(lp.num_rows, lp.num_cols, dev(lp.row_starts), dev(lp.col_indices),
+                       dev(lp.values), dev(np.array(c["x"], dtype=float)), y, 1)
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), np.array(c["Ax"], dtype=float))
+
+
+def test_scaling_bitwise_vs_oracle():
+    lp = lpgen.c1_large(seed=3, m=3000, n=6000).lp
+    got = native.scale_problem(lp)
+    want = ob.scale_problem(lp)
+    for k in ("values", "at_values", "at_row_starts", "at_col_indices", "row_scale",
+              "col_scale", "sigma_base", "tau_base"):
+        assert np.array_equal(got[k], want[k]), k
+    # Global sums are reduced in a different (fixed) order on the device.
+    for k in ("objective_scale", "constraint_scale"):
+        assert abs(got[k] - want[k]) <= 1e-14 * abs(want[k]), k
+    for k in ("cscaled", "col_lower", "col_upper", "row_lower", "row_upper"):
+        np.testing.assert_allclose(got[k], want[k], rtol=1e-14, atol=0)
+    assert abs(got["spectral_norm"] - want["spectral_norm"]) <= 1e-6 * want["spectral_norm"]
+
+
+@pytest.mark.parametrize("group", [1, 4])
+def test_halpern_iterations_vs_oracle(group):
+    lp = lpgen.c0_plain(seed=21, m=400, n=1000).lp
+    s = ob.scale_problem(lp)
+    step = 0.998 / s["spectral_norm"]
+    pw = 0.7
+    rng = np.random.default_rng(4)
+    def state():
+        return dict(cx=rng.uniform(0, 1, lp.num_cols) * 0, ix=np.zeros(lp.num_cols),
+                    px=np.zeros(lp.num_cols), cy=np.zeros(lp.num_rows),
+                    iy=np.zeros(lp.num_rows), py=np.zeros(lp.num_rows))
+    a, b = state(), state()
+    ob.pdhg_iterate(lp, s, 60, 3, step, pw, a)
+    native.pdhg_iterate(lp, s, 60, 3, step, pw, b, precision=0, group=group)
+    for k in ("cx", "px", "cy", "py"):
+        if group == 1:
+            assert np.array_equal(a[k], b[k]), k
+        else:
+            np.testing.assert_allclose(b[k], a[k], rtol=1e-9, atol=1e-12)
+
+
+def test_halpern_fp32_iterations_close_to_fp64():
+    lp = lpgen.c0_plain(seed=22, m=400, n=1000).lp
+    s = ob.scale_problem(lp)
+    step = 0.998 / s["spectral_norm"]
+    mk = lambda: dict(cx=np.zeros(lp.num_cols), ix=np.zeros(lp.num_cols),
+                      px=np.zeros(lp.num_cols), cy=np.zeros(lp.num_rows),
+                      iy=np.zeros(lp.num_rows), py=np.zeros(lp.num_rows))
+    a, b = mk(), mk()
+    ob.pdhg_iterate(lp, s, 40, 0, step, 1.0, a)
+    native.pdhg_iterate(lp, s, 40, 0, step, 1.0, b, precision=1, group=4)
+    for k in ("cx", "px", "cy", "py"):
+        scale = max(1e-30, np.abs(a[k]).max())
+        assert np.abs(a[k] - b[k]).max() <= 1e-4 * scale, k
+
+
+@pytest.mark.parametrize("case", closed_form_cases(), ids=lambda c: c[0]["name"])
+def test_closed_form_on_gpu(case):
+    c, lp = case
+    opts = dict(verbose=0)
+    opts.update(c.get("options", {}))
+    r, x, y = native.solve(lp, **opts)
+    assert r["status"] == STATUS[c["status"]], r
+    if "objective" in c:
+        assert abs(r["objective"] - c["objective"]) <= c["tol"]
+    if "x" in c:
+        np.testing.assert_allclose(x, c["x"], atol=1e-5)
+    if "iterations" in c:
+        assert r["iterations"] == c["iterations"]
+    if c.get("farkas_check"):
+        ray = y[0] / abs(y[0])
+        assert ray >= -1e-6 and ray * (-1.0) - ray * 1.0 < -1e-6
+    ro, _, _ = ob.solve(lp, **opts)
+    assert ro["status"] == r["status"]
+
+
+@pytest.mark.parametrize("group", [1, 0])
+@pytest.mark.parametrize("seed,m,n", [(0xC0, 2000, 5000), (101, 300, 750)])
+def test_solve_fp64_vs_oracle(seed, m, n, group):
+    lp = lpgen.c0_plain(seed=seed, m=m, n=n).lp
+    r, x, y = native.solve(lp, spmv_group=group, **TIGHT)
+    ro, xo, yo = ob.solve(lp, **TIGHT)
+    print(f"gpu iters={r['iterations']} obj={r['objective']!r}  oracle iters={ro['iterations']} "
+          f"obj={ro['objective']!r}")
+    assert r["status"] == ro["status"] == 0
+    assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
+    k = ob.kkt_check(lp, x, y)
+    assert k["rel_primal"] <= 1e-6 * 1.01 and k["rel_gap"] <= 2e-6, k
+    # Same work-unit formula; the iteration count tracks the oracle's.
+    assert r["work_units"] == r["iterations"] * (4.0 * lp.nnz + 2.0 * (m + n))
+    assert abs(r["iterations"] - ro["iterations"]) <= 0.25 * ro["iterations"]
+
+
+def test_solve_mixed_precision():
+    lp = lpgen.c0_plain(seed=0xC0).lp
+    r, x, y = native.solve(lp, precision=1, **TIGHT)
+    ro, _, _ = ob.solve(lp, **TIGHT)
+    print(r)
+    assert r["status"] == 0
+    assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
+    k = ob.kkt_check(lp, x, y)
+    assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
+
+
+def test_medium_lp_gpu_matches_cpu():
+    """Reference [gpu] tests (tests/test_pdlp.cpp:517-546): GPU and CPU
+    objectives agree within 1e-4 on buildMediumLp (default options)."""
+    lp = medium_lp()
+    r, _, _ = native.solve(lp, verbose=0, max_iter=200000)
+    ro, _, _ = ob.solve(lp, verbose=0, max_iter=200000)
+    assert r["status"] == ro["status"] == 0
+    assert abs(r["objective"] - ro["objective"]) <= 1e-4
+
+
+def test_warm_start_reduces_iterations():
+    """Reference warm-start test (tests/test_pdlp.cpp:1076-1113)."""
+    lp = lpgen.c0_plain(seed=101, m=300, n=750).lp
+    r1, x1, y1 = native.solve(lp, **TIGHT)
+    r2, x2, y2 = native.solve(lp, warm=(x1, y1), **TIGHT)
+    assert r2["status"] == 0 and r2["iterations"] < r1["iterations"]
+    ro, _, _ = ob.solve(lp, warm=(x1, y1), **TIGHT)
+    assert abs(r2["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
+
+
+def test_stop_callback_reports_iter_limit():
+    from mipx_amd._abi import STOP_FN
+    lp = lpgen.c0_plain(seed=101, m=300, n=750).lp
+    calls = []
+    def stop(_user):
+        calls.append(1)
+        return 1 if len(calls) > 2 else 0
+    cb = STOP_FN(stop)
+    r, _, _ = native.solve(lp, stop_fn=cb, **TIGHT)
+    assert r["status"] == 3 and r["iterations"] == 400
