@@ -124,9 +124,14 @@ typedef struct mpx_pdlp_options {
     void* stop_user;                  /* NULL */
     /* ---- MI355X extensions ---- */
     int32_t precision;                /* MPX_PREC_FP64 */
-    double refine_tol;                /* mixed precision: FP64 refinement target
-                                         for primal/dual/gap; <=0 = the solve's
-                                         own tolerances */
+    double refine_tol;                /* mixed precision: FP32 iterations run
+                                         until the FP64-evaluated KKT ratio is
+                                         within 10x of the tolerances (or FP32
+                                         stalls); FP64 iterations then refine
+                                         until primal, dual and gap are below
+                                         refine_tol (<= 0: the residual
+                                         tolerances and half the gap
+                                         tolerance) */
     int32_t spmv_group;               /* lanes per row/column in the SpMV
                                          kernels, 0 = autotuned.  1 gives the
                                          sequential per-row summation order of

@@ -59,9 +59,27 @@ def build_lpgen():
           os.path.join(PKG, "libmipx_lpgen.so"), os.path.join(CSRC, "lpgen.cpp")])
 
 
+def build_host():
+    """C++ mirror of mipx::PdlpSolver (libmipx_host.so), the CLI and the C++
+    host tests, linked against the HIP library with an $ORIGIN rpath."""
+    inc = ["-I" + os.path.join(PKG, "host", "include"), "-I" + os.path.join(ROOT, "include")]
+    src = os.path.join(PKG, "host", "src")
+    flags = ["-std=c++20", "-O2", "-Wall", "-fPIC"]
+    _run(["g++"] + flags + inc + ["-shared", "-o", os.path.join(PKG, "libmipx_host.so"),
+                                  os.path.join(src, "pdlp.cpp"), os.path.join(src, "sparse_matrix.cpp"),
+                                  "-L" + PKG, "-lmipx_pdlp", "-Wl,-rpath,$ORIGIN"])
+    bind = os.path.join(PKG, "bin")
+    os.makedirs(bind, exist_ok=True)
+    for exe, extra in (("mipx_solve", ["-lmipx_lpgen"]), ("host_tests", [])):
+        _run(["g++"] + flags + inc + ["-o", os.path.join(bind, exe), os.path.join(src, exe + ".cpp"),
+                                      "-L" + PKG, "-lmipx_host", "-lmipx_pdlp"] + extra +
+             ["-Wl,-rpath,$ORIGIN/.."])
+
+
 def build_all():
     build_lpgen()
     build_hip()
+    build_host()
 
 
 if __name__ == "__main__":

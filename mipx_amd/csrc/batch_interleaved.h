@@ -1,0 +1,262 @@
+// batch_interleaved.h -- lane-per-instance iteration kernels for batches that
+// share one constraint matrix (branch-and-bound node relaxations).
+//
+// Within a round the active instances of one precision are packed into dense
+// groups of 64 (one instance per wave lane) and their iterates are stored
+// interleaved: V[group][row][lane].  A wave then handles one column (or row)
+// for 64 instances at once: the matrix entries of that column are uniform
+// (scalar loads, L2-resident), and every gather of y[row] / x_r[col] is one
+// contiguous 64-element load.  Per-instance data (primal weight, inner count,
+// iteration limit of the round) sits in per-lane registers.
+//
+// Round structure (driven by BatchEngine):
+//   kSlotLists      compacts the active instances of each precision into slots
+//   kInterleave     per-instance storage -> interleaved (x, x0, l, u, y, y0)
+//   N x kILPrimal / kILDual
+//   kDeinterleave   interleaved -> per-instance storage (x, y); x^ and y^ are
+//                   written straight to per-instance storage on the last
+//                   iteration of the round
+// after which the per-instance evaluation / control kernel runs as usual.
+#pragma once
+
+#include "batch_kernels.h"
+
+namespace mpx {
+
+constexpr int kLanes = 64;
+
+struct ILane {
+    double step, pw;
+    int32_t inner, lim;
+};
+
+// Slot lists: slots[p][s] = instance id for precision p (0 = FP32, 1 = FP64),
+// counts[p] = number of slots, lanes[p][s] = its scalars.  One block, in
+// instance order (deterministic).
+__global__ __launch_bounds__(1024) void kSlotLists(const BInst* __restrict__ insts, int K,
+                                                   int* __restrict__ slots32,
+                                                   int* __restrict__ slots64,
+                                                   ILane* __restrict__ lanes32,
+                                                   ILane* __restrict__ lanes64,
+                                                   int* __restrict__ counts) {
+    __shared__ int part[2][1024];
+    __shared__ int carry[2];
+    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __syncthreads();
+    for (int base = 0; base < K; base += 1024) {
+        const int k = base + threadIdx.x;
+        int p32 = 0, p64 = 0;
+        if (k < K) {
+            const BCtrl& c = *insts[k].ctrl;
+            const bool act = c.phase != kPhaseDone && c.round_iters > 0;
+            p32 = act && c.phase == kPhaseFp32;
+            p64 = act && c.phase != kPhaseFp32;
+        }
+        part[0][threadIdx.x] = p32;
+        part[1][threadIdx.x] = p64;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int a0 = threadIdx.x >= off ? part[0][threadIdx.x - off] : 0;
+            const int a1 = threadIdx.x >= off ? part[1][threadIdx.x - off] : 0;
+            __syncthreads();
+            part[0][threadIdx.x] += a0;
+            part[1][threadIdx.x] += a1;
+            __syncthreads();
+        }
+        if (k < K && (p32 || p64)) {
+            const BCtrl& c = *insts[k].ctrl;
+            const ILane ln{c.step, c.pw, c.inner, c.round_iters};
+            if (p32) {
+                const int s = carry[0] + part[0][threadIdx.x] - 1;
+                slots32[s] = k;
+                lanes32[s] = ln;
+            } else {
+                const int s = carry[1] + part[1][threadIdx.x] - 1;
+                slots64[s] = k;
+                lanes64[s] = ln;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            carry[0] += part[0][1023];
+            carry[1] += part[1][1023];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        counts[0] = carry[0];
+        counts[1] = carry[1];
+    }
+}
+
+// Work items of a group pass: (group, block of 4 columns or rows).  Fixed
+// grid, count from device memory; groups are dealt XCD by XCD (group g's
+// items run on XCD g % 8 when there are at least 8 groups) so that a group's
+// interleaved y / x_r slice stays in one L2.
+template <typename F>
+__device__ inline void forGroupItems(int nslots, int per_group, F&& f) {
+    const int ngroups = (nslots + kLanes - 1) / kLanes;
+    const int total = ngroups * per_group;
+    if (ngroups < 8) {
+        for (int t = blockIdx.x; t < total; t += gridDim.x) f(t / per_group, t % per_group);
+        return;
+    }
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8, nslot = gridDim.x / 8;
+    for (int g = xcd; g < ngroups; g += 8)
+        for (int r = slot; r < per_group; r += nslot) f(g, r);
+}
+
+// Per-instance -> interleaved for `rows` elements of each slot's vector
+// (selected by `which`: 0 x/x0/l/u, 1 y/y0).  A block transposes a tile of
+// 64 instances x 64 elements through LDS: coalesced reads per instance,
+// coalesced writes per element.
+template <typename T>
+__device__ inline const BVecs<T>& vecsOf(const BInst& in);
+template <>
+__device__ inline const BVecs<float>& vecsOf<float>(const BInst& in) { return in.f; }
+template <>
+__device__ inline const BVecs<double>& vecsOf<double>(const BInst& in) { return in.d; }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kInterleave(const BInst* __restrict__ insts,
+                                                       const int* __restrict__ slots,
+                                                       const int* __restrict__ count, int len,
+                                                       int which, T* __restrict__ A,
+                                                       T* __restrict__ B, T* __restrict__ C,
+                                                       T* __restrict__ D) {
+    __shared__ T tile[kLanes][kLanes + 1];
+    const int nslots = *count;
+    const int per_group = (len + kLanes - 1) / kLanes;
+    const int nsrc = which == 0 ? 4 : 2;
+    forGroupItems(nslots, per_group, [&](int g, int tb) {
+        const int e0 = tb * kLanes;
+        for (int a = 0; a < nsrc; ++a) {
+            // load: row r = slot within the group, column = element offset
+            for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
+                const int s = g * kLanes + r;
+                const int e = e0 + static_cast<int>(threadIdx.x % kLanes);
+                T v = T(0);
+                if (s < nslots && e < len) {
+                    const BVecs<T>& vv = vecsOf<T>(insts[slots[s]]);
+                    const T* src = which == 0 ? (a == 0 ? vv.x : a == 1 ? vv.x0 : a == 2 ? vv.l : vv.u)
+                                              : (a == 0 ? vv.y : vv.y0);
+                    v = src[e];
+                }
+                tile[r][threadIdx.x % kLanes] = v;
+            }
+            __syncthreads();
+            T* dst = a == 0 ? A : a == 1 ? B : a == 2 ? C : D;
+            for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
+                const int e = e0 + r;
+                const int lane = static_cast<int>(threadIdx.x % kLanes);
+                if (e < len) dst[(static_cast<size_t>(g) * len + e) * kLanes + lane] = tile[lane][r];
+            }
+            __syncthreads();
+        }
+    });
+}
+
+// Interleaved -> per-instance x (which = 0, len = n) or y (which = 1, len = m).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kDeinterleave(const BInst* __restrict__ insts,
+                                                         const int* __restrict__ slots,
+                                                         const int* __restrict__ count, int len,
+                                                         int which, const T* __restrict__ A) {
+    __shared__ T tile[kLanes][kLanes + 1];
+    const int nslots = *count;
+    const int per_group = (len + kLanes - 1) / kLanes;
+    forGroupItems(nslots, per_group, [&](int g, int tb) {
+        const int e0 = tb * kLanes;
+        for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
+            const int e = e0 + r;
+            const int lane = static_cast<int>(threadIdx.x % kLanes);
+            tile[lane][r] = e < len ? A[(static_cast<size_t>(g) * len + e) * kLanes + lane] : T(0);
+        }
+        __syncthreads();
+        for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
+            const int s = g * kLanes + r;
+            const int e = e0 + static_cast<int>(threadIdx.x % kLanes);
+            if (s < nslots && e < len) {
+                const BVecs<T>& vv = vecsOf<T>(insts[slots[s]]);
+                (which == 0 ? vv.x : vv.y)[e] = tile[r][threadIdx.x % kLanes];
+            }
+        }
+        __syncthreads();
+    });
+}
+
+// Primal half step for one column j of 64 instances per wave (4 columns per
+// block).  Shared data: CSC structure/values, c, tau_base.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kILPrimal(
+    const BInst* __restrict__ insts, const int* __restrict__ slots, const int* __restrict__ count,
+    const ILane* __restrict__ lanes, int n, int m, const int* __restrict__ cs,
+    const int* __restrict__ ri, const T* __restrict__ valt, const T* __restrict__ c,
+    const T* __restrict__ tau_base, const T* __restrict__ IY, T* __restrict__ IX,
+    const T* __restrict__ IX0, T* __restrict__ IXR, const T* __restrict__ IL,
+    const T* __restrict__ IU, int kiter) {
+    const int nslots = *count;
+    const int per_group = (n + 3) / 4;
+    forGroupItems(nslots, per_group, [&](int g, int jb) {
+        const int j = jb * 4 + static_cast<int>(threadIdx.x) / kLanes;
+        const int lane = static_cast<int>(threadIdx.x) % kLanes;
+        if (j >= n) return;
+        const int s = g * kLanes + lane;
+        const bool live = s < nslots;
+        const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+        if (kiter > ln.lim) return;  // lane idle this iteration
+        const T* yg = IY + static_cast<size_t>(g) * m * kLanes + lane;
+        T acc = T(0);
+        const int qb = cs[j], qe = cs[j + 1];
+        for (int q = qb; q < qe; ++q) acc += valt[q] * yg[static_cast<size_t>(ri[q]) * kLanes];
+        const size_t o = (static_cast<size_t>(g) * n + j) * kLanes + lane;
+        const int kk = ln.inner + kiter;
+        const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
+        const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
+        const T tau = step * tau_base[j] / pw;
+        const T xj = IX[o];
+        const T t = xj - tau * (c[j] + acc);
+        const T h = clampRef(t, IL[o], IU[o]);
+        const T r = T(2) * h - xj;
+        IXR[o] = r;
+        IX[o] = lam * r + (T(1) - lam) * IX0[o];
+        if (kiter == ln.lim) vecsOf<T>(insts[slots[s]]).xh[j] = h;
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kILDual(
+    const BInst* __restrict__ insts, const int* __restrict__ slots, const int* __restrict__ count,
+    const ILane* __restrict__ lanes, int n, int m, const int* __restrict__ rs,
+    const int* __restrict__ ci, const T* __restrict__ val, const T* __restrict__ sigma_base,
+    const T* __restrict__ rl, const T* __restrict__ ru, const T* __restrict__ IXR,
+    T* __restrict__ IY, const T* __restrict__ IY0, int kiter) {
+    const int nslots = *count;
+    const int per_group = (m + 3) / 4;
+    forGroupItems(nslots, per_group, [&](int g, int ib) {
+        const int i = ib * 4 + static_cast<int>(threadIdx.x) / kLanes;
+        const int lane = static_cast<int>(threadIdx.x) % kLanes;
+        if (i >= m) return;
+        const int s = g * kLanes + lane;
+        const bool live = s < nslots;
+        const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+        if (kiter > ln.lim) return;
+        const T* xg = IXR + static_cast<size_t>(g) * n * kLanes + lane;
+        T acc = T(0);
+        const int qb = rs[i], qe = rs[i + 1];
+        for (int q = qb; q < qe; ++q) acc += val[q] * xg[static_cast<size_t>(ci[q]) * kLanes];
+        const size_t o = (static_cast<size_t>(g) * m + i) * kLanes + lane;
+        const int kk = ln.inner + kiter;
+        const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
+        const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
+        const T sig = step * sigma_base[i] * pw;
+        const T yi = IY[o];
+        const T v = yi + sig * acc;
+        const T h = dualProx(v, sig, rl[i], ru[i]);
+        const T r = T(2) * h - yi;
+        IY[o] = lam * r + (T(1) - lam) * IY0[o];
+        if (kiter == ln.lim) vecsOf<T>(insts[slots[s]]).yh[i] = h;
+    });
+}
+
+}  // namespace mpx

@@ -1,0 +1,292 @@
+// batch_kernels.h -- batched PDLP: one workgroup per LP instance.
+//
+// Many LPs (branch-and-bound node relaxations sharing one A, or independent
+// LPs of uneven size) run in one launch; each 256-thread workgroup owns one
+// instance and runs its whole restarted-Halpern PDHG loop: iterations, the
+// FP64 evaluation every N iterations, the restart / PI primal-weight /
+// termination / certificate decisions (the reference's host logic,
+// src/lp/pdlp.cpp:2228-2469, executed by thread 0 in FP64) and the polish
+// phase.  Instances never share mutable state, so an instance's result does
+// not depend on what else is in the launch.
+//
+// Precision: iterations in T (float for MPX_PREC_MIXED, with float
+// accumulation), evaluation always in FP64 against the FP64 scaled matrix.
+// A launch runs at most `budget` iterations per instance and returns; state
+// stays resident in HBM between launches (hipGraph-captured rounds).
+//
+// When y (m) and x_r (n) fit the workgroup's LDS they are kept there for the
+// whole block of iterations, so both SpMV gathers hit LDS instead of L2/HBM.
+#pragma once
+
+#include "device_common.h"
+#include "pdlp_kernels.h"
+
+namespace mpx {
+
+// Per-instance control state (FP64 scalars of the reference's host loop).
+struct BCtrl {
+    double step, pw;
+    double fpe0, last_fpe;
+    double pw_err_sum, pw_last_err, best_pw, best_score, precond_pw;
+    double cnorm, bnorm;       // original-problem termination norms
+    double os, cs, sgn, offset;
+    double best_kkt;
+    // last evaluation
+    double rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj;
+    int32_t inner, total, pcount, phase, status, restarts, fp32_iters, stall;
+    int32_t power_iters, g_row, g_col, use_lds;
+    int32_t round_iters, skip_eval;
+};
+
+enum BPhase { kPhaseFp32 = 0, kPhaseFp64 = 1, kPhasePolish = 2, kPhaseDone = 3 };
+
+struct BOptions {
+    int32_t max_iter, N, polish_max_iter, norm_inf, update_pw, enable_polish, precond_refresh,
+        do_pc;
+    double primal_tol, dual_tol, opt_tol, polish_p_tol, polish_d_tol;
+    double refine_p_tol, refine_d_tol, refine_g_tol;  // FP64 phase of mixed precision
+    int32_t mixed, pad;
+    double pid_kp, pid_ki, pid_kd, pid_i_smooth;
+    double rs_suff, rs_nec, rs_art, precond_ratio;
+};
+
+template <typename T>
+struct BVecs {
+    T *x, *x0, *xr, *xh, *y, *y0, *yh;
+    const T *val, *valt, *c, *tau, *l, *u, *sig, *rl, *ru;
+};
+
+struct BInst {
+    int32_t m, n, nnz, sense_max;
+    int32_t preset, use_lds, owner, pad;
+    const int *rs, *ci, *cs, *ri;
+    // original bounds / objective (certificates)
+    const double *lo_o, *hi_o, *rl_o, *ru_o, *obj_o;
+    // scaled fp64 problem
+    const double *val, *valt, *c, *l, *u, *rl, *ru, *row_scale, *col_scale, *sig, *tau;
+    BVecs<float> f;
+    BVecs<double> d;
+    BCtrl* ctrl;
+};
+
+// ---- block helpers -----------------------------------------------------------
+__device__ inline double blockSum1(double v) {
+    double a[1] = {v};
+    const bool mx[1] = {false};
+    double o[1];
+    blockReduce<1>(a, mx, o);
+    __shared__ double bc;
+    if (threadIdx.x == 0) bc = o[0];
+    __syncthreads();
+    const double r = bc;
+    __syncthreads();
+    return r;
+}
+
+template <int G, typename T, typename TX>
+__device__ inline T bDot(const int* __restrict__ starts, const int* __restrict__ idx,
+                         const T* __restrict__ val, const TX* x, int row, int lane) {
+    const int b = starts[row], e = starts[row + 1];
+    T acc = T(0);
+    for (int k = b + lane; k < e; k += G) acc += val[k] * static_cast<T>(x[idx[k]]);
+    return groupSum<G>(acc);
+}
+
+constexpr int kNumEval = static_cast<int>(kNumColFields) + static_cast<int>(kNumRowFields);
+
+struct BEval {
+    double fpe, rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj, dx2, dy2;
+};
+
+// FP64 evaluation of (xh, yh) against (x0, y0), same formulas as the single
+// path (src/lp/pdlp.cpp:1076-1202).  Result valid in all threads.
+template <int GC, int GR, typename T>
+__device__ inline BEval bEvaluate(const BInst& in, const BVecs<T>& v, const BCtrl& c) {
+    const int tid = threadIdx.x;
+    double cf[kNumColFields] = {0, 0, 0, 0, 0, 0};
+    for (int base = 0; base < in.n; base += kBlock / GC) {
+        const int j = base + tid / GC, lane = tid % GC;
+        if (j < in.n) {
+            double aty = 0.0, atdy = 0.0;
+            for (int q = in.cs[j] + lane; q < in.cs[j + 1]; q += GC) {
+                const int i = in.ri[q];
+                const double a = in.valt[q];
+                const double yi = static_cast<double>(v.yh[i]);
+                aty += a * yi;
+                atdy += a * (yi - static_cast<double>(v.y0[i]));
+            }
+            aty = groupSum<GC>(aty);
+            atdy = groupSum<GC>(atdy);
+            if (lane == 0) {
+                const double xj = static_cast<double>(v.xh[j]);
+                const double dx = xj - static_cast<double>(v.x0[j]);
+                cf[kDx2] += dx * dx;
+                cf[kInter] += atdy * dx;
+                const double g = in.c[j] + aty;
+                const double lb = in.l[j], ub = in.u[j];
+                double dr;
+                if (xj <= lb + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
+                else if (xj >= ub - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
+                else dr = g;
+                const double d = dr / (c.os * in.col_scale[j]);
+                cf[kDsq] += d * d;
+                cf[kDmax] = maxRef(cf[kDmax], fabs(d));
+                cf[kPobj] += in.c[j] * xj;
+                double gp = g;
+                if (!isfinite(lb)) gp = minRef(gp, 0.0);
+                if (!isfinite(ub)) gp = maxRef(gp, 0.0);
+                if (gp > 0.0 && isfinite(lb)) cf[kDcol] += gp * lb;
+                else if (gp < 0.0 && isfinite(ub)) cf[kDcol] += gp * ub;
+            }
+        }
+    }
+    double rf[kNumRowFields] = {0, 0, 0, 0};
+    for (int base = 0; base < in.m; base += kBlock / GR) {
+        const int i = base + tid / GR, lane = tid % GR;
+        if (i < in.m) {
+            const double ax = bDot<GR>(in.rs, in.ci, in.val, v.xh, i, lane);
+            if (lane == 0) {
+                const double lo = in.rl[i], hi = in.ru[i];
+                const double vs = ax - clampRef(ax, lo, hi);
+                const double p = vs / (c.cs * in.row_scale[i]);
+                rf[kPsq] += p * p;
+                rf[kPmax] = maxRef(rf[kPmax], fabs(p));
+                const double yi = static_cast<double>(v.yh[i]);
+                const double dy = yi - static_cast<double>(v.y0[i]);
+                rf[kDy2] += dy * dy;
+                double yp = yi;
+                if (!isfinite(hi)) yp = minRef(yp, 0.0);
+                if (!isfinite(lo)) yp = maxRef(yp, 0.0);
+                if (yp > 0.0 && isfinite(hi)) rf[kDrow] -= yp * hi;
+                else if (yp < 0.0 && isfinite(lo)) rf[kDrow] -= yp * lo;
+            }
+        }
+    }
+    double all[kNumEval], out[kNumEval];
+    bool mx[kNumEval];
+    for (int f = 0; f < kNumColFields; ++f) {
+        all[f] = cf[f];
+        mx[f] = f == kDmax;
+    }
+    for (int f = 0; f < kNumRowFields; ++f) {
+        all[kNumColFields + f] = rf[f];
+        mx[kNumColFields + f] = f == kPmax;
+    }
+    blockReduce<kNumEval>(all, mx, out);
+    __shared__ double sh[kNumEval];
+    if (tid == 0)
+        for (int f = 0; f < kNumEval; ++f) sh[f] = out[f];
+    __syncthreads();
+    const double* mc = sh;
+    const double* mr = sh + kNumColFields;
+    BEval e;
+    e.dx2 = mc[kDx2];
+    e.dy2 = mr[kDy2];
+    const double movement = e.dx2 * c.pw + e.dy2 / c.pw;
+    const double interaction = mc[kInter] * (2.0 * c.step);
+    e.fpe = sqrt(maxRef(movement + interaction, 0.0));
+    e.rel_p = sqrt(mr[kPsq]) / (1.0 + c.bnorm);
+    e.rel_p_inf = mr[kPmax] / (1.0 + c.bnorm);
+    e.rel_d = sqrt(mc[kDsq]) / (1.0 + c.cnorm);
+    e.rel_d_inf = mc[kDmax] / (1.0 + c.cnorm);
+    const double unscale = 1.0 / (c.os * c.cs);
+    e.pobj = c.offset + c.sgn * mc[kPobj] * unscale;
+    e.dobj = c.offset + c.sgn * (mc[kDcol] + mr[kDrow]) * unscale;
+    e.gap = fabs(e.pobj - e.dobj) / (1.0 + fabs(e.pobj) + fabs(e.dobj));
+    __syncthreads();
+    return e;
+}
+
+// Farkas and primal-ray certificates (src/lp/pdlp.cpp:119-284) evaluated on
+// the device in the scaled space: with y = sgn y^ R / os and x = C x^ / cs,
+// A'y = sgn C^-1 (A~' y^) / os and A x = R^-1 (A~ x^) / cs.  Returns
+// 1 = infeasible, 2 = unbounded, 0 = neither.  Uses the original row / column
+// bounds recovered from the scaled ones.
+template <int GC, int GR, typename T>
+__device__ inline int bCertificates(const BInst& in, const BVecs<T>& v, const BCtrl& c,
+                                    const double* lo_o, const double* hi_o, const double* rl_o,
+                                    const double* ru_o, const double* obj_o, int sense_max) {
+    const int tid = threadIdx.x;
+    // --- Farkas on y_orig ---
+    double ysq = 0.0;
+    for (int i = tid; i < in.m; i += kBlock) {
+        const double yo = c.sgn * static_cast<double>(v.yh[i]) * in.row_scale[i] * (1.0 / c.os);
+        ysq += yo * yo;
+    }
+    const double ny = sqrt(blockSum1(ysq));
+    const double tol = 1e-6;
+    int verdict = 0;
+    if (ny >= 1e-15) {
+        double bad = 0.0, rowpart = 0.0;
+        for (int i = tid; i < in.m; i += kBlock) {
+            const double r = c.sgn * static_cast<double>(v.yh[i]) * in.row_scale[i] * (1.0 / c.os) / ny;
+            const bool lo = isfinite(rl_o[i]), hi = isfinite(ru_o[i]);
+            if (hi && !lo && r < -tol) bad = 1.0;
+            if (lo && !hi && r > tol) bad = 1.0;
+            if (rl_o[i] == ru_o[i]) rowpart += r * rl_o[i];
+            else if (r > 0.0 && hi) rowpart += r * ru_o[i];
+            else if (r < 0.0 && lo) rowpart += r * rl_o[i];
+        }
+        double colpart = 0.0;
+        for (int base = 0; base < in.n; base += kBlock / GC) {
+            const int j = base + tid / GC, lane = tid % GC;
+            if (j < in.n) {
+                double acc = 0.0;
+                for (int q = in.cs[j] + lane; q < in.cs[j + 1]; q += GC)
+                    acc += in.valt[q] * static_cast<double>(v.yh[in.ri[q]]);
+                acc = groupSum<GC>(acc);
+                if (lane == 0) {
+                    const double atr = c.sgn * acc / in.col_scale[j] / c.os / ny;
+                    if (isfinite(lo_o[j]) && atr < -tol) bad = 1.0;
+                    if (isfinite(hi_o[j]) && atr > tol) bad = 1.0;
+                    if (atr > 0.0 && isfinite(lo_o[j])) colpart += atr * lo_o[j];
+                    else if (atr < 0.0 && isfinite(hi_o[j])) colpart += atr * hi_o[j];
+                }
+            }
+        }
+        const double sbad = blockSum1(bad), srow = blockSum1(rowpart), scol = blockSum1(colpart);
+        if (sbad == 0.0 && srow - scol < -tol) verdict = 1;
+    }
+    if (verdict) return verdict;
+    // --- primal ray on x_orig ---
+    double xsq = 0.0;
+    for (int j = tid; j < in.n; j += kBlock) {
+        const double xo = static_cast<double>(v.xh[j]) * in.col_scale[j] * (1.0 / c.cs);
+        xsq += xo * xo;
+    }
+    const double nx = sqrt(blockSum1(xsq));
+    if (nx < 1e-15) return 0;
+    double bad = 0.0, cr = 0.0;
+    for (int j = tid; j < in.n; j += kBlock) {
+        const double r = static_cast<double>(v.xh[j]) * in.col_scale[j] * (1.0 / c.cs) / nx;
+        if (isfinite(lo_o[j]) && r < -tol) bad = 1.0;
+        if (isfinite(hi_o[j]) && r > tol) bad = 1.0;
+        cr += obj_o[j] * r;
+    }
+    for (int base = 0; base < in.m; base += kBlock / GR) {
+        const int i = base + tid / GR, lane = tid % GR;
+        if (i < in.m) {
+            double acc = 0.0;
+            for (int q = in.rs[i] + lane; q < in.rs[i + 1]; q += GR)
+                acc += in.val[q] * static_cast<double>(v.xh[in.ci[q]]);
+            acc = groupSum<GR>(acc);
+            if (lane == 0) {
+                const double ar = acc / in.row_scale[i] / c.cs / nx;
+                const bool lo = isfinite(rl_o[i]), hi = isfinite(ru_o[i]);
+                if (lo && hi) {
+                    if (fabs(ar) > tol) bad = 1.0;
+                } else if (hi) {
+                    if (ar > tol) bad = 1.0;
+                } else if (lo) {
+                    if (ar < -tol) bad = 1.0;
+                }
+            }
+        }
+    }
+    const double sbad = blockSum1(bad), scr = blockSum1(cr);
+    const double sg = sense_max ? -1.0 : 1.0;
+    if (sbad == 0.0 && sg * scr < -tol) return 2;
+    return 0;
+}
+
+}  // namespace mpx

@@ -259,10 +259,22 @@ void mpxgen_branch_bounds(void* h, uint64_t seed, double int_fraction, double fr
     Rng rng(seed);
     const double frac = rng.uniform(frac_lo, frac_hi);
     const int32_t nb = std::min(nint, static_cast<int32_t>(std::lround(frac * nint)));
-    // Partial Fisher-Yates over the integer columns.
-    std::vector<int32_t> ints(perm.begin(), perm.begin() + nint);
-    for (int32_t t = 0; t < nb; ++t) {
-        const int32_t r = t + static_cast<int32_t>(rng.below(static_cast<uint32_t>(nint - t)));
+    // Branchable integer columns: those whose branch does not fix the
+    // variable (floor(x0) > lower and ceil(x0) < upper).  A fixed column
+    // (lower == upper) with a negative reduced cost never passes the
+    // reference's dual-residual test, which checks the lower bound first
+    // (src/lp/pdlp.cpp:1135-1142), so such a node would only exercise the
+    // iteration limit.
+    std::vector<int32_t> ints;
+    for (int32_t t = 0; t < nint; ++t) {
+        const int32_t j = perm[t];
+        if (std::floor(g->x0[j]) > g->cl[j] && std::ceil(g->x0[j]) < g->cu[j]) ints.push_back(j);
+    }
+    const int32_t ne = static_cast<int32_t>(ints.size());
+    const int32_t take = std::min(nb, ne);
+    // Partial Fisher-Yates over the branchable integer columns.
+    for (int32_t t = 0; t < take; ++t) {
+        const int32_t r = t + static_cast<int32_t>(rng.below(static_cast<uint32_t>(ne - t)));
         std::swap(ints[t], ints[r]);
         const int32_t j = ints[t];
         if (rng.next() >> 63) cu_out[j] = std::floor(g->x0[j]);

@@ -1,23 +1,1756 @@
-// pdlp_batch.hip -- batched PDLP (many LPs in one persistent launch) and the
-// serving loop.  Work in progress: the entry points are exported and report
-// MPX_ERR_UNSUPPORTED until the persistent kernel lands.
+// pdlp_batch.hip -- batched PDLP (many LPs, one workgroup each, one launch per
+// round) and the continuous-batching serving loop.
+//
+// Device work per round: one launch of kBatchIterate<float> (instances in
+// their FP32 phase) and one of kBatchIterate<double> (FP64 refinement and
+// polish), each running up to `blocks_per_round` evaluation blocks of N
+// iterations per instance; the pair is captured once in a hipGraph and
+// replayed.  The only host synchronisation per round is a 4-byte read of the
+// number of unfinished instances.
+//
+// Setup is also on the device, one workgroup per distinct matrix: Ruiz,
+// objective/bound scaling, Pock-Chambolle bases, bound/objective rescaling
+// (bit-identical per element to the reference's buildScaledProblem,
+// src/lp/pdlp.cpp:469-669) and the power iteration (pdlp.cpp:748-795).
+// Branch-and-bound instances that share A share all of it and only scale
+// their own column bounds.
+
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
 #include "../../include/mipx_pdlp.h"
+#include "batch_interleaved.h"
+#include "batch_kernels.h"
 #include "device_common.h"
+
+namespace mpx {
+
+// (lanes per column, lanes per row) presets the autotuner chooses from.
+constexpr int kNumPresets = 8;
+__host__ __device__ constexpr int presetGC(int p) {
+    return p == 0 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : p <= 6 ? 8 : 16;
+}
+__host__ __device__ constexpr int presetGR(int p) {
+    return p == 0 ? 1 : p == 1 ? 2 : p <= 3 ? 4 : p <= 5 ? 8 : 16;
+}
+
+// Heuristic preset from mean entries per column / row (about two entries per
+// lane); the measured autotuner (mpx_batch_autotune) can override it.
+int heuristicPreset(long long nnz, int m, int n) {
+    const double ac = n ? static_cast<double>(nnz) / n : 1.0;
+    const double ar = m ? static_cast<double>(nnz) / m : 1.0;
+    int best = 0;
+    double bestScore = 1e30;
+    for (int p = 0; p < kNumPresets; ++p) {
+        const double sc = std::abs(std::log2(2.0 * presetGC(p) / ac)) +
+                          std::abs(std::log2(2.0 * presetGR(p) / ar));
+        if (sc < bestScore) {
+            bestScore = sc;
+            best = p;
+        }
+    }
+    return best;
+}
+
+// ---- control logic (thread 0; the reference's host loop) -------------------
+__device__ inline double kktRatio(const BEval& e, const BOptions& o) {
+    const double tp = o.norm_inf ? e.rel_p_inf : e.rel_p;
+    const double td = o.norm_inf ? e.rel_d_inf : e.rel_d;
+    return maxRef(maxRef(tp / o.primal_tol, td / o.dual_tol), e.gap / o.opt_tol);
+}
+
+__device__ inline void updatePrimalWeight(BCtrl& c, const BEval& e, const BOptions& o,
+                                          bool polish) {
+    const double pd = sqrt(e.dx2), dd = sqrt(e.dy2);
+    const double ratio = (e.rel_p > 1e-15) ? e.rel_d / e.rel_p : 0.0;
+    if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+        const double err = log(dd) - log(pd) - log(c.pw);
+        c.pw_err_sum = o.pid_i_smooth * c.pw_err_sum + err;
+        const double derr = err - c.pw_last_err;
+        c.pw *= exp(o.pid_kp * err + o.pid_ki * c.pw_err_sum + o.pid_kd * derr);
+        c.pw = clampRef(c.pw, 1e-4, 1e8);
+        c.pw_last_err = err;
+        if (!polish) {
+            const double score =
+                fabs(log10(maxRef(e.rel_d, 1e-15)) - log10(maxRef(e.rel_p, 1e-15)));
+            if (score < c.best_score) {
+                c.best_score = score;
+                c.best_pw = c.pw;
+            }
+        }
+    } else if (!polish) {
+        c.pw = c.best_pw;
+        c.pw_err_sum = 0.0;
+        c.pw_last_err = 0.0;
+    }
+}
+
+// Decision codes written by thread 0 for the workgroup.
+enum BAction { kActNone = 0, kActRestart = 1, kActToFp64 = 2, kActConverged = 3, kActFinish = 4,
+               kActCert = 5 };
+
+// Mixed precision: FP32 hands over to FP64 once the FP64-judged KKT ratio
+// (max of residual / tolerance) is within this factor, or after this many
+// evaluations without halving it.
+constexpr double kFp32ExitRatio = 10.0;
+constexpr int kFp32Stall = 20;
+
+template <typename T>
+__device__ inline const BVecs<T>& pickVecs(const BInst& in);
+template <>
+__device__ inline const BVecs<float>& pickVecs<float>(const BInst& in) { return in.f; }
+template <>
+__device__ inline const BVecs<double>& pickVecs<double>(const BInst& in) { return in.d; }
+
+__global__ void kZeroInt(int* p) { *p = 0; }
+
+// ---- packed iteration kernels ----------------------------------------------
+// All active instances of one precision form one flat list of tiles; a tile
+// is (instance, first column) covering kBlock/G columns (or rows) of that
+// instance.  A fixed grid walks the list (persistent), so one captured launch
+// serves any number of active instances.  XCD-aware dealing: workgroup b runs
+// on XCD b % 8; tiles come in chunks of (grid / 8) consecutive tiles and
+// chunk q is dealt to the workgroups of XCD q % 8, so the tiles of one
+// instance share one XCD's L2 for the gathered y / x_r slice.  Lists shorter
+// than one chunk per XCD are dealt round-robin over the whole grid.
+struct Tile {
+    int32_t inst;
+    int32_t first;
+};
+constexpr int kNumXcd = 8;
+
+template <typename F>
+__device__ inline void forTiles(int cnt, F&& f) {
+    const int nslot = static_cast<int>(gridDim.x) / kNumXcd;
+    if (cnt < kNumXcd * nslot) {
+        for (int t = blockIdx.x; t < cnt; t += gridDim.x) f(t);
+        return;
+    }
+    const int xcd = blockIdx.x % kNumXcd, slot = blockIdx.x / kNumXcd;
+    for (int ch = xcd; ch * nslot < cnt; ch += kNumXcd) {
+        const int t = ch * nslot + slot;
+        if (t < cnt) f(t);
+    }
+}
+
+template <int G, typename T>
+__device__ inline void primalTile(const BInst& in, int first, int kiter) {
+    const BCtrl& c = *in.ctrl;
+    const int lim = c.round_iters;
+    if (kiter > lim) return;
+    const BVecs<T>& v = pickVecs<T>(in);
+    const int j = first + static_cast<int>(threadIdx.x) / G;
+    const int lane = static_cast<int>(threadIdx.x) % G;
+    if (j >= in.n) return;
+    const T aty = bDot<G>(in.cs, in.ri, v.valt, v.y, j, lane);
+    if (lane != 0) return;
+    const int kk = c.inner + kiter;
+    const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
+    const T step = static_cast<T>(c.step), pw = static_cast<T>(c.pw);
+    const T tau = step * v.tau[j] / pw;
+    const T xj = v.x[j];
+    const T tt = xj - tau * (v.c[j] + aty);
+    const T h = clampRef(tt, v.l[j], v.u[j]);
+    const T r = T(2) * h - xj;
+    if (kiter == lim) v.xh[j] = h;
+    v.xr[j] = r;
+    v.x[j] = lam * r + (T(1) - lam) * v.x0[j];
+}
+
+template <int G, typename T>
+__device__ inline void dualTile(const BInst& in, int first, int kiter) {
+    const BCtrl& c = *in.ctrl;
+    const int lim = c.round_iters;
+    if (kiter > lim) return;
+    const BVecs<T>& v = pickVecs<T>(in);
+    const int i = first + static_cast<int>(threadIdx.x) / G;
+    const int lane = static_cast<int>(threadIdx.x) % G;
+    if (i >= in.m) return;
+    const T axr = bDot<G>(in.rs, in.ci, v.val, v.xr, i, lane);
+    if (lane != 0) return;
+    const int kk = c.inner + kiter;
+    const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
+    const T step = static_cast<T>(c.step), pw = static_cast<T>(c.pw);
+    const T sig = step * v.sig[i] * pw;
+    const T yi = v.y[i];
+    const T vv = yi + sig * axr;
+    const T h = dualProx(vv, sig, v.rl[i], v.ru[i]);
+    const T r = T(2) * h - yi;
+    if (kiter == lim) v.yh[i] = h;
+    v.y[i] = lam * r + (T(1) - lam) * v.y0[i];
+}
+
+// The lane-group size of a tile is its instance's own preset, so an
+// instance's arithmetic does not depend on the rest of the batch.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kPackedPrimal(const BInst* __restrict__ insts,
+                                                         const Tile* __restrict__ tiles,
+                                                         const int* __restrict__ total, int kiter) {
+    forTiles(*total, [&](int t) {
+        const Tile tl = tiles[t];
+        const BInst& in = insts[tl.inst];
+        switch (presetGC(in.preset)) {
+            case 1: primalTile<1, T>(in, tl.first, kiter); break;
+            case 2: primalTile<2, T>(in, tl.first, kiter); break;
+            case 4: primalTile<4, T>(in, tl.first, kiter); break;
+            case 8: primalTile<8, T>(in, tl.first, kiter); break;
+            default: primalTile<16, T>(in, tl.first, kiter); break;
+        }
+    });
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kPackedDual(const BInst* __restrict__ insts,
+                                                       const Tile* __restrict__ tiles,
+                                                       const int* __restrict__ total, int kiter) {
+    forTiles(*total, [&](int t) {
+        const Tile tl = tiles[t];
+        const BInst& in = insts[tl.inst];
+        switch (presetGR(in.preset)) {
+            case 1: dualTile<1, T>(in, tl.first, kiter); break;
+            case 2: dualTile<2, T>(in, tl.first, kiter); break;
+            case 4: dualTile<4, T>(in, tl.first, kiter); break;
+            case 8: dualTile<8, T>(in, tl.first, kiter); break;
+            default: dualTile<16, T>(in, tl.first, kiter); break;
+        }
+    });
+}
+
+// Per-instance tile counts for the four lists (FP32 columns, FP32 rows,
+// FP64 columns, FP64 rows) of the coming round.
+__global__ void kTileCounts(const BInst* __restrict__ insts, int K, int* __restrict__ cnt) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= K) return;
+    const BInst& in = insts[k];
+    const int cpt = kBlock / presetGC(in.preset), rpt = kBlock / presetGR(in.preset);
+    const BCtrl& c = *in.ctrl;
+    const bool act = c.phase != kPhaseDone && c.round_iters > 0;
+    const bool p32 = act && c.phase == kPhaseFp32, p64 = act && c.phase != kPhaseFp32;
+    const int ct = (in.n + cpt - 1) / cpt, rt = (in.m + rpt - 1) / rpt;
+    cnt[0 * K + k] = p32 ? ct : 0;
+    cnt[1 * K + k] = p32 ? rt : 0;
+    cnt[2 * K + k] = p64 ? ct : 0;
+    cnt[3 * K + k] = p64 ? rt : 0;
+}
+
+// In-place exclusive scan of the four count arrays (one block), totals[4];
+// totals[4] = number of unfinished instances.
+__global__ __launch_bounds__(1024) void kScan4(int* __restrict__ cnt, int K,
+                                               const BInst* __restrict__ insts,
+                                               int* __restrict__ totals) {
+    __shared__ int part[1024];
+    __shared__ int carry;
+    for (int a = 0; a < 4; ++a) {
+        int* arr = cnt + static_cast<size_t>(a) * K;
+        if (threadIdx.x == 0) carry = 0;
+        __syncthreads();
+        for (int base = 0; base < K; base += 1024) {
+            const int k = base + threadIdx.x;
+            const int v = k < K ? arr[k] : 0;
+            part[threadIdx.x] = v;
+            __syncthreads();
+            for (int off = 1; off < 1024; off <<= 1) {
+                const int add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+                __syncthreads();
+                part[threadIdx.x] += add;
+                __syncthreads();
+            }
+            if (k < K) arr[k] = carry + part[threadIdx.x] - v;
+            __syncthreads();
+            if (threadIdx.x == 0) carry += part[1023];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) totals[a] = carry;
+        __syncthreads();
+    }
+    // Unfinished instances (order-independent integer count).
+    int act = 0;
+    for (int k = threadIdx.x; k < K; k += 1024) act += insts[k].ctrl->phase != kPhaseDone;
+    part[threadIdx.x] = act;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[4] = part[0];
+}
+
+__global__ void kTileFill(const BInst* __restrict__ insts, int K, const int* __restrict__ off,
+                          Tile* __restrict__ c32, Tile* __restrict__ r32, Tile* __restrict__ c64,
+                          Tile* __restrict__ r64) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= K) return;
+    const BInst& in = insts[k];
+    const int cpt = kBlock / presetGC(in.preset), rpt = kBlock / presetGR(in.preset);
+    const BCtrl& c = *in.ctrl;
+    const bool act = c.phase != kPhaseDone && c.round_iters > 0;
+    if (!act) return;
+    const bool p32 = c.phase == kPhaseFp32;
+    Tile* cl = p32 ? c32 : c64;
+    Tile* rl = p32 ? r32 : r64;
+    const int oc = off[(p32 ? 0 : 2) * K + k], orr = off[(p32 ? 1 : 3) * K + k];
+    const int ct = (in.n + cpt - 1) / cpt, rt = (in.m + rpt - 1) / rpt;
+    for (int t = 0; t < ct; ++t) cl[oc + t] = Tile{k, t * cpt};
+    for (int t = 0; t < rt; ++t) rl[orr + t] = Tile{k, t * rpt};
+}
+
+// ---- per-instance evaluation and control (one workgroup per instance) --------
+// Runs once per round after the round's iterations: the FP64 evaluation, the
+// termination / certificate / restart / primal-weight decisions of the
+// reference's loop (src/lp/pdlp.cpp:2228-2469) and the polish phase switch
+// (2477-2676); then sets the instance's iteration count for the next round.
+template <int GC, int GR, typename T>
+__device__ void evalControl(const BInst& in, const BOptions& o) {
+    BCtrl& c = *in.ctrl;
+    const int tid = threadIdx.x;
+    constexpr bool kFp32 = sizeof(T) == 4;
+    const BVecs<T>& v = pickVecs<T>(in);
+    __shared__ int s_act, s_phase, s_ran;
+    if (tid == 0) {
+        s_phase = c.phase;
+        s_ran = c.round_iters;
+    }
+    __syncthreads();
+    if (s_phase == kPhaseDone || kFp32 != (s_phase == kPhaseFp32) || s_ran <= 0) return;
+    if (!kFp32) {
+        // An instance handed over from FP32 earlier in this round has not
+        // iterated in FP64 yet.
+        __shared__ int s_skip;
+        if (tid == 0) {
+            s_skip = c.skip_eval;
+            c.skip_eval = 0;
+        }
+        __syncthreads();
+        if (s_skip) return;
+    }
+    const bool polish = s_phase == kPhasePolish;
+    const int ran = s_ran;
+    const BEval e = bEvaluate<GC, GR, T>(in, v, c);
+    if (tid == 0) {
+        c.inner += ran;
+        c.total += ran;
+        if (polish) c.pcount += ran;
+        if (kFp32) c.fp32_iters += ran;
+        if (c.fpe0 < 0.0) c.fpe0 = e.fpe;
+        c.rel_p = e.rel_p;
+        c.rel_d = e.rel_d;
+        c.rel_p_inf = e.rel_p_inf;
+        c.rel_d_inf = e.rel_d_inf;
+        c.gap = e.gap;
+        c.pobj = e.pobj;
+        c.dobj = e.dobj;
+        const double tp = o.norm_inf ? e.rel_p_inf : e.rel_p;
+        const double td = o.norm_inf ? e.rel_d_inf : e.rel_d;
+        int act = -1;
+        if (polish) {
+            if (tp <= o.polish_p_tol && td <= o.polish_d_tol) {
+                act = kActFinish;
+            } else {
+                bool restart;
+                if (c.pcount == ran) restart = true;
+                else if (c.fpe0 > 0.0 && e.fpe <= o.rs_suff * c.fpe0) restart = true;
+                else if (c.fpe0 > 0.0 && e.fpe <= o.rs_nec * c.fpe0 && e.fpe > c.last_fpe)
+                    restart = true;
+                else
+                    restart = c.inner >= static_cast<int>(o.rs_art * static_cast<double>(c.pcount));
+                c.last_fpe = e.fpe;
+                act = kActNone;
+                if (restart) {
+                    ++c.restarts;
+                    if (o.update_pw) updatePrimalWeight(c, e, o, true);
+                    act = kActRestart;
+                }
+            }
+        } else if (kFp32) {
+            // FP32 iterations judged in FP64: hand over to FP64 within a
+            // decade of the tolerances, or when FP32 stops making progress.
+            const double k = kktRatio(e, o);
+            bool handover = k <= kFp32ExitRatio;
+            if (k < 0.5 * c.best_kkt) {
+                c.best_kkt = k;
+                c.stall = 0;
+            } else if (++c.stall >= kFp32Stall) {
+                handover = true;
+            }
+            if (handover) act = kActToFp64;
+        } else if (tp <= (o.mixed ? o.refine_p_tol : o.primal_tol) &&
+                   td <= (o.mixed ? o.refine_d_tol : o.dual_tol) &&
+                   e.gap <= (o.mixed ? o.refine_g_tol : o.opt_tol)) {
+            act = kActConverged;
+        }
+        if (act == -1 && c.total >= 10 * o.N && c.total % (10 * o.N) == 0) act = kActCert;
+        s_act = act;
+    }
+    __syncthreads();
+    int act = s_act;
+    if (act == kActCert) {
+        const int verdict = bCertificates<GC, GR, T>(in, v, c, in.lo_o, in.hi_o, in.rl_o, in.ru_o,
+                                                     in.obj_o, in.sense_max);
+        if (tid == 0) {
+            if (verdict) {
+                c.phase = kPhaseDone;
+                c.status = verdict == 1 ? MPX_STATUS_INFEASIBLE : MPX_STATUS_UNBOUNDED;
+                c.round_iters = 0;
+                s_act = kActFinish + 100;
+            } else {
+                s_act = -1;
+            }
+        }
+        __syncthreads();
+        act = s_act;
+        if (act == kActFinish + 100) return;
+    }
+    if (act == -1) {
+        if (tid == 0) {
+            bool restart;
+            if (c.total == o.N) restart = true;
+            else if (c.fpe0 > 0.0 && e.fpe <= o.rs_suff * c.fpe0) restart = true;
+            else if (c.fpe0 > 0.0 && e.fpe <= o.rs_nec * c.fpe0 && e.fpe > c.last_fpe)
+                restart = true;
+            else
+                restart = c.inner >= static_cast<int>(o.rs_art * static_cast<double>(c.total));
+            c.last_fpe = e.fpe;
+            s_act = kActNone;
+            if (restart) {
+                ++c.restarts;
+                if (o.update_pw) updatePrimalWeight(c, e, o, false);
+                if (o.precond_refresh && c.precond_pw > 0.0 && o.do_pc) {
+                    const double drift = c.pw / c.precond_pw;
+                    if (drift > o.precond_ratio || drift < 1.0 / o.precond_ratio) c.precond_pw = c.pw;
+                }
+                s_act = kActRestart;
+            }
+        }
+        __syncthreads();
+        act = s_act;
+    }
+    if (act == kActConverged) {
+        const bool needPolish = o.enable_polish &&
+                                ((o.norm_inf ? e.rel_p_inf : e.rel_p) > o.polish_p_tol ||
+                                 (o.norm_inf ? e.rel_d_inf : e.rel_d) > o.polish_d_tol);
+        if (needPolish) {
+            for (int j = tid; j < in.n; j += kBlock) in.d.x0[j] = in.d.x[j] = in.d.xh[j];
+            for (int i = tid; i < in.m; i += kBlock) in.d.y0[i] = in.d.y[i] = in.d.yh[i];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            if (needPolish) {
+                c.phase = kPhasePolish;
+                c.inner = 0;
+                c.pcount = 0;
+                c.fpe0 = -1.0;
+                c.last_fpe = INFINITY;
+            } else {
+                c.phase = kPhaseDone;
+                c.status = MPX_STATUS_OPTIMAL;
+            }
+        }
+    } else if (act == kActFinish) {
+        if (tid == 0) {
+            c.phase = kPhaseDone;
+            c.status = MPX_STATUS_OPTIMAL;
+        }
+    } else if (act == kActToFp64) {
+        // Continue in FP64 from the FP32 point, restarted there.
+        for (int j = tid; j < in.n; j += kBlock) {
+            const double h = static_cast<double>(in.f.xh[j]);
+            in.d.xh[j] = h;
+            in.d.x[j] = h;
+            in.d.x0[j] = h;
+        }
+        for (int i = tid; i < in.m; i += kBlock) {
+            const double h = static_cast<double>(in.f.yh[i]);
+            in.d.yh[i] = h;
+            in.d.y[i] = h;
+            in.d.y0[i] = h;
+        }
+        if (tid == 0) {
+            c.phase = kPhaseFp64;
+            c.skip_eval = 1;
+            c.inner = 0;
+            c.fpe0 = -1.0;
+            c.last_fpe = INFINITY;
+            ++c.restarts;
+        }
+    } else if (act == kActRestart) {
+        for (int j = tid; j < in.n; j += kBlock) {
+            const T h = v.xh[j];
+            v.x[j] = h;
+            v.x0[j] = h;
+        }
+        for (int i = tid; i < in.m; i += kBlock) {
+            const T h = v.yh[i];
+            v.y[i] = h;
+            v.y0[i] = h;
+        }
+        if (tid == 0) {
+            c.inner = 0;
+            c.fpe0 = -1.0;
+            c.last_fpe = INFINITY;
+        }
+    }
+    __syncthreads();
+    // Iterations of the next round (the reference's batch = min(N, budget left)).
+    if (tid == 0 && c.phase != kPhaseDone) {
+        const bool pol = c.phase == kPhasePolish;
+        const int left = pol ? o.polish_max_iter - c.pcount : o.max_iter - c.total;
+        if (left <= 0) {
+            // Budget spent: polish keeps the converged answer, the main loop
+            // stops at the iteration limit.
+            c.phase = kPhaseDone;
+            c.status = pol ? MPX_STATUS_OPTIMAL : MPX_STATUS_ITER_LIMIT;
+            c.round_iters = 0;
+        } else {
+            c.round_iters = left < o.N ? left : o.N;
+        }
+    }
+    if (tid == 0 && c.phase == kPhaseDone) c.round_iters = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kEvalControl(const BInst* __restrict__ insts, int K,
+                                                        BOptions o) {
+    if (static_cast<int>(blockIdx.x) >= K) return;
+    const BInst& in = insts[blockIdx.x];
+    switch (in.preset) {
+        case 0: evalControl<1, 1, T>(in, o); break;
+        case 1: evalControl<2, 2, T>(in, o); break;
+        case 2: evalControl<2, 4, T>(in, o); break;
+        case 3: evalControl<4, 4, T>(in, o); break;
+        case 4: evalControl<4, 8, T>(in, o); break;
+        case 5: evalControl<8, 8, T>(in, o); break;
+        case 6: evalControl<8, 16, T>(in, o); break;
+        default: evalControl<16, 16, T>(in, o); break;
+    }
+}
+
+// Counts unfinished instances (one thread per instance, sum by atomics on an
+// int counter is order-independent and exact).
+__global__ void kCountActive(const BInst* __restrict__ insts, int count, int* __restrict__ out) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < count && insts[t].ctrl->phase != kPhaseDone) atomicAdd(out, 1);
+}
+
+// Marks unfinished instances with `status` (time limit / cancellation).
+__global__ void kAbortActive(const BInst* __restrict__ insts, int count, int status) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < count && insts[t].ctrl->phase != kPhaseDone) {
+        insts[t].ctrl->phase = kPhaseDone;
+        insts[t].ctrl->status = status;
+    }
+}
+
+// ---- setup kernels (one workgroup per distinct matrix) ---------------------
+struct BOwner {
+    int32_t m, n, nnz, do_ruiz, ruiz_iters, do_pc, do_rescale, norm_inf, sense_max, pad;
+    const int *rs, *ci, *cs, *ri, *perm;
+    const double *val_o, *obj_o, *lo_o, *hi_o, *rl_o, *ru_o;
+    double *val, *valt, *c, *l, *u, *rl, *ru, *row_scale, *col_scale, *cmax, *sig, *tau;
+    float *valf, *valtf, *cf, *tauf, *sigf, *rlf, *ruf;
+    double* scal;  // [os, cs, cnorm, bnorm, sigma_sq, power_iters]
+    // power iteration scratch (n, m, n)
+    double *px, *py, *pxn;
+    double offset;
+    int32_t sv_max_iter;
+    double sv_tol;
+};
+
+__global__ __launch_bounds__(kBlock) void kOwnerScale(const BOwner* __restrict__ owners) {
+    const BOwner w = owners[blockIdx.x];
+    const int tid = threadIdx.x;
+    const double sgn = w.sense_max ? -1.0 : 1.0;
+    for (long long k = tid; k < w.nnz; k += kBlock) w.val[k] = w.val_o[k];
+    for (int i = tid; i < w.m; i += kBlock) w.row_scale[i] = 1.0;
+    for (int j = tid; j < w.n; j += kBlock) w.col_scale[j] = 1.0;
+    __syncthreads();
+    if (w.do_ruiz && w.m > 0 && w.n > 0 && w.nnz > 0) {
+        for (int it = 0; it < w.ruiz_iters; ++it) {
+            for (int i = tid; i < w.m; i += kBlock) {
+                double rmax = 0.0;
+                for (int k = w.rs[i]; k < w.rs[i + 1]; ++k) rmax = maxRef(rmax, fabs(w.val[k]));
+                if (rmax <= 1e-12) continue;
+                const double f = 1.0 / sqrt(rmax);
+                w.row_scale[i] *= f;
+                for (int k = w.rs[i]; k < w.rs[i + 1]; ++k) w.val[k] *= f;
+            }
+            __syncthreads();
+            for (int j = tid; j < w.n; j += kBlock) {
+                double mx = 0.0;
+                for (int q = w.cs[j]; q < w.cs[j + 1]; ++q) mx = maxRef(mx, fabs(w.val[w.perm[q]]));
+                w.cmax[j] = mx;
+                if (mx > 1e-12) w.col_scale[j] *= 1.0 / sqrt(mx);
+            }
+            __syncthreads();
+            for (long long k = tid; k < w.nnz; k += kBlock) {
+                const double mx = w.cmax[w.ci[k]];
+                if (mx > 1e-12) w.val[k] *= 1.0 / sqrt(mx);
+            }
+            __syncthreads();
+        }
+    }
+    for (int j = tid; j < w.n; j += kBlock) {
+        const double cj = sgn * w.obj_o[j];
+        w.c[j] = cj * w.col_scale[j];
+        const double lj = w.lo_o[j], uj = w.hi_o[j];
+        w.l[j] = isfinite(lj) ? lj / w.col_scale[j] : lj;
+        w.u[j] = isfinite(uj) ? uj / w.col_scale[j] : uj;
+    }
+    for (int i = tid; i < w.m; i += kBlock) {
+        const double a = w.rl_o[i], b = w.ru_o[i];
+        w.rl[i] = isfinite(a) ? a * w.row_scale[i] : a;
+        w.ru[i] = isfinite(b) ? b * w.row_scale[i] : b;
+    }
+    __syncthreads();
+    for (long long k = tid; k < w.nnz; k += kBlock) w.valt[k] = w.val[w.perm[k]];
+    for (int i = tid; i < w.m; i += kBlock) w.sig[i] = 1.0;
+    for (int j = tid; j < w.n; j += kBlock) w.tau[j] = 1.0;
+    __syncthreads();
+    if (w.do_pc && w.m > 0 && w.n > 0) {
+        for (int i = tid; i < w.m; i += kBlock) {
+            double s = 0.0;
+            for (int k = w.rs[i]; k < w.rs[i + 1]; ++k) s += w.val[k] * w.val[k];
+            w.sig[i] = 1.0 / maxRef(sqrt(s), 1e-8);
+        }
+        for (int j = tid; j < w.n; j += kBlock) {
+            double s = 0.0;
+            for (int q = w.cs[j]; q < w.cs[j + 1]; ++q) s += w.valt[q] * w.valt[q];
+            w.tau[j] = 1.0 / maxRef(sqrt(s), 1e-8);
+        }
+    }
+    __syncthreads();
+    double os = 1.0, cs = 1.0;
+    if (w.do_rescale && w.m > 0 && w.n > 0) {
+        double cq = 0.0, bq = 0.0;
+        for (int j = tid; j < w.n; j += kBlock) cq += w.c[j] * w.c[j];
+        for (int i = tid; i < w.m; i += kBlock) {
+            if (isfinite(w.rl[i])) bq += w.rl[i] * w.rl[i];
+            if (isfinite(w.ru[i])) bq += w.ru[i] * w.ru[i];
+        }
+        cq = blockSum1(cq);
+        bq = blockSum1(bq);
+        os = 1.0 / (sqrt(cq) + 1.0);
+        cs = 1.0 / (sqrt(bq) + 1.0);
+        for (int j = tid; j < w.n; j += kBlock) {
+            w.c[j] *= os;
+            if (isfinite(w.l[j])) w.l[j] *= cs;
+            if (isfinite(w.u[j])) w.u[j] *= cs;
+        }
+        for (int i = tid; i < w.m; i += kBlock) {
+            if (isfinite(w.rl[i])) w.rl[i] *= cs;
+            if (isfinite(w.ru[i])) w.ru[i] *= cs;
+        }
+    }
+    // Termination norms of the original problem (pdlp.cpp:53-81).
+    double cn = 0.0, bn = 0.0;
+    if (w.norm_inf) {
+        for (int j = tid; j < w.n; j += kBlock) cn = maxRef(cn, fabs(w.obj_o[j]));
+        for (int i = tid; i < w.m; i += kBlock) {
+            const double lo = w.rl_o[i], hi = w.ru_o[i];
+            if (isfinite(hi)) bn = maxRef(bn, fabs(hi));
+            if (isfinite(lo) && lo != hi) bn = maxRef(bn, fabs(lo));
+        }
+        double a[2] = {cn, bn}, o2[2];
+        const bool mx[2] = {true, true};
+        blockReduce<2>(a, mx, o2);
+        __shared__ double sb[2];
+        if (tid == 0) {
+            sb[0] = o2[0];
+            sb[1] = o2[1];
+        }
+        __syncthreads();
+        cn = sb[0];
+        bn = sb[1];
+    } else {
+        for (int j = tid; j < w.n; j += kBlock) cn += w.obj_o[j] * w.obj_o[j];
+        for (int i = tid; i < w.m; i += kBlock) {
+            const double lo = w.rl_o[i], hi = w.ru_o[i];
+            if (isfinite(hi)) bn += hi * hi;
+            if (isfinite(lo) && lo != hi) bn += lo * lo;
+        }
+        cn = sqrt(blockSum1(cn));
+        bn = sqrt(blockSum1(bn));
+    }
+    __syncthreads();
+    // FP32 copies of the iteration data.
+    for (long long k = tid; k < w.nnz; k += kBlock) {
+        w.valf[k] = static_cast<float>(w.val[k]);
+        w.valtf[k] = static_cast<float>(w.valt[k]);
+    }
+    for (int j = tid; j < w.n; j += kBlock) {
+        w.cf[j] = static_cast<float>(w.c[j]);
+        w.tauf[j] = static_cast<float>(w.tau[j]);
+    }
+    for (int i = tid; i < w.m; i += kBlock) {
+        w.sigf[i] = static_cast<float>(w.sig[i]);
+        w.rlf[i] = static_cast<float>(w.rl[i]);
+        w.ruf[i] = static_cast<float>(w.ru[i]);
+    }
+    if (tid == 0) {
+        w.scal[0] = os;
+        w.scal[1] = cs;
+        w.scal[2] = cn;
+        w.scal[3] = bn;
+    }
+}
+
+// Power iteration of one owner (pdlp.cpp:748-795): px holds the normalized
+// start vector on entry.
+template <int GC, int GR>
+__device__ void ownerPower(const BOwner& w) {
+    const int tid = threadIdx.x;
+    double s2 = 1.0;
+    int it = 0;
+    for (; it < w.sv_max_iter; ++it) {
+        for (int base = 0; base < w.m; base += kBlock / GR) {
+            const int i = base + tid / GR, lane = tid % GR;
+            if (i < w.m) {
+                const double a = bDot<GR>(w.rs, w.ci, w.val, w.px, i, lane);
+                if (lane == 0) w.py[i] = a;
+            }
+        }
+        __syncthreads();
+        double dot = 0.0;
+        for (int base = 0; base < w.n; base += kBlock / GC) {
+            const int j = base + tid / GC, lane = tid % GC;
+            if (j < w.n) {
+                const double a = bDot<GC>(w.cs, w.ri, w.valt, w.py, j, lane);
+                if (lane == 0) {
+                    w.pxn[j] = a;
+                    dot += w.px[j] * a;
+                }
+            }
+        }
+        s2 = blockSum1(dot);
+        double r2 = 0.0, xq = 0.0;
+        for (int j = tid; j < w.n; j += kBlock) {
+            const double d = w.pxn[j] - s2 * w.px[j];
+            r2 += d * d;
+            xq += w.pxn[j] * w.pxn[j];
+        }
+        const double res = sqrt(blockSum1(r2));
+        const double xn = sqrt(blockSum1(xq));
+        if (xn < 1e-15) break;
+        for (int j = tid; j < w.n; j += kBlock) w.px[j] = w.pxn[j] / xn;
+        __syncthreads();
+        if (res < w.sv_tol * fabs(s2)) break;
+    }
+    if (tid == 0) {
+        w.scal[4] = s2;
+        w.scal[5] = static_cast<double>(it < w.sv_max_iter ? it + 1 : w.sv_max_iter);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kOwnerPower(const BOwner* __restrict__ owners,
+                                                      const int* __restrict__ presets) {
+    const BOwner w = owners[blockIdx.x];
+    if (w.m == 0 || w.n == 0) {
+        if (threadIdx.x == 0) {
+            w.scal[4] = 1.0;
+            w.scal[5] = 0.0;
+        }
+        return;
+    }
+    switch (presets[blockIdx.x]) {
+        case 0: ownerPower<1, 1>(w); break;
+        case 1: ownerPower<2, 2>(w); break;
+        case 2: ownerPower<2, 4>(w); break;
+        case 3: ownerPower<4, 4>(w); break;
+        case 4: ownerPower<4, 8>(w); break;
+        case 5: ownerPower<8, 8>(w); break;
+        case 6: ownerPower<8, 16>(w); break;
+        default: ownerPower<16, 16>(w); break;
+    }
+}
+
+// Per-instance initialisation: scaled column bounds (its own bounds when the
+// instance overrides them), FP32 copies, zero iterates, control state.
+struct BInitArgs {
+    const double* lo_o;  // instance original column bounds
+    const double* hi_o;
+    double* l;           // instance scaled fp64 bounds (may alias the owner's)
+    double* u;
+    float* lf;
+    float* uf;
+    const double* col_scale;
+    const double* scal;  // owner scal
+    int n, m, own_bounds, mixed;
+    double offset, sgn;
+};
+
+__global__ __launch_bounds__(kBlock) void kInstInit(const BInst* __restrict__ insts,
+                                                    const BInitArgs* __restrict__ args,
+                                                    const BOptions o, double primal_weight) {
+    const BInst in = insts[blockIdx.x];
+    const BInitArgs a = args[blockIdx.x];
+    const int tid = threadIdx.x;
+    const double cs = a.scal[1];
+    for (int j = tid; j < a.n; j += kBlock) {
+        if (a.own_bounds) {
+            const double lj = a.lo_o[j], uj = a.hi_o[j];
+            double l = isfinite(lj) ? lj / a.col_scale[j] : lj;
+            double u = isfinite(uj) ? uj / a.col_scale[j] : uj;
+            if (isfinite(l)) l *= cs;
+            if (isfinite(u)) u *= cs;
+            a.l[j] = l;
+            a.u[j] = u;
+        }
+        a.lf[j] = static_cast<float>(a.l[j]);
+        a.uf[j] = static_cast<float>(a.u[j]);
+        in.d.x[j] = in.d.x0[j] = in.d.xh[j] = in.d.xr[j] = 0.0;
+        in.f.x[j] = in.f.x0[j] = in.f.xh[j] = in.f.xr[j] = 0.0f;
+    }
+    for (int i = tid; i < a.m; i += kBlock) {
+        in.d.y[i] = in.d.y0[i] = in.d.yh[i] = 0.0;
+        in.f.y[i] = in.f.y0[i] = in.f.yh[i] = 0.0f;
+    }
+    if (tid == 0) {
+        BCtrl& c = *in.ctrl;
+        c.step = 0.998 / sqrt(maxRef(a.scal[4], 1e-12));
+        c.pw = primal_weight;
+        c.fpe0 = -1.0;
+        c.last_fpe = INFINITY;
+        c.pw_err_sum = 0.0;
+        c.pw_last_err = 0.0;
+        c.best_pw = primal_weight;
+        c.best_score = INFINITY;
+        c.precond_pw = primal_weight;
+        c.os = a.scal[0];
+        c.cs = a.scal[1];
+        c.cnorm = a.scal[2];
+        c.bnorm = a.scal[3];
+        c.sgn = a.sgn;
+        c.offset = a.offset;
+        c.best_kkt = INFINITY;
+        c.rel_p = c.rel_d = c.rel_p_inf = c.rel_d_inf = c.gap = c.pobj = c.dobj = 0.0;
+        c.inner = c.total = c.pcount = 0;
+        c.phase = a.mixed ? kPhaseFp32 : kPhaseFp64;
+        c.status = MPX_STATUS_ITER_LIMIT;
+        c.restarts = c.fp32_iters = c.stall = 0;
+        c.power_iters = static_cast<int>(a.scal[5]);
+        c.g_col = presetGC(in.preset);
+        c.g_row = presetGR(in.preset);
+        c.use_lds = 0;
+        c.skip_eval = 0;
+        c.round_iters = o.max_iter < o.N ? o.max_iter : o.N;
+        if (c.round_iters <= 0) {
+            c.phase = kPhaseDone;
+            c.round_iters = 0;
+        }
+    }
+}
+
+// Recovery into original space (pdlp.cpp:1677-1697) into per-instance
+// device output buffers.
+__global__ void kInstRecover(const BInst* __restrict__ insts, double* const* __restrict__ xo,
+                             double* const* __restrict__ yo) {
+    const BInst& in = insts[blockIdx.x];
+    const BCtrl& c = *in.ctrl;
+    const int tid = threadIdx.x;
+    const double inv_cs = 1.0 / c.cs, inv_os = 1.0 / c.os;
+    const bool f32 = c.fp32_iters == c.total && c.total > 0;  // never left FP32
+    const bool clamp = c.status != MPX_STATUS_UNBOUNDED;
+    double* x = xo[blockIdx.x];
+    double* y = yo[blockIdx.x];
+    for (int j = tid; j < in.n; j += kBlock) {
+        const double h = f32 ? static_cast<double>(in.f.xh[j]) : in.d.xh[j];
+        double xj = h * in.col_scale[j] * inv_cs;
+        if (clamp) {
+            if (isfinite(in.lo_o[j])) xj = maxRef(xj, in.lo_o[j]);
+            if (isfinite(in.hi_o[j])) xj = minRef(xj, in.hi_o[j]);
+        }
+        x[j] = xj;
+    }
+    for (int i = tid; i < in.m; i += kBlock) {
+        const double h = f32 ? static_cast<double>(in.f.yh[i]) : in.d.yh[i];
+        y[i] = c.sgn * h * in.row_scale[i] * inv_os;
+    }
+}
+
+// ---- host engine ----------------------------------------------------------------
+void hostTranspose(int m, int n, const int* rs, const int* ci, std::vector<int>& cs,
+                   std::vector<int>& ri, std::vector<int>& perm) {
+    const int nnz = rs[m];
+    cs.assign(static_cast<size_t>(n) + 1, 0);
+    ri.resize(static_cast<size_t>(nnz));
+    perm.resize(static_cast<size_t>(nnz));
+    for (int k = 0; k < nnz; ++k) ++cs[ci[k] + 1];
+    for (int j = 0; j < n; ++j) cs[j + 1] += cs[j];
+    std::vector<int> next(cs.begin(), cs.end() - 1);
+    for (int i = 0; i < m; ++i)
+        for (int k = rs[i]; k < rs[i + 1]; ++k) {
+            const int dst = next[ci[k]]++;
+            ri[dst] = i;
+            perm[dst] = k;
+        }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) {
+        if (bytes) MPX_HIP(hipMalloc(&p, bytes));
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Bump allocator over one device allocation.
+struct Carve {
+    char* base;
+    size_t used = 0, cap;
+    template <typename T>
+    T* take(size_t count) {
+        const size_t b = (count * sizeof(T) + 255) & ~size_t(255);
+        if (used + b > cap) {
+            setError("internal: batch arena overflow");
+            throw DeviceError();
+        }
+        T* p = reinterpret_cast<T*>(base + used);
+        used += b;
+        return p;
+    }
+};
+inline size_t pad(size_t count, size_t elt) { return (count * elt + 255) & ~size_t(255); }
+
+size_t ownerBytes(int m, int n, int nnz) {
+    const size_t M = m, N = n, Z = nnz;
+    return pad(M + 1, 4) + 3 * pad(Z, 4) + pad(N + 1, 4) + pad(Z, 4)  // rs ci ri perm cs
+           + 3 * pad(Z, 8)                                          // val_o val valt
+           + 6 * pad(N, 8) + 6 * pad(M, 8)                          // obj lo hi c l u cmax, rl ru rl_o ru_o rs sig ..
+           + 3 * pad(N, 8) + 2 * pad(M, 8)                          // col_scale tau + power px pxn, py
+           + 2 * pad(Z, 4) + 4 * pad(N, 4) + 4 * pad(M, 4)          // fp32 copies
+           + pad(8, 8) + 4096;
+}
+
+size_t instBytes(int m, int n, bool own_bounds) {
+    const size_t M = m, N = n;
+    size_t b = 4 * pad(N, 8) + 3 * pad(M, 8) + 4 * pad(N, 4) + 3 * pad(M, 4);  // iterates
+    b += 2 * pad(N, 4);                                                     // fp32 bounds
+    if (own_bounds) b += 4 * pad(N, 8);                                     // lo_o hi_o l u
+    b += pad(1, sizeof(BCtrl)) + 1024;
+    return b;
+}
+
+// Iteration presets the online autotuner tries: (lanes per column, lanes per
+// row) of the packed iteration kernels.
+struct IterPreset {
+    int gc, gr;
+};
+
+class BatchEngine {
+public:
+    explicit BatchEngine(const mpx_pdlp_options& o) : o_(o) {
+        MPX_HIP(hipSetDevice(o.device));
+        MPX_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        MPX_HIP(hipEventCreate(&ev0_));
+        MPX_HIP(hipEventCreate(&ev1_));
+        bo_.max_iter = o.max_iter;
+        bo_.N = o.termination_eval_frequency;
+        bo_.polish_max_iter = o.polish_max_iter;
+        bo_.norm_inf = o.optimality_norm == 1;
+        bo_.update_pw = o.update_primal_weight;
+        bo_.enable_polish = o.enable_polishing;
+        bo_.precond_refresh = o.preconditioner_refresh;
+        bo_.do_pc = o.do_pock_chambolle_scaling;
+        bo_.primal_tol = o.primal_tol;
+        bo_.dual_tol = o.dual_tol;
+        bo_.opt_tol = o.optimality_tol;
+        bo_.polish_p_tol = o.polish_primal_tol;
+        bo_.polish_d_tol = o.polish_dual_tol;
+        bo_.pid_kp = o.pid_kp;
+        bo_.pid_ki = o.pid_ki;
+        bo_.pid_kd = o.pid_kd;
+        bo_.pid_i_smooth = o.pid_i_smooth;
+        bo_.rs_suff = o.restart_sufficient_decay;
+        bo_.rs_nec = o.restart_necessary_decay;
+        bo_.rs_art = o.restart_artificial_fraction;
+        bo_.precond_ratio = o.preconditioner_refresh_ratio;
+        mixed_ = o.precision == MPX_PREC_MIXED;
+        bo_.mixed = mixed_;
+        // FP64 phase of mixed precision: refine_tol, or the solve's own tolerances.
+        bo_.refine_p_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.primal_tol) : o.primal_tol;
+        bo_.refine_d_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.dual_tol) : o.dual_tol;
+        // The gap target is halved: the reference's relative gap divides by
+        // 1 + |pobj| + |dobj| (about twice |objective|), so gap <= tol/2 keeps
+        // the objective within tol of the optimum.
+        bo_.refine_g_tol =
+            o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol) : 0.5 * o.optimality_tol;
+    }
+    ~BatchEngine() {
+        for (auto& kv : graphs_) kv.second.reset();
+        if (ev0_) (void)hipEventDestroy(ev0_);
+        if (ev1_) (void)hipEventDestroy(ev1_);
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+
+    // Registers a distinct LP (matrix owner).  Returns its id.
+    int addOwner(const mpx_lp& lp) {
+        owners_host_.push_back(&lp);
+        return static_cast<int>(owners_host_.size()) - 1;
+    }
+    // Registers an instance of owner `ow` (col bounds override or null).
+    int addInstance(int ow, const double* lo, const double* hi) {
+        inst_owner_.push_back(ow);
+        inst_lo_.push_back(lo);
+        inst_hi_.push_back(hi);
+        return static_cast<int>(inst_owner_.size()) - 1;
+    }
+
+    void setup();
+    // Runs rounds until every instance is done (or the time limit / stop).
+    void run(std::chrono::steady_clock::time_point t0);
+    void results(double* const* x_out, double* const* y_out, mpx_pdlp_result* res);
+    int count() const { return static_cast<int>(inst_owner_.size()); }
+    double setupSeconds() const { return setup_seconds_; }
+    // Shared-matrix batches may use the lane-per-instance kernels.
+    void allowInterleaved(bool on) { allow_interleaved_ = on; }
+    // Runs `rounds` rounds; returns the number of unfinished instances.
+    int step(int rounds);
+    // Statistics of the iteration kernels (HIP events around every round's
+    // iteration part): total milliseconds, rounds timed, instance-iterations.
+    double iterMs() const { return iter_ms_; }
+    int roundsTimed() const { return rounds_timed_; }
+    IterPreset preset() const { return cur_; }
+
+private:
+    struct G {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        void reset() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+            e = nullptr;
+            g = nullptr;
+        }
+    };
+    void enqueueRound(const IterPreset& p, bool timed);
+    void launchRound(const IterPreset& p, bool timed);
+    std::vector<IterPreset> candidates() const;
+
+    mpx_pdlp_options o_;
+    BOptions bo_{};
+    bool mixed_ = false;
+    hipStream_t st_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    std::vector<const mpx_lp*> owners_host_;
+    std::vector<int> inst_owner_;
+    std::vector<const double*> inst_lo_, inst_hi_;
+    std::unique_ptr<DevBuf> arena_;
+    std::vector<BOwner> owners_;
+    std::vector<BInst> insts_;
+    std::vector<int> owner_preset_;
+    std::unique_ptr<DevBuf> d_insts_buf_, d_owners_buf_, d_presets_buf_, d_args_buf_, d_count_buf_;
+    std::unique_ptr<DevBuf> d_tiles_buf_, d_cnt_buf_;
+    Tile *tc32_ = nullptr, *tr32_ = nullptr, *tc64_ = nullptr, *tr64_ = nullptr;
+    int* totals_ = nullptr;  // [0..3] tile counts, [4] unfinished instances
+    size_t max_col_tiles_ = 0, max_row_tiles_ = 0;
+    int grid_ = 2048;
+    std::map<std::pair<int, int>, G> graphs_;
+    bool graph_failed_ = false;
+    IterPreset cur_{4, 8};
+    std::vector<IterPreset> tune_left_;
+    std::vector<std::pair<double, IterPreset>> tune_res_;
+    double iter_ms_ = 0.0;
+    int rounds_timed_ = 0;
+    double setup_seconds_ = 0.0;
+    BCtrl* ctrl_all_ = nullptr;
+    bool allow_interleaved_ = false;
+    // Lane-per-instance mode (every instance shares owner 0).
+    bool interleaved_ = false;
+    std::unique_ptr<DevBuf> d_il_buf_;
+    template <typename T>
+    struct IlBufs {
+        T *x = nullptr, *x0 = nullptr, *xr = nullptr, *l = nullptr, *u = nullptr, *y = nullptr,
+          *y0 = nullptr;
+    };
+    IlBufs<float> il32_;
+    IlBufs<double> il64_;
+    int *slots32_ = nullptr, *slots64_ = nullptr, *slot_counts_ = nullptr;
+    ILane *lanes32_ = nullptr, *lanes64_ = nullptr;
+    void enqueueInterleavedRound(bool timed);
+};
+
+void BatchEngine::setup() {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int no = static_cast<int>(owners_host_.size());
+    const int ni = count();
+    // Which owners have per-instance bound overrides.
+    size_t bytes = 0;
+    for (int w = 0; w < no; ++w) {
+        const mpx_lp& lp = *owners_host_[w];
+        bytes += ownerBytes(lp.num_rows, lp.num_cols, lp.row_starts[lp.num_rows]);
+    }
+    for (int i = 0; i < ni; ++i) {
+        const mpx_lp& lp = *owners_host_[inst_owner_[i]];
+        bytes += instBytes(lp.num_rows, lp.num_cols, inst_lo_[i] != nullptr);
+    }
+    arena_ = std::make_unique<DevBuf>(bytes);
+    Carve cv{static_cast<char*>(arena_->p), 0, bytes};
+    owners_.resize(static_cast<size_t>(no));
+    owner_preset_.resize(static_cast<size_t>(no));
+    std::vector<std::vector<double>> starts(static_cast<size_t>(no));
+    int max_need = 0;
+    for (int w = 0; w < no; ++w) {
+        const mpx_lp& lp = *owners_host_[w];
+        const int m = lp.num_rows, n = lp.num_cols, nnz = lp.row_starts[m];
+        BOwner& b = owners_[w];
+        std::memset(&b, 0, sizeof(b));
+        b.m = m;
+        b.n = n;
+        b.nnz = nnz;
+        b.do_ruiz = o_.do_ruiz_scaling;
+        b.ruiz_iters = o_.ruiz_iterations;
+        b.do_pc = o_.do_pock_chambolle_scaling;
+        b.do_rescale = o_.do_bound_obj_rescaling;
+        b.norm_inf = o_.optimality_norm == 1;
+        b.sense_max = lp.sense != 0;
+        b.offset = lp.obj_offset;
+        b.sv_max_iter = o_.sv_max_iter;
+        b.sv_tol = o_.sv_tol;
+        int* rs = cv.take<int>(m + 1);
+        int* ci = cv.take<int>(nnz);
+        int* ri = cv.take<int>(nnz);
+        int* perm = cv.take<int>(nnz);
+        int* cs = cv.take<int>(n + 1);
+        double* val_o = cv.take<double>(nnz);
+        b.val = cv.take<double>(nnz);
+        b.valt = cv.take<double>(nnz);
+        double* obj_o = cv.take<double>(n);
+        double* lo_o = cv.take<double>(n);
+        double* hi_o = cv.take<double>(n);
+        b.c = cv.take<double>(n);
+        b.l = cv.take<double>(n);
+        b.u = cv.take<double>(n);
+        b.cmax = cv.take<double>(n);
+        b.col_scale = cv.take<double>(n);
+        b.tau = cv.take<double>(n);
+        b.px = cv.take<double>(n);
+        b.pxn = cv.take<double>(n);
+        double* rl_o = cv.take<double>(m);
+        double* ru_o = cv.take<double>(m);
+        b.rl = cv.take<double>(m);
+        b.ru = cv.take<double>(m);
+        b.row_scale = cv.take<double>(m);
+        b.sig = cv.take<double>(m);
+        b.py = cv.take<double>(m);
+        b.valf = cv.take<float>(nnz);
+        b.valtf = cv.take<float>(nnz);
+        b.cf = cv.take<float>(n);
+        b.tauf = cv.take<float>(n);
+        b.sigf = cv.take<float>(m);
+        b.rlf = cv.take<float>(m);
+        b.ruf = cv.take<float>(m);
+        b.scal = cv.take<double>(8);
+        std::vector<int> hcs, hri, hperm;
+        hostTranspose(m, n, lp.row_starts, lp.col_indices, hcs, hri, hperm);
+        MPX_HIP(hipMemcpyAsync(rs, lp.row_starts, (m + 1) * 4ull, hipMemcpyHostToDevice, st_));
+        if (nnz) {
+            MPX_HIP(hipMemcpyAsync(ci, lp.col_indices, nnz * 4ull, hipMemcpyHostToDevice, st_));
+            MPX_HIP(hipMemcpyAsync(ri, hri.data(), nnz * 4ull, hipMemcpyHostToDevice, st_));
+            MPX_HIP(hipMemcpyAsync(perm, hperm.data(), nnz * 4ull, hipMemcpyHostToDevice, st_));
+            MPX_HIP(hipMemcpyAsync(val_o, lp.values, nnz * 8ull, hipMemcpyHostToDevice, st_));
+        }
+        MPX_HIP(hipMemcpyAsync(cs, hcs.data(), (n + 1) * 4ull, hipMemcpyHostToDevice, st_));
+        MPX_HIP(hipMemcpyAsync(obj_o, lp.obj, n * 8ull, hipMemcpyHostToDevice, st_));
+        MPX_HIP(hipMemcpyAsync(lo_o, lp.col_lower, n * 8ull, hipMemcpyHostToDevice, st_));
+        MPX_HIP(hipMemcpyAsync(hi_o, lp.col_upper, n * 8ull, hipMemcpyHostToDevice, st_));
+        if (m) {
+            MPX_HIP(hipMemcpyAsync(rl_o, lp.row_lower, m * 8ull, hipMemcpyHostToDevice, st_));
+            MPX_HIP(hipMemcpyAsync(ru_o, lp.row_upper, m * 8ull, hipMemcpyHostToDevice, st_));
+        }
+        b.rs = rs;
+        b.ci = ci;
+        b.cs = cs;
+        b.ri = ri;
+        b.perm = perm;
+        b.val_o = val_o;
+        b.obj_o = obj_o;
+        b.lo_o = lo_o;
+        b.hi_o = hi_o;
+        b.rl_o = rl_o;
+        b.ru_o = ru_o;
+        owner_preset_[w] = o_.spmv_group > 0 ? 0 : heuristicPreset(nnz, m, n);
+        if (o_.spmv_group > 0) {
+            // Forced group: the preset with that many lanes per column.
+            int best = 0;
+            for (int p = 0; p < kNumPresets; ++p)
+                if (presetGC(p) <= o_.spmv_group) best = p;
+            owner_preset_[w] = o_.spmv_group == 1 ? 0 : best;
+        }
+        // Power-iteration start vector, exactly as the reference (pdlp.cpp:756-764).
+        std::mt19937 rng(42);
+        std::normal_distribution<double> dist(0.0, 1.0);
+        std::vector<double>& xs = starts[w];
+        xs.resize(static_cast<size_t>(n));
+        for (double& v : xs) v = dist(rng);
+        double s = 0.0;
+        for (double v : xs) s += v * v;
+        double xn = std::sqrt(s);
+        if (xn < 1e-15) xn = 1.0;
+        for (double& v : xs) v /= xn;
+        MPX_HIP(hipMemcpyAsync(b.px, xs.data(), n * 8ull, hipMemcpyHostToDevice, st_));
+        const int need = (m + n) * (mixed_ ? 4 : 8);
+        max_need = std::max(max_need, need);
+    }
+    // Instances.
+    insts_.resize(static_cast<size_t>(ni));
+    ctrl_all_ = cv.take<BCtrl>(static_cast<size_t>(std::max(1, ni)));
+    BCtrl* ctrl_all = ctrl_all_;
+    std::vector<BInitArgs> args(static_cast<size_t>(ni));
+    for (int i = 0; i < ni; ++i) {
+        const BOwner& w = owners_[inst_owner_[i]];
+        const mpx_lp& lp = *owners_host_[inst_owner_[i]];
+        const int m = w.m, n = w.n;
+        BInst& in = insts_[i];
+        std::memset(&in, 0, sizeof(in));
+        in.m = m;
+        in.n = n;
+        in.nnz = w.nnz;
+        in.sense_max = w.sense_max;
+        in.owner = inst_owner_[i];
+        in.preset = owner_preset_[inst_owner_[i]];
+        in.rs = w.rs;
+        in.ci = w.ci;
+        in.cs = w.cs;
+        in.ri = w.ri;
+        in.rl_o = w.rl_o;
+        in.ru_o = w.ru_o;
+        in.obj_o = w.obj_o;
+        in.val = w.val;
+        in.valt = w.valt;
+        in.c = w.c;
+        in.rl = w.rl;
+        in.ru = w.ru;
+        in.row_scale = w.row_scale;
+        in.col_scale = w.col_scale;
+        in.sig = w.sig;
+        in.tau = w.tau;
+        BInitArgs& a = args[i];
+        a.n = n;
+        a.m = m;
+        a.col_scale = w.col_scale;
+        a.scal = w.scal;
+        a.mixed = mixed_;
+        a.offset = lp.obj_offset;
+        a.sgn = lp.sense ? -1.0 : 1.0;
+        if (inst_lo_[i]) {
+            double* lo_o = cv.take<double>(n);
+            double* hi_o = cv.take<double>(n);
+            MPX_HIP(hipMemcpyAsync(lo_o, inst_lo_[i], n * 8ull, hipMemcpyHostToDevice, st_));
+            MPX_HIP(hipMemcpyAsync(hi_o, inst_hi_[i], n * 8ull, hipMemcpyHostToDevice, st_));
+            in.lo_o = lo_o;
+            in.hi_o = hi_o;
+            double* l = cv.take<double>(n);
+            double* u = cv.take<double>(n);
+            in.l = l;
+            in.u = u;
+            a.own_bounds = 1;
+            a.lo_o = lo_o;
+            a.hi_o = hi_o;
+            a.l = l;
+            a.u = u;
+        } else {
+            in.lo_o = w.lo_o;
+            in.hi_o = w.hi_o;
+            in.l = w.l;
+            in.u = w.u;
+            a.own_bounds = 0;
+            a.l = w.l;
+            a.u = w.u;
+        }
+        float* lf = cv.take<float>(n);
+        float* uf = cv.take<float>(n);
+        a.lf = lf;
+        a.uf = uf;
+        auto fillVecs = [&](auto& v, auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            v.x = cv.take<T>(n);
+            v.x0 = cv.take<T>(n);
+            v.xr = cv.take<T>(n);
+            v.xh = cv.take<T>(n);
+            v.y = cv.take<T>(m);
+            v.y0 = cv.take<T>(m);
+            v.yh = cv.take<T>(m);
+        };
+        fillVecs(in.d, static_cast<double*>(nullptr));
+        fillVecs(in.f, static_cast<float*>(nullptr));
+        in.d.val = w.val;
+        in.d.valt = w.valt;
+        in.d.c = w.c;
+        in.d.tau = w.tau;
+        in.d.l = in.l;
+        in.d.u = in.u;
+        in.d.sig = w.sig;
+        in.d.rl = w.rl;
+        in.d.ru = w.ru;
+        in.f.val = w.valf;
+        in.f.valt = w.valtf;
+        in.f.c = w.cf;
+        in.f.tau = w.tauf;
+        in.f.l = lf;
+        in.f.u = uf;
+        in.f.sig = w.sigf;
+        in.f.rl = w.rlf;
+        in.f.ru = w.ruf;
+        in.ctrl = ctrl_all + i;
+    }
+    // Tile lists for the packed kernels, sized for the smallest tiles (16
+    // columns or rows per tile at 16 lanes each).
+    max_col_tiles_ = 0;
+    max_row_tiles_ = 0;
+    for (int i = 0; i < ni; ++i) {
+        max_col_tiles_ += static_cast<size_t>((insts_[i].n + 15) / 16);
+        max_row_tiles_ += static_cast<size_t>((insts_[i].m + 15) / 16);
+    }
+    {
+        const size_t tb = 2 * (max_col_tiles_ + max_row_tiles_) * sizeof(Tile) + 16 * sizeof(int);
+        d_tiles_buf_ = std::make_unique<DevBuf>(tb);
+        Tile* t = static_cast<Tile*>(d_tiles_buf_->p);
+        tc32_ = t;
+        tr32_ = tc32_ + max_col_tiles_;
+        tc64_ = tr32_ + max_row_tiles_;
+        tr64_ = tc64_ + max_col_tiles_;
+        totals_ = reinterpret_cast<int*>(tr64_ + max_row_tiles_);
+        d_cnt_buf_ = std::make_unique<DevBuf>(sizeof(int) * 4 * static_cast<size_t>(std::max(1, ni)));
+    }
+    // Lane-per-instance mode when every instance shares one matrix.
+    interleaved_ = allow_interleaved_ && no == 1 && ni >= 1;
+    if (interleaved_) {
+        const size_t ng = static_cast<size_t>((ni + kLanes - 1) / kLanes);
+        const size_t N = static_cast<size_t>(owners_[0].n), M = static_cast<size_t>(owners_[0].m);
+        const size_t e64 = ng * kLanes * (5 * N + 2 * M);
+        const size_t bytes_il = e64 * sizeof(double) + (mixed_ ? e64 * sizeof(float) : 0) +
+                                2 * static_cast<size_t>(ni) * (sizeof(int) + sizeof(ILane)) + 4096;
+        d_il_buf_ = std::make_unique<DevBuf>(bytes_il);
+        Carve c{static_cast<char*>(d_il_buf_->p), 0, bytes_il};
+        auto carve = [&](auto& b, auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            b.x = c.take<T>(ng * kLanes * N);
+            b.x0 = c.take<T>(ng * kLanes * N);
+            b.xr = c.take<T>(ng * kLanes * N);
+            b.l = c.take<T>(ng * kLanes * N);
+            b.u = c.take<T>(ng * kLanes * N);
+            b.y = c.take<T>(ng * kLanes * M);
+            b.y0 = c.take<T>(ng * kLanes * M);
+        };
+        carve(il64_, static_cast<double*>(nullptr));
+        if (mixed_) carve(il32_, static_cast<float*>(nullptr));
+        slots32_ = c.take<int>(ni);
+        slots64_ = c.take<int>(ni);
+        lanes32_ = c.take<ILane>(ni);
+        lanes64_ = c.take<ILane>(ni);
+        slot_counts_ = c.take<int>(4);
+    }
+    int dev = 0, cus = 256;
+    MPX_HIP(hipGetDevice(&dev));
+    MPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    grid_ = std::max(kNumXcd, (cus * 8 / kNumXcd) * kNumXcd);
+    cur_ = {0, 0};
+    d_owners_buf_ = std::make_unique<DevBuf>(sizeof(BOwner) * std::max(1, no));
+    d_presets_buf_ = std::make_unique<DevBuf>(sizeof(int) * std::max(1, no));
+    d_insts_buf_ = std::make_unique<DevBuf>(sizeof(BInst) * std::max(1, ni));
+    d_args_buf_ = std::make_unique<DevBuf>(sizeof(BInitArgs) * std::max(1, ni));
+    d_count_buf_ = std::make_unique<DevBuf>(sizeof(int) * 4);
+    MPX_HIP(hipMemcpyAsync(d_owners_buf_->p, owners_.data(), sizeof(BOwner) * no,
+                           hipMemcpyHostToDevice, st_));
+    MPX_HIP(hipMemcpyAsync(d_presets_buf_->p, owner_preset_.data(), sizeof(int) * no,
+                           hipMemcpyHostToDevice, st_));
+    MPX_HIP(hipMemcpyAsync(d_insts_buf_->p, insts_.data(), sizeof(BInst) * ni,
+                           hipMemcpyHostToDevice, st_));
+    MPX_HIP(hipMemcpyAsync(d_args_buf_->p, args.data(), sizeof(BInitArgs) * ni,
+                           hipMemcpyHostToDevice, st_));
+    if (no) {
+        kOwnerScale<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p));
+        kOwnerPower<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p),
+                                            static_cast<const int*>(d_presets_buf_->p));
+    }
+    if (ni)
+        kInstInit<<<ni, kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
+                                          static_cast<const BInitArgs*>(d_args_buf_->p), bo_,
+                                          o_.primal_weight);
+    MPX_HIP(hipGetLastError());
+    MPX_HIP(hipStreamSynchronize(st_));
+    setup_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+std::vector<IterPreset> BatchEngine::candidates() const {
+    // Around the batch's mean entries per column / row: about two entries per
+    // lane, one size smaller and one larger.
+    long long nnz = 0, rows = 0, cols = 0;
+    for (const BInst& in : insts_) {
+        nnz += in.nnz;
+        rows += in.m;
+        cols += in.n;
+    }
+    const double ac = cols ? static_cast<double>(nnz) / cols : 1.0;
+    const double ar = rows ? static_cast<double>(nnz) / rows : 1.0;
+    auto base = [](double a) {
+        int g = 1;
+        while (g < 16 && 2.0 * g < a) g <<= 1;
+        return g;
+    };
+    const int bc = base(ac), br = base(ar);
+    std::vector<IterPreset> out;
+    out.push_back({bc, br});
+    if (bc > 1) out.push_back({bc / 2, br});
+    if (br > 1) out.push_back({bc, br / 2});
+    if (bc < 16) out.push_back({bc * 2, br});
+    if (br < 16) out.push_back({bc, br * 2});
+    return out;
+}
+
+template <typename F>
+void withG16(int g, F&& f) {
+    switch (g) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: f(std::integral_constant<int, 16>{}); break;
+    }
+}
+
+// One round: tile lists for the coming iterations, N iterations of the
+// packed kernels (FP32 and/or FP64), then evaluation and control.
+void BatchEngine::enqueueRound(const IterPreset& p, bool timed) {
+    (void)p;
+    const int K = count();
+    const BInst* di = static_cast<const BInst*>(d_insts_buf_->p);
+    int* cnt = static_cast<int*>(d_cnt_buf_->p);
+    const int gk = ceilDiv(K, kBlock);
+    kTileCounts<<<gk, kBlock, 0, st_>>>(di, K, cnt);
+    kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
+    kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
+    if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
+    for (int k = 1; k <= bo_.N; ++k) {
+        if (mixed_) {
+            kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
+            kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, tr32_, totals_ + 1, k);
+        }
+        kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, tc64_, totals_ + 2, k);
+        kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, tr64_, totals_ + 3, k);
+    }
+    if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
+    if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
+}
+
+void BatchEngine::enqueueInterleavedRound(bool timed) {
+    const int K = count();
+    const BInst* di = static_cast<const BInst*>(d_insts_buf_->p);
+    const BOwner& w = owners_[0];
+    const int n = w.n, m = w.m;
+    kSlotLists<<<1, 1024, 0, st_>>>(di, K, slots32_, slots64_, lanes32_, lanes64_, slot_counts_);
+    const int* c32 = slot_counts_;
+    const int* c64 = slot_counts_ + 1;
+    if (mixed_) {
+        kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, n, 0, il32_.x, il32_.x0,
+                                                      il32_.l, il32_.u);
+        kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, m, 1, il32_.y, il32_.y0,
+                                                      nullptr, nullptr);
+    }
+    kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, n, 0, il64_.x, il64_.x0,
+                                                   il64_.l, il64_.u);
+    kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, m, 1, il64_.y, il64_.y0,
+                                                   nullptr, nullptr);
+    if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
+    for (int k = 1; k <= bo_.N; ++k) {
+        if (mixed_) {
+            kILPrimal<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, lanes32_, n, m, w.cs,
+                                                        w.ri, w.valtf, w.cf, w.tauf, il32_.y,
+                                                        il32_.x, il32_.x0, il32_.xr, il32_.l,
+                                                        il32_.u, k);
+            kILDual<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, lanes32_, n, m, w.rs,
+                                                      w.ci, w.valf, w.sigf, w.rlf, w.ruf, il32_.xr,
+                                                      il32_.y, il32_.y0, k);
+        }
+        kILPrimal<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, lanes64_, n, m, w.cs, w.ri,
+                                                     w.valt, w.c, w.tau, il64_.y, il64_.x,
+                                                     il64_.x0, il64_.xr, il64_.l, il64_.u, k);
+        kILDual<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, lanes64_, n, m, w.rs, w.ci,
+                                                   w.val, w.sig, w.rl, w.ru, il64_.xr, il64_.y,
+                                                   il64_.y0, k);
+    }
+    if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
+    if (mixed_) {
+        kDeinterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, n, 0, il32_.x);
+        kDeinterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, m, 1, il32_.y);
+    }
+    kDeinterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, n, 0, il64_.x);
+    kDeinterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, m, 1, il64_.y);
+    if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
+}
+
+void BatchEngine::launchRound(const IterPreset& p, bool timed) {
+    if (timed || graph_failed_) {
+        if (interleaved_) enqueueInterleavedRound(timed);
+        else enqueueRound(p, timed);
+        MPX_HIP(hipGetLastError());
+        return;
+    }
+    G& g = graphs_[{p.gc, p.gr}];
+    if (g.e == nullptr) {
+        bool ok = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            if (interleaved_) enqueueInterleavedRound(false);
+            else enqueueRound(p, false);
+            ok = hipStreamEndCapture(st_, &g.g) == hipSuccess && g.g != nullptr &&
+                 hipGraphInstantiate(&g.e, g.g, nullptr, nullptr, 0) == hipSuccess;
+        }
+        if (!ok) {
+            (void)hipGetLastError();
+            g.reset();
+            graph_failed_ = true;
+            if (interleaved_) enqueueInterleavedRound(false);
+            else enqueueRound(p, false);
+            MPX_HIP(hipGetLastError());
+            return;
+        }
+    }
+    MPX_HIP(hipGraphLaunch(g.e, st_));
+}
+
+int BatchEngine::step(int rounds) {
+    const int K = count();
+    if (K == 0) return 0;
+    int* dcount = static_cast<int*>(d_count_buf_->p);
+    for (int r = 0; r < rounds; ++r) {
+        // Online autotuning: the first rounds try the candidate presets (all
+        // rounds do real work), timing the iteration part with HIP events.
+        if (!tune_left_.empty()) {
+            const IterPreset p = tune_left_.back();
+            tune_left_.pop_back();
+            launchRound(p, true);
+            float ms = 0.0f;
+            MPX_HIP(hipEventSynchronize(ev1_));
+            MPX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            tune_res_.push_back({ms, p});
+            iter_ms_ += ms;
+            ++rounds_timed_;
+            if (tune_left_.empty()) {
+                std::sort(tune_res_.begin(), tune_res_.end(),
+                          [](const auto& a, const auto& b) { return a.first < b.first; });
+                cur_ = tune_res_.front().second;
+            }
+            continue;
+        }
+        launchRound(cur_, false);
+    }
+    kZeroInt<<<1, 1, 0, st_>>>(dcount);
+    kCountActive<<<ceilDiv(K, kBlock), kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
+                                                        K, dcount);
+    MPX_HIP(hipGetLastError());
+    int active = 0;
+    MPX_HIP(hipMemcpyAsync(&active, dcount, sizeof(int), hipMemcpyDeviceToHost, st_));
+    MPX_HIP(hipStreamSynchronize(st_));
+    return active;
+}
+
+void BatchEngine::run(std::chrono::steady_clock::time_point t0) {
+    const int K = count();
+    if (K == 0) return;
+    while (true) {
+        const int active = step(2);
+        if (active == 0) break;
+        const bool timeUp = o_.max_solve_seconds >= 0.0 &&
+                            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                                    .count() >= o_.max_solve_seconds;
+        const bool stop = o_.stop_fn != nullptr && o_.stop_fn(o_.stop_user) != 0;
+        if (timeUp || stop) {
+            kAbortActive<<<ceilDiv(K, kBlock), kBlock, 0, st_>>>(
+                static_cast<const BInst*>(d_insts_buf_->p), K,
+                timeUp ? MPX_STATUS_TIME_LIMIT : MPX_STATUS_ITER_LIMIT);
+            MPX_HIP(hipStreamSynchronize(st_));
+            break;
+        }
+    }
+}
+
+void BatchEngine::results(double* const* x_out, double* const* y_out, mpx_pdlp_result* res) {
+    const int ni = count();
+    if (ni == 0) return;
+    // Recover all instances into one device staging block, then one copy out.
+    std::vector<size_t> xo(static_cast<size_t>(ni)), yo(static_cast<size_t>(ni));
+    size_t total = 0;
+    for (int i = 0; i < ni; ++i) {
+        xo[i] = total;
+        total += static_cast<size_t>(insts_[i].n);
+        yo[i] = total;
+        total += static_cast<size_t>(insts_[i].m);
+    }
+    DevBuf stage(total * sizeof(double) + 256);
+    double* base = static_cast<double*>(stage.p);
+    std::vector<double*> xs(static_cast<size_t>(ni)), ys(static_cast<size_t>(ni));
+    for (int i = 0; i < ni; ++i) {
+        xs[i] = base + xo[i];
+        ys[i] = base + yo[i];
+    }
+    DevBuf ptrs(sizeof(double*) * 2 * ni);
+    MPX_HIP(hipMemcpyAsync(ptrs.p, xs.data(), sizeof(double*) * ni, hipMemcpyHostToDevice, st_));
+    MPX_HIP(hipMemcpyAsync(static_cast<double**>(ptrs.p) + ni, ys.data(), sizeof(double*) * ni,
+                           hipMemcpyHostToDevice, st_));
+    kInstRecover<<<ni, kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
+                                         static_cast<double* const*>(ptrs.p),
+                                         static_cast<double* const*>(ptrs.p) + ni);
+    MPX_HIP(hipGetLastError());
+    std::vector<BCtrl> ctrl(static_cast<size_t>(ni));
+    std::vector<double> host(total);
+    MPX_HIP(hipMemcpyAsync(ctrl.data(), ctrl_all_, sizeof(BCtrl) * ni, hipMemcpyDeviceToHost, st_));
+    MPX_HIP(hipMemcpyAsync(host.data(), base, total * sizeof(double), hipMemcpyDeviceToHost, st_));
+    MPX_HIP(hipStreamSynchronize(st_));
+    for (int i = 0; i < ni; ++i) {
+        const mpx_lp& lp = *owners_host_[inst_owner_[i]];
+        const int n = insts_[i].n, m = insts_[i].m;
+        const BCtrl& c = ctrl[i];
+        mpx_pdlp_result& r = res[i];
+        std::memset(&r, 0, sizeof(r));
+        r.status = c.status;
+        r.iterations = c.total;
+        r.work_units = static_cast<double>(c.total) *
+                       (4.0 * static_cast<double>(insts_[i].nnz) + 2.0 * static_cast<double>(m + n));
+        r.rel_primal = o_.optimality_norm == 1 ? c.rel_p_inf : c.rel_p;
+        r.rel_dual = o_.optimality_norm == 1 ? c.rel_d_inf : c.rel_d;
+        r.rel_gap = c.gap;
+        r.primal_obj = c.pobj;
+        r.dual_obj = c.dobj;
+        r.step = c.step;
+        r.primal_weight = c.pw;
+        r.restarts = c.restarts;
+        r.iterations_fp32 = c.fp32_iters;
+        r.setup_seconds = setup_seconds_;
+        const double* hx = host.data() + xo[i];
+        const double* hy = host.data() + yo[i];
+        // Optimal: solution; Infeasible: Farkas ray in y; Unbounded: primal
+        // ray in x (reference pdlp.cpp:2318-2340); otherwise zeros.
+        const bool keepX = c.status == MPX_STATUS_OPTIMAL || c.status == MPX_STATUS_UNBOUNDED;
+        const bool keepY = c.status == MPX_STATUS_OPTIMAL || c.status == MPX_STATUS_INFEASIBLE;
+        if (c.status == MPX_STATUS_OPTIMAL) {
+            double objv = lp.obj_offset;
+            for (int j = 0; j < n; ++j) objv += lp.obj[j] * hx[j];
+            r.objective = objv;
+        }
+        if (x_out && x_out[i]) {
+            if (keepX) std::memcpy(x_out[i], hx, n * 8ull);
+            else std::memset(x_out[i], 0, n * 8ull);
+        }
+        if (y_out && y_out[i]) {
+            if (keepY) std::memcpy(y_out[i], hy, m * 8ull);
+            else std::memset(y_out[i], 0, m * 8ull);
+        }
+    }
+}
+
+bool validLpB(const mpx_lp* lp) {
+    if (!lp || lp->num_rows <= 0 || lp->num_cols <= 0 || !lp->row_starts) return false;
+    if (lp->row_starts[0] != 0) return false;
+    const int nnz = lp->row_starts[lp->num_rows];
+    if (nnz < 0 || (nnz > 0 && (!lp->col_indices || !lp->values))) return false;
+    if (!lp->obj || !lp->col_lower || !lp->col_upper || !lp->row_lower || !lp->row_upper)
+        return false;
+    for (int i = 0; i < lp->num_rows; ++i)
+        if (lp->row_starts[i + 1] < lp->row_starts[i]) return false;
+    for (int k = 0; k < nnz; ++k)
+        if (lp->col_indices[k] < 0 || lp->col_indices[k] >= lp->num_cols) return false;
+    return true;
+}
+
+bool batchOptionsOk(const mpx_pdlp_options* o) {
+    if (!o || o->termination_eval_frequency <= 0) return false;
+    if (o->adaptive_step || (o->anderson_acceleration && o->anderson_window_size >= 2) ||
+        o->matrix_zero_tol > 0.0) {
+        setError("batched PDLP: adaptive_step, anderson_acceleration and matrix_zero_tol are not "
+                 "supported");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace mpx
+
+using namespace mpx;
 
 extern "C" {
 
-int mpx_pdlp_solve_batch_shared(const mpx_lp*, int32_t, const double*, const double*,
-                                const mpx_pdlp_options*, double*, double*, mpx_pdlp_result*) {
-    mpx::setError("mpx_pdlp_solve_batch_shared: not implemented yet");
-    return MPX_ERR_UNSUPPORTED;
+int mpx_pdlp_solve_batch_shared(const mpx_lp* base, int32_t count, const double* col_lower,
+                                const double* col_upper, const mpx_pdlp_options* opts,
+                                double* x_out, double* y_out, mpx_pdlp_result* results) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!validLpB(base) || count < 0 || !results || (col_lower == nullptr) != (col_upper == nullptr)) {
+        setError("mpx_pdlp_solve_batch_shared: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    if (!batchOptionsOk(opts)) {
+        return MPX_ERR_UNSUPPORTED;
+    }
+    const int n = base->num_cols, m = base->num_rows;
+    try {
+        BatchEngine eng(*opts);
+        eng.allowInterleaved(true);
+        const int ow = eng.addOwner(*base);
+        for (int i = 0; i < count; ++i)
+            eng.addInstance(ow, col_lower ? col_lower + static_cast<size_t>(i) * n : nullptr,
+                            col_upper ? col_upper + static_cast<size_t>(i) * n : nullptr);
+        eng.setup();
+        eng.run(t0);
+        std::vector<double*> xs(static_cast<size_t>(count)), ys(static_cast<size_t>(count));
+        for (int i = 0; i < count; ++i) {
+            xs[i] = x_out ? x_out + static_cast<size_t>(i) * n : nullptr;
+            ys[i] = y_out ? y_out + static_cast<size_t>(i) * m : nullptr;
+        }
+        eng.results(xs.data(), ys.data(), results);
+        const double secs =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int i = 0; i < count; ++i) results[i].seconds = secs;
+    } catch (const DeviceError&) {
+        for (int i = 0; i < count; ++i) results[i].status = MPX_STATUS_ERROR;
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
 }
 
-int mpx_pdlp_solve_batch(const mpx_lp*, int32_t, const mpx_pdlp_options*, double* const*,
-                         double* const*, mpx_pdlp_result*) {
-    mpx::setError("mpx_pdlp_solve_batch: not implemented yet");
-    return MPX_ERR_UNSUPPORTED;
+int mpx_pdlp_solve_batch(const mpx_lp* lps, int32_t count, const mpx_pdlp_options* opts,
+                         double* const* x_out, double* const* y_out, mpx_pdlp_result* results) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (count < 0 || (count > 0 && (!lps || !results))) {
+        setError("mpx_pdlp_solve_batch: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    for (int i = 0; i < count; ++i)
+        if (!validLpB(&lps[i])) {
+            setError("mpx_pdlp_solve_batch: invalid LP " + std::to_string(i) +
+                     " (batched LPs need m > 0 and n > 0)");
+            return MPX_ERR_INVALID;
+        }
+    if (!batchOptionsOk(opts)) return MPX_ERR_UNSUPPORTED;
+    try {
+        BatchEngine eng(*opts);
+        for (int i = 0; i < count; ++i) eng.addInstance(eng.addOwner(lps[i]), nullptr, nullptr);
+        eng.setup();
+        eng.run(t0);
+        eng.results(x_out, y_out, results);
+        const double secs =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int i = 0; i < count; ++i) results[i].seconds = secs;
+    } catch (const DeviceError&) {
+        for (int i = 0; i < count; ++i) results[i].status = MPX_STATUS_ERROR;
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
 }
 
 int mpx_server_create(const mpx_pdlp_options*, double, int32_t, mpx_server** out) {

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void kEvalRow(
 // Final reduction of `nf` fields of `nblocks` partials each, in a fixed order:
 // thread t sums partials t, t+256, ... sequentially, then a block reduction.
 // maxMask bit f marks a max field.  One block.
-__global__ __launch_bounds__(kBlock) void kFinalize(const double* __restrict__ partials,
+[[maybe_unused]] static __global__ __launch_bounds__(kBlock) void kFinalize(const double* __restrict__ partials,
                                                     int nblocks, int nf, unsigned maxMask,
                                                     double* __restrict__ out) {
     __shared__ double red[kBlock];

@@ -875,6 +875,16 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
         convertProblem<float>(stream_.s, v32_, v64_, m_, n_, nnz_);
         convertIterates64to32();
     }
+    // Mixed precision: the FP64 phase runs to refine_tol (default: the solve's
+    // own tolerances), so the answer is an FP64-converged point.
+    const double rfac = 1.0;
+    const double rtol_p = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.primal_tol) : rfac * o.primal_tol;
+    const double rtol_d = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.dual_tol) : rfac * o.dual_tol;
+    // Gap target halved: the relative gap divides by 1 + |pobj| + |dobj|
+    // (about twice |objective|), so gap <= tol/2 keeps the objective within
+    // tol of the optimum.
+    const double rtol_g = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol)
+                                             : 0.5 * rfac * o.optimality_tol;
     auto kktOf = [&](const EvalOut& ev) {
         const double tp = o.optimality_norm == 1 ? ev.rel_p_inf : ev.rel_p;
         const double td = o.optimality_norm == 1 ? ev.rel_d_inf : ev.rel_d;
@@ -938,29 +948,29 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
             std::printf("PDLP %7d  pobj=% .8e  pinf=% .2e  dinf=% .2e  gap=% .2e  fpe=% .2e  "
                         "pw=% .2e  step=% .2e  [MI355X%s]\n",
                         total, e.pobj, tp, td, e.gap, e.fpe, pw_, step_, fp32_phase ? " fp32" : "");
-        if (tp <= o.primal_tol && td <= o.dual_tol && e.gap <= o.optimality_tol) {
-            if (fp32_phase) {
-                // Converged on FP32 iterates judged in FP64: hand over to FP64.
-                convertIterates32to64();
-                fp32_phase = false;
-            }
-            converged = true;
-            break;
-        }
         if (fp32_phase) {
+            // FP32 iterations judged in FP64: hand over to FP64 (restarted at
+            // the FP32 point) within a decade of the tolerances, or when FP32
+            // stops making progress.
             const double k = kktOf(e);
+            bool handover = k <= 10.0;
             if (k < 0.5 * best_kkt) {
                 best_kkt = k;
                 stall = 0;
-            } else if (++stall >= 10) {
-                // FP32 stagnated: refine in FP64 from the FP32 point
-                // (restart anchor = current point).
+            } else if (++stall >= 20) {
+                handover = true;
+            }
+            if (handover) {
                 convertIterates32to64();
                 fp32_phase = false;
                 restartNow(false);
                 ++restarts;
                 continue;
             }
+        } else if (tp <= (mixed_ ? rtol_p : o.primal_tol) && td <= (mixed_ ? rtol_d : o.dual_tol) &&
+                   e.gap <= (mixed_ ? rtol_g : o.optimality_tol)) {
+            converged = true;
+            break;
         }
         // Certificates every 10 N (pdlp.cpp:2309-2341), on the host.
         if (total >= 10 * N && total % (10 * N) == 0) {

@@ -1,0 +1,167 @@
+// host_tests.cpp -- C++ tests of mipx::PdlpSolver on the device, after the
+// reference's tests/test_pdlp.cpp closed-form cases (line numbers cited per
+// case).  A tiny runner (no test framework): prints one line per check and
+// exits nonzero on any failure.
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "mipx/pdlp.h"
+
+using namespace mipx;
+
+static int g_fail = 0, g_pass = 0;
+#define CHECK(cond)                                                             \
+    do {                                                                        \
+        if (cond) {                                                             \
+            ++g_pass;                                                           \
+        } else {                                                                \
+            ++g_fail;                                                           \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);         \
+        }                                                                       \
+    } while (0)
+
+static LpProblem lp2(std::vector<Real> obj, std::vector<Real> lo, std::vector<Real> hi,
+                     std::vector<Real> rl, std::vector<Real> ru, std::vector<Triplet> t) {
+    LpProblem p;
+    p.num_cols = static_cast<Index>(obj.size());
+    p.num_rows = static_cast<Index>(rl.size());
+    p.obj = std::move(obj);
+    p.col_lower = std::move(lo);
+    p.col_upper = std::move(hi);
+    p.row_lower = std::move(rl);
+    p.row_upper = std::move(ru);
+    p.matrix = SparseMatrix(p.num_rows, p.num_cols, std::move(t));
+    return p;
+}
+
+static PdlpOptions quiet(Int max_iter = 50000) {
+    PdlpOptions o;
+    o.verbose = false;
+    o.max_iter = max_iter;
+    return o;
+}
+
+int main() {
+    const LpProblem simple = lp2({-1, -1}, {0, 0}, {kInf, kInf}, {-kInf}, {4.0},
+                                 {{0, 0, 1.0}, {0, 1, 1.0}});
+    {  // test_pdlp.cpp:88-107
+        PdlpSolver s;
+        s.setOptions(quiet(10000));
+        s.load(simple);
+        const LpResult r = s.solve();
+        CHECK(r.status == Status::Optimal);
+        CHECK(std::abs(r.objective + 4.0) <= 1e-4);
+        CHECK(r.work_units > 0.0);
+        const auto x = s.getPrimalValues();
+        CHECK(x.size() == 2 && x[0] >= -1e-6 && x[1] >= -1e-6 && x[0] + x[1] <= 4.0 + 1e-3);
+        CHECK(s.usedGpu());
+        const auto rc = s.getReducedCosts();
+        CHECK(rc.size() == 2);
+    }
+    {  // bounds-only, test_pdlp.cpp:109-124
+        PdlpSolver s;
+        s.setOptions(quiet());
+        s.load(lp2({1.0}, {1.0}, {3.0}, {}, {}, {}));
+        const LpResult r = s.solve();
+        CHECK(r.status == Status::Optimal && std::abs(r.objective - 1.0) <= 1e-6);
+        CHECK(std::abs(s.getPrimalValues()[0] - 1.0) <= 1e-5);
+    }
+    {  // empty, offset 42, test_pdlp.cpp:303-322
+        LpProblem p;
+        p.obj_offset = 42.0;
+        PdlpSolver s;
+        s.setOptions(quiet());
+        s.load(p);
+        const LpResult r = s.solve();
+        CHECK(r.status == Status::Optimal && std::abs(r.objective - 42.0) <= 1e-9 &&
+              r.iterations == 0);
+    }
+    {  // infeasible, test_pdlp.cpp:347-376 + Farkas ray 445-496
+        PdlpSolver s;
+        s.setOptions(quiet(100000));
+        s.load(lp2({1.0}, {1.0}, {kInf}, {-kInf}, {-1.0}, {{0, 0, 1.0}}));
+        CHECK(s.solve().status == Status::Infeasible);
+        const auto y = s.getDualValues();
+        CHECK(y.size() == 1 && std::abs(y[0]) > 1e-10);
+        const Real ray = y[0] / std::abs(y[0]);
+        CHECK(ray >= -1e-6 && ray * (-1.0) - ray * 1.0 < -1e-6);
+    }
+    {  // unbounded constrained, test_pdlp.cpp:414-443
+        PdlpSolver s;
+        s.setOptions(quiet(100000));
+        s.load(lp2({-1.0, 0.0}, {0, 0}, {kInf, kInf}, {-kInf}, {0.0}, {{0, 0, 1.0}, {0, 1, -1.0}}));
+        CHECK(s.solve().status == Status::Unbounded);
+    }
+    {  // binding column bounds, -13 (test_pdlp.cpp:639-663); then setColBounds
+        PdlpSolver s;
+        s.setOptions(quiet());
+        s.load(lp2({-1, -2}, {0, 0}, {3, 5}, {-kInf}, {100.0}, {{0, 0, 1.0}, {0, 1, 1.0}}));
+        LpResult r = s.solve();
+        CHECK(r.status == Status::Optimal && std::abs(r.objective + 13.0) <= 1e-3);
+        s.setColBounds(1, 0.0, 4.0);  // optimum moves to -11
+        r = s.solve();
+        CHECK(r.status == Status::Optimal && std::abs(r.objective + 11.0) <= 1e-3);
+        s.setColBounds(7, 0.0, 1.0);  // invalid column: ignored
+        CHECK(s.solve().status == Status::Optimal);
+    }
+    {  // equality, 10 (test_pdlp.cpp:692-716); addRows / removeRows round trip
+        PdlpSolver s;
+        s.setOptions(quiet());
+        s.load(lp2({1, 2}, {0, 0}, {kInf, kInf}, {10.0}, {10.0}, {{0, 0, 1.0}, {0, 1, 1.0}}));
+        CHECK(std::abs(s.solve().objective - 10.0) <= 0.1);
+        const std::vector<Index> st = {0, 1}, idx = {0};
+        const std::vector<Real> val = {1.0}, lo = {-kInf}, hi = {4.0};
+        s.addRows(st, idx, val, lo, hi);  // x0 <= 4 forces x1 = 6: objective 16
+        LpResult r = s.solve();
+        CHECK(r.status == Status::Optimal && std::abs(r.objective - 16.0) <= 0.1);
+        const std::vector<Index> rm = {1};
+        s.removeRows(rm);
+        r = s.solve();
+        CHECK(std::abs(r.objective - 10.0) <= 0.1);
+    }
+    {  // time limit 0 (test_pdlp.cpp:141-154)
+        PdlpSolver s;
+        PdlpOptions o = quiet();
+        o.max_solve_seconds = 0.0;
+        s.setOptions(o);
+        s.load(simple);
+        CHECK(s.solve().status == Status::TimeLimit);
+    }
+    {  // stop flag: cancellation reports IterLimit (pdlp.cpp:972-975)
+        std::atomic<bool> stop{true};
+        PdlpSolver s;
+        PdlpOptions o = quiet();
+        o.stop_flag = &stop;
+        s.setOptions(o);
+        s.load(simple);
+        CHECK(s.solve().status == Status::IterLimit);
+    }
+    {  // solve before load is an error (pdlp.cpp:802-805)
+        PdlpSolver s;
+        CHECK(s.solve().status == Status::Error);
+    }
+    {  // warm start with wrong sizes is ignored (test_pdlp.cpp:1182-1201)
+        PdlpSolver s;
+        s.setOptions(quiet(10000));
+        s.load(simple);
+        const std::vector<Real> wx = {1.0, 2.0, 3.0}, wy = {0.5};
+        s.setWarmStart(wx, wy);
+        const LpResult r = s.solve();
+        CHECK(r.status == Status::Optimal && std::abs(r.objective + 4.0) <= 1e-4);
+    }
+    {  // mixed precision on the simple LP
+        PdlpSolver s;
+        PdlpOptions o = quiet(10000);
+        o.precision = PdlpPrecision::Mixed;
+        s.setOptions(o);
+        s.load(simple);
+        const LpResult r = s.solve();
+        CHECK(r.status == Status::Optimal && std::abs(r.objective + 4.0) <= 1e-4);
+    }
+    std::printf("host_tests: %d passed, %d failed\n", g_pass, g_fail);
+    return g_fail == 0 ? 0 : 1;
+}

@@ -58,3 +58,22 @@ def medium_lp():
                 trips.append((i, j, 0.1 + rng.uniform() * 2.0))
     return from_triplets(rows, cols, trips, obj, np.zeros(cols), np.full(cols, 100.0),
                          np.full(rows, -np.inf), np.array([50.0 + 2.0 * i for i in range(rows)]))
+
+
+def highs_objective(lp):
+    """Exact optimum of a (feasible, bounded) LP from scipy's HiGHS, as an
+    independent optimum for accuracy checks."""
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+    a = sp.csr_matrix((lp.values, lp.col_indices, lp.row_starts), shape=(lp.num_rows, lp.num_cols))
+    eq = lp.row_lower == lp.row_upper
+    ub_hi = ~eq & np.isfinite(lp.row_upper)
+    ub_lo = ~eq & np.isfinite(lp.row_lower)
+    import scipy.sparse as sps
+    a_ub = sps.vstack([a[ub_hi], -a[ub_lo]])
+    b_ub = np.concatenate([lp.row_upper[ub_hi], -lp.row_lower[ub_lo]])
+    sign = -1.0 if lp.sense else 1.0
+    res = linprog(sign * lp.obj, A_ub=a_ub, b_ub=b_ub, A_eq=a[eq], b_eq=lp.row_lower[eq],
+                  bounds=list(zip(lp.col_lower, lp.col_upper)), method="highs")
+    assert res.status == 0, res.message
+    return float(sign * res.fun + lp.obj_offset)

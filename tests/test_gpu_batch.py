@@ -1,0 +1,81 @@
+"""GPU parity tests of the batched path (one workgroup per LP) against the
+oracle, through the C-ABI."""
+import numpy as np
+import pytest
+
+from conftest import TIGHT
+from lp_cases import highs_objective
+from mipx_amd import lpgen, native
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_vs_oracle(lp, r, x, y, tag="", mixed=False):
+    """FP64: objective within 1e-6 of the oracle's (same trajectory).
+    Mixed precision (FP32 iterations, FP64 evaluation and refinement) lands
+    on a different point of the 1e-6 KKT region.  A 1e-6 relative KKT
+    tolerance does not bound the objective error by 1e-6: the oracle itself
+    (bit-identical to the reference) is up to 2.6e-6 from the exact optimum
+    on these LP families.  Mixed results are held to that envelope: within
+    3e-6 of the exact optimum (scipy HiGHS), within 5e-6 of the oracle, and
+    KKT-certified by the independent long-double checker."""
+    ro, xo, yo = ob.solve(lp, **TIGHT)
+    assert r["status"] == ro["status"], (tag, r, ro["status"])
+    if ro["status"] == 0:
+        rel = abs(r["objective"] - ro["objective"]) / (1 + abs(ro["objective"]))
+        assert rel <= (5e-6 if mixed else 1e-6), (tag, r["objective"], ro["objective"], rel)
+        if mixed:
+            opt = highs_objective(lp)
+            assert abs(r["objective"] - opt) <= 3e-6 * (1 + abs(opt)), (tag, r["objective"], opt)
+        k = ob.kkt_check(lp, x, y)
+        assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, (tag, k)
+    return ro
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_shared_bounds_vs_oracle(precision):
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    count = 24
+    lo = np.empty((count, 800))
+    hi = np.empty((count, 800))
+    for i in range(count):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    res, X, Y = native.solve_batch_shared(g.lp, lo, hi, precision=precision, max_iter=200000,
+                                          **TIGHT)
+    statuses = [r["status"] for r in res]
+    print("statuses", statuses, "iters", [r["iterations"] for r in res])
+    for i in range(count):
+        _check_vs_oracle(g.lp.with_col_bounds(lo[i], hi[i]), res[i], X[i], Y[i], f"inst{i}",
+                         mixed=precision == 1)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_heterogeneous_vs_oracle(precision):
+    sizes = [40, 75, 120, 200, 333, 60, 90, 150]
+    gens = [lpgen.c3_instance(i, m) for i, m in enumerate(sizes)]
+    lps = [g.lp for g in gens]
+    res, xs, ys = native.solve_batch(lps, precision=precision, max_iter=200000, **TIGHT)
+    for i, lp in enumerate(lps):
+        _check_vs_oracle(lp, res[i], xs[i], ys[i], f"lp{i}", mixed=precision == 1)
+
+
+def test_batch_result_same_alone_and_in_mix():
+    """An instance's result does not depend on what else is in the launch."""
+    sizes = [80, 150, 60, 220]
+    lps = [lpgen.c3_instance(i, m).lp for i, m in enumerate(sizes)]
+    mix, xm, ym = native.solve_batch(lps, precision=1, max_iter=200000, **TIGHT)
+    for i in (1, 3):
+        alone, xa, ya = native.solve_batch([lps[i]], precision=1, max_iter=200000, **TIGHT)
+        assert alone[0]["iterations"] == mix[i]["iterations"]
+        assert alone[0]["objective"] == mix[i]["objective"]
+        assert np.array_equal(xa[0], xm[i]) and np.array_equal(ya[0], ym[i])
+
+
+def test_batch_detects_infeasible_and_unbounded():
+    from lp_cases import closed_form_cases
+    cases = {c["name"]: lp for c, lp in closed_form_cases()}
+    lps = [cases["infeasible"], cases["unbounded_constrained"], cases["simple"]]
+    res, xs, ys = native.solve_batch(lps, verbose=0, max_iter=100000)
+    assert [r["status"] for r in res] == [1, 2, 0]
+    assert abs(res[2]["objective"] + 4.0) <= 1e-4

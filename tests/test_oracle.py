@@ -160,3 +160,19 @@ def test_generator_is_deterministic_and_feasible():
         seg = lp.col_indices[lp.row_starts[i]:lp.row_starts[i + 1]]
         assert np.all(np.diff(seg) > 0)
     assert lp.nnz == 250 * 5
+
+
+def test_oracle_objective_envelope_vs_exact_optimum():
+    """Pins how far the reference's PDLP (oracle, bit-identical to it) lands
+    from the exact optimum at tol 1e-6: the mixed-precision bar is set from
+    this envelope."""
+    from lp_cases import highs_objective
+    errs = []
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    for i in range(8):
+        lo, hi = g.branch_bounds(lpgen.c2_instance_seed(i))
+        lp = g.lp.with_col_bounds(lo, hi)
+        r, _, _ = ob.solve(lp, **TIGHT)
+        opt = highs_objective(lp)
+        errs.append(abs(r["objective"] - opt) / (1 + abs(opt)))
+    assert max(errs) <= 3e-6
