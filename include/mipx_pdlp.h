@@ -163,6 +163,8 @@ void mpx_pdlp_default_options(mpx_pdlp_options* opts);
 const char* mpx_last_error(void);
 const char* mpx_version(void);
 int mpx_device_count(int32_t* count);
+/* Waits for all work on the device (hipDeviceSynchronize). */
+int mpx_device_synchronize(int32_t device);
 
 /*
  * Solve one LP on the GPU.  Replaces `PdlpSolver::solveGpu()`
@@ -208,6 +210,47 @@ int mpx_pdlp_solve_batch(const mpx_lp* lps, int32_t count,
                          const mpx_pdlp_options* opts,
                          double* const* x_out, double* const* y_out,
                          mpx_pdlp_result* results);
+
+/* ---- resident batches ----------------------------------------------------
+ * The same batched solves with the problem data kept in HBM: create uploads
+ * once (the arrays passed in are only read during create, except base/lps
+ * objective vectors, which are copied), run solves on the device only (it
+ * can be called again: it re-runs scaling, power iteration and PDHG from
+ * the uploaded data), results copies solutions back.
+ */
+typedef struct mpx_batch mpx_batch;
+
+typedef struct mpx_batch_stats {
+    double upload_ms;          /* host -> device at create */
+    double setup_ms;           /* device scaling + power iteration + init (last run) */
+    double run_ms;             /* wall time of the last mpx_batch_run (incl. setup) */
+    int32_t rounds;            /* rounds of N iterations in the last run */
+    int32_t interleaved;       /* 1: lane-per-instance kernels (shared matrix) */
+    int64_t instance_iterations; /* sum of PDHG iterations over instances */
+    double latency_p50_ms;     /* per-instance completion time from run start */
+    double latency_p99_ms;
+    double latency_max_ms;
+    /* mpx_batch_profile: averages of individually timed kernel launches
+       (HIP events on the batch's stream) at the start of a fresh run */
+    double primal_kernel_us;
+    double dual_kernel_us;
+    int32_t profiled_slots;    /* instances active in the profiled launches */
+    int32_t profiled_launches;
+} mpx_batch_stats;
+
+int mpx_batch_create_shared(const mpx_lp* base, int32_t count, const double* col_lower,
+                            const double* col_upper, const mpx_pdlp_options* opts,
+                            mpx_batch** out);
+int mpx_batch_create(const mpx_lp* lps, int32_t count, const mpx_pdlp_options* opts,
+                     mpx_batch** out);
+int mpx_batch_run(mpx_batch* b);
+int mpx_batch_results(mpx_batch* b, double* const* x_out, double* const* y_out,
+                      mpx_pdlp_result* results);
+int mpx_batch_get_stats(mpx_batch* b, mpx_batch_stats* stats);
+/* Re-initialises the batch and times `launches` launches of the two
+ * iteration kernels one by one (fills the primal/dual kernel fields). */
+int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats);
+int mpx_batch_destroy(mpx_batch* b);
 
 /* ---- serving loop (continuous batching) ---------------------------------
  * A server owns a device context and an HBM budget.  Submitted requests are

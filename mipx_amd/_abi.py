@@ -197,3 +197,24 @@ def scaled_from_arrays(arrs):
             ptype = c_int32_p if a.dtype == np.int32 else c_double_p
             setattr(sp, k, a.ctypes.data_as(ptype))
     return sp
+
+
+class MpxBatchStats(C.Structure):
+    _fields_ = [
+        ("upload_ms", C.c_double),
+        ("setup_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("rounds", C.c_int32),
+        ("interleaved", C.c_int32),
+        ("instance_iterations", C.c_int64),
+        ("latency_p50_ms", C.c_double),
+        ("latency_p99_ms", C.c_double),
+        ("latency_max_ms", C.c_double),
+        ("primal_kernel_us", C.c_double),
+        ("dual_kernel_us", C.c_double),
+        ("profiled_slots", C.c_int32),
+        ("profiled_launches", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}

@@ -185,6 +185,38 @@ __global__ __launch_bounds__(kBlock) void kDeinterleave(const BInst* __restrict_
     });
 }
 
+// Streamed per-lane vectors use non-temporal accesses so that they do not
+// evict the gathered y / x_r slice of the group from L2.
+template <typename T>
+__device__ inline T ldNT(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ inline void stNT(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
+// Gathered dot product sum_q val[q] * V[idx[q] * 64] with four entries in
+// flight; the additions stay in storage order.
+template <typename T>
+__device__ inline T laneDot(const int* __restrict__ idx, const T* __restrict__ val, int qb, int qe,
+                            const T* __restrict__ V) {
+    T acc = T(0);
+    int q = qb;
+    for (; q + 4 <= qe; q += 4) {
+        const int i0 = idx[q], i1 = idx[q + 1], i2 = idx[q + 2], i3 = idx[q + 3];
+        const T a0 = val[q], a1 = val[q + 1], a2 = val[q + 2], a3 = val[q + 3];
+        const T v0 = V[static_cast<size_t>(i0) * kLanes], v1 = V[static_cast<size_t>(i1) * kLanes];
+        const T v2 = V[static_cast<size_t>(i2) * kLanes], v3 = V[static_cast<size_t>(i3) * kLanes];
+        acc += a0 * v0;
+        acc += a1 * v1;
+        acc += a2 * v2;
+        acc += a3 * v3;
+    }
+    for (; q < qe; ++q) acc += val[q] * V[static_cast<size_t>(idx[q]) * kLanes];
+    return acc;
+}
+
 // Primal half step for one column j of 64 instances per wave (4 columns per
 // block).  Shared data: CSC structure/values, c, tau_base.
 template <typename T>
@@ -205,21 +237,19 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         const bool live = s < nslots;
         const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
         if (kiter > ln.lim) return;  // lane idle this iteration
-        const T* yg = IY + static_cast<size_t>(g) * m * kLanes + lane;
-        T acc = T(0);
-        const int qb = cs[j], qe = cs[j + 1];
-        for (int q = qb; q < qe; ++q) acc += valt[q] * yg[static_cast<size_t>(ri[q]) * kLanes];
+        const T acc = laneDot(ri, valt, cs[j], cs[j + 1],
+                              IY + static_cast<size_t>(g) * m * kLanes + lane);
         const size_t o = (static_cast<size_t>(g) * n + j) * kLanes + lane;
         const int kk = ln.inner + kiter;
         const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
         const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
         const T tau = step * tau_base[j] / pw;
-        const T xj = IX[o];
+        const T xj = ldNT(IX + o);
         const T t = xj - tau * (c[j] + acc);
-        const T h = clampRef(t, IL[o], IU[o]);
+        const T h = clampRef(t, ldNT(IL + o), ldNT(IU + o));
         const T r = T(2) * h - xj;
-        IXR[o] = r;
-        IX[o] = lam * r + (T(1) - lam) * IX0[o];
+        IXR[o] = r;  // gathered by the dual step: keep it cacheable
+        stNT(IX + o, lam * r + (T(1) - lam) * ldNT(IX0 + o));
         if (kiter == ln.lim) vecsOf<T>(insts[slots[s]]).xh[j] = h;
     });
 }
@@ -241,20 +271,18 @@ __global__ __launch_bounds__(kBlock) void kILDual(
         const bool live = s < nslots;
         const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
         if (kiter > ln.lim) return;
-        const T* xg = IXR + static_cast<size_t>(g) * n * kLanes + lane;
-        T acc = T(0);
-        const int qb = rs[i], qe = rs[i + 1];
-        for (int q = qb; q < qe; ++q) acc += val[q] * xg[static_cast<size_t>(ci[q]) * kLanes];
+        const T acc = laneDot(ci, val, rs[i], rs[i + 1],
+                              IXR + static_cast<size_t>(g) * n * kLanes + lane);
         const size_t o = (static_cast<size_t>(g) * m + i) * kLanes + lane;
         const int kk = ln.inner + kiter;
         const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
         const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
         const T sig = step * sigma_base[i] * pw;
-        const T yi = IY[o];
+        const T yi = IY[o];  // y is gathered by the next primal step: cacheable
         const T v = yi + sig * acc;
         const T h = dualProx(v, sig, rl[i], ru[i]);
         const T r = T(2) * h - yi;
-        IY[o] = lam * r + (T(1) - lam) * IY0[o];
+        IY[o] = lam * r + (T(1) - lam) * ldNT(IY0 + o);
         if (kiter == ln.lim) vecsOf<T>(insts[slots[s]]).yh[i] = h;
     });
 }

@@ -36,6 +36,7 @@ struct BCtrl {
     int32_t inner, total, pcount, phase, status, restarts, fp32_iters, stall;
     int32_t power_iters, g_row, g_col, use_lds;
     int32_t round_iters, skip_eval;
+    unsigned long long t_finish;  // device wall clock when the instance finished
 };
 
 enum BPhase { kPhaseFp32 = 0, kPhaseFp64 = 1, kPhasePolish = 2, kPhaseDone = 3 };
@@ -124,9 +125,12 @@ __device__ inline BEval bEvaluate(const BInst& in, const BVecs<T>& v, const BCtr
                 cf[kInter] += atdy * dx;
                 const double g = in.c[j] + aty;
                 const double lb = in.l[j], ub = in.u[j];
+                // Activity against the bounds the iteration projected onto.
+                const double lbi = static_cast<double>(static_cast<T>(lb));
+                const double ubi = static_cast<double>(static_cast<T>(ub));
                 double dr;
-                if (xj <= lb + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
-                else if (xj >= ub - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
+                if (xj <= lbi + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
+                else if (xj >= ubi - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
                 else dr = g;
                 const double d = dr / (c.os * in.col_scale[j]);
                 cf[kDsq] += d * d;

@@ -115,6 +115,13 @@ __device__ inline const BVecs<double>& pickVecs<double>(const BInst& in) { retur
 
 __global__ void kZeroInt(int* p) { *p = 0; }
 
+__global__ void kStampStart(unsigned long long* t) { *t = wall_clock64(); }
+
+// Completion stamp of an instance (device wall clock), first time it is done.
+__device__ inline void stampDone(BCtrl& c) {
+    if (c.phase == kPhaseDone && c.t_finish == 0ull) c.t_finish = wall_clock64();
+}
+
 // ---- packed iteration kernels ----------------------------------------------
 // All active instances of one precision form one flat list of tiles; a tile
 // is (instance, first column) covering kBlock/G columns (or rows) of that
@@ -402,6 +409,7 @@ __device__ void evalControl(const BInst& in, const BOptions& o) {
                 c.phase = kPhaseDone;
                 c.status = verdict == 1 ? MPX_STATUS_INFEASIBLE : MPX_STATUS_UNBOUNDED;
                 c.round_iters = 0;
+                stampDone(c);
                 s_act = kActFinish + 100;
             } else {
                 s_act = -1;
@@ -516,6 +524,7 @@ __device__ void evalControl(const BInst& in, const BOptions& o) {
         }
     }
     if (tid == 0 && c.phase == kPhaseDone) c.round_iters = 0;
+    if (tid == 0) stampDone(c);
 }
 
 template <typename T>
@@ -548,6 +557,7 @@ __global__ void kAbortActive(const BInst* __restrict__ insts, int count, int sta
     if (t < count && insts[t].ctrl->phase != kPhaseDone) {
         insts[t].ctrl->phase = kPhaseDone;
         insts[t].ctrl->status = status;
+        stampDone(*insts[t].ctrl);
     }
 }
 
@@ -842,6 +852,7 @@ __global__ __launch_bounds__(kBlock) void kInstInit(const BInst* __restrict__ in
         c.g_row = presetGR(in.preset);
         c.use_lds = 0;
         c.skip_eval = 0;
+        c.t_finish = 0ull;
         c.round_iters = o.max_iter < o.N ? o.max_iter : o.N;
         if (c.round_iters <= 0) {
             c.phase = kPhaseDone;
@@ -1010,7 +1021,18 @@ public:
         return static_cast<int>(inst_owner_.size()) - 1;
     }
 
+    // Upload (host -> device) followed by prepare().
     void setup();
+    // Device-side setup on uploaded data (re-runnable).
+    void prepare();
+    // One instrumented round: every launch of the iteration kernels timed
+    // with HIP events on the engine's stream.  Returns averages in us.
+    void profileRound(int launches, double* primal_us, double* dual_us, int* slots);
+    // Per-instance completion times (ms since the end of prepare()).
+    std::vector<double> finishTimesMs();
+    int rounds() const { return rounds_; }
+    double uploadSeconds() const { return upload_seconds_; }
+    bool interleaved() const { return interleaved_; }
     // Runs rounds until every instance is done (or the time limit / stop).
     void run(std::chrono::steady_clock::time_point t0);
     void results(double* const* x_out, double* const* y_out, mpx_pdlp_result* res);
@@ -1067,6 +1089,9 @@ private:
     double iter_ms_ = 0.0;
     int rounds_timed_ = 0;
     double setup_seconds_ = 0.0;
+    double upload_seconds_ = 0.0;
+    int rounds_ = 0;
+    unsigned long long* stamp_ = nullptr;  // device wall clock at the end of prepare()
     BCtrl* ctrl_all_ = nullptr;
     bool allow_interleaved_ = false;
     // Lane-per-instance mode (every instance shares owner 0).
@@ -1364,7 +1389,8 @@ void BatchEngine::setup() {
     d_presets_buf_ = std::make_unique<DevBuf>(sizeof(int) * std::max(1, no));
     d_insts_buf_ = std::make_unique<DevBuf>(sizeof(BInst) * std::max(1, ni));
     d_args_buf_ = std::make_unique<DevBuf>(sizeof(BInitArgs) * std::max(1, ni));
-    d_count_buf_ = std::make_unique<DevBuf>(sizeof(int) * 4);
+    d_count_buf_ = std::make_unique<DevBuf>(sizeof(int) * 4 + 64);
+    stamp_ = reinterpret_cast<unsigned long long*>(static_cast<char*>(d_count_buf_->p) + 32);
     MPX_HIP(hipMemcpyAsync(d_owners_buf_->p, owners_.data(), sizeof(BOwner) * no,
                            hipMemcpyHostToDevice, st_));
     MPX_HIP(hipMemcpyAsync(d_presets_buf_->p, owner_preset_.data(), sizeof(int) * no,
@@ -1373,6 +1399,20 @@ void BatchEngine::setup() {
                            hipMemcpyHostToDevice, st_));
     MPX_HIP(hipMemcpyAsync(d_args_buf_->p, args.data(), sizeof(BInitArgs) * ni,
                            hipMemcpyHostToDevice, st_));
+    MPX_HIP(hipStreamSynchronize(st_));
+    upload_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    prepare();
+}
+
+// Device-side setup, re-runnable on uploaded data: scaling and power
+// iteration per distinct matrix, then per-instance initialisation.
+void BatchEngine::prepare() {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int no = static_cast<int>(owners_host_.size());
+    const int ni = count();
+    for (auto& kv : graphs_) kv.second.reset();
+    graphs_.clear();
+    rounds_ = 0;
     if (no) {
         kOwnerScale<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p));
         kOwnerPower<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p),
@@ -1382,6 +1422,7 @@ void BatchEngine::setup() {
         kInstInit<<<ni, kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
                                           static_cast<const BInitArgs*>(d_args_buf_->p), bo_,
                                           o_.primal_weight);
+    kStampStart<<<1, 1, 0, st_>>>(stamp_);
     MPX_HIP(hipGetLastError());
     MPX_HIP(hipStreamSynchronize(st_));
     setup_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1496,6 +1537,112 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
 }
 
+void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us, int* slots) {
+    const int K = count();
+    const BInst* di = static_cast<const BInst*>(d_insts_buf_->p);
+    launches = std::max(1, std::min(launches, bo_.N));
+    hipEvent_t e[4];
+    for (auto& ev : e) MPX_HIP(hipEventCreate(&ev));
+    double tp = 0.0, td = 0.0;
+    int active = 0;
+    const bool f32 = mixed_;  // the FP32 kernels carry the work in mixed mode
+    if (interleaved_) {
+        const BOwner& w = owners_[0];
+        const int n = w.n, m = w.m;
+        kSlotLists<<<1, 1024, 0, st_>>>(di, K, slots32_, slots64_, lanes32_, lanes64_, slot_counts_);
+        const int* cnt = slot_counts_ + (f32 ? 0 : 1);
+        const int* sl = f32 ? slots32_ : slots64_;
+        const ILane* ln = f32 ? lanes32_ : lanes64_;
+        if (f32) {
+            kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, n, 0, il32_.x, il32_.x0,
+                                                          il32_.l, il32_.u);
+            kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, m, 1, il32_.y, il32_.y0,
+                                                          nullptr, nullptr);
+        } else {
+            kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, n, 0, il64_.x, il64_.x0,
+                                                           il64_.l, il64_.u);
+            kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, m, 1, il64_.y, il64_.y0,
+                                                           nullptr, nullptr);
+        }
+        for (int k = 1; k <= launches; ++k) {
+            MPX_HIP(hipEventRecord(e[0], st_));
+            if (f32)
+                kILPrimal<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.cs, w.ri,
+                                                            w.valtf, w.cf, w.tauf, il32_.y,
+                                                            il32_.x, il32_.x0, il32_.xr, il32_.l,
+                                                            il32_.u, k);
+            else
+                kILPrimal<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.cs, w.ri,
+                                                             w.valt, w.c, w.tau, il64_.y, il64_.x,
+                                                             il64_.x0, il64_.xr, il64_.l, il64_.u,
+                                                             k);
+            MPX_HIP(hipEventRecord(e[1], st_));
+            if (f32)
+                kILDual<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs, w.ci, w.valf,
+                                                          w.sigf, w.rlf, w.ruf, il32_.xr, il32_.y,
+                                                          il32_.y0, k);
+            else
+                kILDual<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs, w.ci, w.val,
+                                                           w.sig, w.rl, w.ru, il64_.xr, il64_.y,
+                                                           il64_.y0, k);
+            MPX_HIP(hipEventRecord(e[2], st_));
+            MPX_HIP(hipEventSynchronize(e[2]));
+            float a = 0.0f, b = 0.0f;
+            MPX_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+            MPX_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+            tp += a;
+            td += b;
+        }
+        int c2[2];
+        MPX_HIP(hipMemcpy(c2, slot_counts_, sizeof(c2), hipMemcpyDeviceToHost));
+        active = f32 ? c2[0] : c2[1];
+    } else {
+        int* cnt = static_cast<int*>(d_cnt_buf_->p);
+        const int gk = ceilDiv(K, kBlock);
+        kTileCounts<<<gk, kBlock, 0, st_>>>(di, K, cnt);
+        kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
+        kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
+        for (int k = 1; k <= launches; ++k) {
+            MPX_HIP(hipEventRecord(e[0], st_));
+            if (f32) kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
+            else kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, tc64_, totals_ + 2, k);
+            MPX_HIP(hipEventRecord(e[1], st_));
+            if (f32) kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, tr32_, totals_ + 1, k);
+            else kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, tr64_, totals_ + 3, k);
+            MPX_HIP(hipEventRecord(e[2], st_));
+            MPX_HIP(hipEventSynchronize(e[2]));
+            float a = 0.0f, b = 0.0f;
+            MPX_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+            MPX_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+            tp += a;
+            td += b;
+        }
+        int h5[5];
+        MPX_HIP(hipMemcpy(h5, totals_, sizeof(h5), hipMemcpyDeviceToHost));
+        active = h5[4];
+    }
+    MPX_HIP(hipGetLastError());
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    *primal_us = 1000.0 * tp / launches;
+    *dual_us = 1000.0 * td / launches;
+    *slots = active;
+}
+
+std::vector<double> BatchEngine::finishTimesMs() {
+    const int ni = count();
+    std::vector<BCtrl> ctrl(static_cast<size_t>(ni));
+    unsigned long long t0 = 0;
+    MPX_HIP(hipMemcpy(ctrl.data(), ctrl_all_, sizeof(BCtrl) * ni, hipMemcpyDeviceToHost));
+    MPX_HIP(hipMemcpy(&t0, stamp_, sizeof(t0), hipMemcpyDeviceToHost));
+    int dev = 0, khz = 100000;
+    MPX_HIP(hipGetDevice(&dev));
+    MPX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    std::vector<double> out(static_cast<size_t>(ni), 0.0);
+    for (int i = 0; i < ni; ++i)
+        out[i] = ctrl[i].t_finish > t0 ? static_cast<double>(ctrl[i].t_finish - t0) / khz : 0.0;
+    return out;
+}
+
 void BatchEngine::launchRound(const IterPreset& p, bool timed) {
     if (timed || graph_failed_) {
         if (interleaved_) enqueueInterleavedRound(timed);
@@ -1540,6 +1687,7 @@ int BatchEngine::step(int rounds) {
             MPX_HIP(hipEventSynchronize(ev1_));
             MPX_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
             tune_res_.push_back({ms, p});
+            ++rounds_;
             iter_ms_ += ms;
             ++rounds_timed_;
             if (tune_left_.empty()) {
@@ -1550,6 +1698,7 @@ int BatchEngine::step(int rounds) {
             continue;
         }
         launchRound(cur_, false);
+        ++rounds_;
     }
     kZeroInt<<<1, 1, 0, st_>>>(dcount);
     kCountActive<<<ceilDiv(K, kBlock), kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
@@ -1750,6 +1899,174 @@ int mpx_pdlp_solve_batch(const mpx_lp* lps, int32_t count, const mpx_pdlp_option
         for (int i = 0; i < count; ++i) results[i].status = MPX_STATUS_ERROR;
         return MPX_ERR_DEVICE;
     }
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+struct mpx_batch {
+    mpx_pdlp_options opts;
+    std::vector<mpx_lp> lps;                 // descriptors (objective copied)
+    std::vector<std::vector<double>> objs;
+    std::unique_ptr<mpx::BatchEngine> eng;
+    mpx_batch_stats stats{};
+};
+
+namespace {
+
+int createBatch(mpx_batch* b) {
+    try {
+        b->eng->setup();
+        b->stats.upload_ms = 1000.0 * b->eng->uploadSeconds();
+        b->stats.setup_ms = 1000.0 * b->eng->setupSeconds();
+        b->stats.interleaved = b->eng->interleaved() ? 1 : 0;
+    } catch (const mpx::DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
+void copyLps(mpx_batch* b, const mpx_lp* lps, int count) {
+    b->lps.assign(lps, lps + count);
+    b->objs.resize(static_cast<size_t>(count));
+    for (int i = 0; i < count; ++i) {
+        b->objs[i].assign(lps[i].obj, lps[i].obj + lps[i].num_cols);
+        b->lps[i].obj = b->objs[i].data();
+    }
+}
+
+double pct(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const size_t k = static_cast<size_t>(std::ceil(q * static_cast<double>(v.size()))) - 1;
+    return v[std::min(v.size() - 1, k)];
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpx_batch_create_shared(const mpx_lp* base, int32_t count, const double* col_lower,
+                            const double* col_upper, const mpx_pdlp_options* opts,
+                            mpx_batch** out) {
+    if (!out || !validLpB(base) || count <= 0 || (col_lower == nullptr) != (col_upper == nullptr)) {
+        setError("mpx_batch_create_shared: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    if (!batchOptionsOk(opts)) return MPX_ERR_UNSUPPORTED;
+    *out = nullptr;
+    auto b = std::make_unique<mpx_batch>();
+    b->opts = *opts;
+    copyLps(b.get(), base, 1);
+    try {
+        b->eng = std::make_unique<BatchEngine>(b->opts);
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    b->eng->allowInterleaved(true);
+    const int ow = b->eng->addOwner(b->lps[0]);
+    const size_t n = static_cast<size_t>(base->num_cols);
+    for (int i = 0; i < count; ++i)
+        b->eng->addInstance(ow, col_lower ? col_lower + i * n : nullptr,
+                            col_upper ? col_upper + i * n : nullptr);
+    const int rc = createBatch(b.get());
+    if (rc != MPX_OK) return rc;
+    *out = b.release();
+    return MPX_OK;
+}
+
+int mpx_batch_create(const mpx_lp* lps, int32_t count, const mpx_pdlp_options* opts,
+                     mpx_batch** out) {
+    if (!out || count <= 0 || !lps) {
+        setError("mpx_batch_create: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    for (int i = 0; i < count; ++i)
+        if (!validLpB(&lps[i])) {
+            setError("mpx_batch_create: invalid LP " + std::to_string(i));
+            return MPX_ERR_INVALID;
+        }
+    if (!batchOptionsOk(opts)) return MPX_ERR_UNSUPPORTED;
+    *out = nullptr;
+    auto b = std::make_unique<mpx_batch>();
+    b->opts = *opts;
+    copyLps(b.get(), lps, count);
+    try {
+        b->eng = std::make_unique<BatchEngine>(b->opts);
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    for (int i = 0; i < count; ++i) b->eng->addInstance(b->eng->addOwner(b->lps[i]), nullptr, nullptr);
+    const int rc = createBatch(b.get());
+    if (rc != MPX_OK) return rc;
+    *out = b.release();
+    return MPX_OK;
+}
+
+int mpx_batch_run(mpx_batch* b) {
+    if (!b) return MPX_ERR_INVALID;
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+        if (b->stats.rounds > 0 || b->eng->rounds() > 0) b->eng->prepare();
+        b->eng->run(t0);
+        b->stats.run_ms =
+            1000.0 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        b->stats.setup_ms = 1000.0 * b->eng->setupSeconds();
+        b->stats.rounds = b->eng->rounds();
+        const std::vector<double> ft = b->eng->finishTimesMs();
+        b->stats.latency_p50_ms = pct(ft, 0.50);
+        b->stats.latency_p99_ms = pct(ft, 0.99);
+        b->stats.latency_max_ms = pct(ft, 1.0);
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
+int mpx_batch_results(mpx_batch* b, double* const* x_out, double* const* y_out,
+                      mpx_pdlp_result* results) {
+    if (!b || !results) return MPX_ERR_INVALID;
+    try {
+        b->eng->results(x_out, y_out, results);
+        int64_t it = 0;
+        for (int i = 0; i < b->eng->count(); ++i) {
+            it += results[i].iterations;
+            results[i].seconds = b->stats.run_ms / 1000.0;
+        }
+        b->stats.instance_iterations = it;
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
+int mpx_batch_get_stats(mpx_batch* b, mpx_batch_stats* stats) {
+    if (!b || !stats) return MPX_ERR_INVALID;
+    *stats = b->stats;
+    return MPX_OK;
+}
+
+int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats) {
+    if (!b || launches <= 0) return MPX_ERR_INVALID;
+    try {
+        b->eng->prepare();
+        double p = 0.0, d = 0.0;
+        int slots = 0;
+        b->eng->profileRound(launches, &p, &d, &slots);
+        b->stats.primal_kernel_us = p;
+        b->stats.dual_kernel_us = d;
+        b->stats.profiled_slots = slots;
+        b->stats.profiled_launches = std::min(launches, b->opts.termination_eval_frequency);
+        b->stats.rounds = 1;  // state is mid-run: the next run re-prepares
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    if (stats) *stats = b->stats;
+    return MPX_OK;
+}
+
+int mpx_batch_destroy(mpx_batch* b) {
+    delete b;
     return MPX_OK;
 }
 

@@ -160,9 +160,13 @@ __global__ __launch_bounds__(kBlock) void kEvalCol(
             v[kInter] = atdy * dx;
             const double g = c[j] + aty;
             const double lb = lo[j], ub = hi[j];
+            // Activity is tested against the bounds the iteration projected
+            // onto (the iterate type's rounding of the FP64 bounds).
+            const double lbi = static_cast<double>(static_cast<TI>(lb));
+            const double ubi = static_cast<double>(static_cast<TI>(ub));
             double dr;
-            if (xj <= lb + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
-            else if (xj >= ub - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
+            if (xj <= lbi + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
+            else if (xj >= ubi - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
             else dr = g;
             const double d = dr / (os * col_scale[j]);
             v[kDsq] = d * d;

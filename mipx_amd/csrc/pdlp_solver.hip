@@ -1235,6 +1235,14 @@ int mpx_device_count(int32_t* count) {
     return MPX_OK;
 }
 
+int mpx_device_synchronize(int32_t device) {
+    if (hipSetDevice(device) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        setError("mpx_device_synchronize failed");
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
 int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double* warm_x,
                    const double* warm_y, double* x_out, double* y_out, mpx_pdlp_result* res) {
     const auto t0 = std::chrono::steady_clock::now();

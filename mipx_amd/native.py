@@ -7,18 +7,20 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (MpxLp, MpxPdlpOptions, MpxPdlpResult, MpxScaledProblem, alloc_scaled,
-                   make_options, scaled_from_arrays, c_double_p)
+from ._abi import (MpxBatchStats, MpxLp, MpxPdlpOptions, MpxPdlpResult, MpxScaledProblem,
+                   alloc_scaled, make_options, scaled_from_arrays, c_double_p)
 from ._loader import load_library
 
 _lib = None
 
 EXPORTED = [
     "mpx_pdlp_default_options", "mpx_last_error", "mpx_version", "mpx_device_count",
+    "mpx_device_synchronize",
     "mpx_pdlp_solve", "mpx_pdlp_solve_batch_shared", "mpx_pdlp_solve_batch",
     "mpx_server_create", "mpx_server_submit", "mpx_server_step", "mpx_server_result",
     "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
-    "mpx_pdhg_iterate",
+    "mpx_pdhg_iterate", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
+    "mpx_batch_results", "mpx_batch_get_stats", "mpx_batch_profile", "mpx_batch_destroy",
 ]
 
 
@@ -31,6 +33,7 @@ def lib():
     L.mpx_version.restype = C.c_char_p
     L.mpx_pdlp_default_options.argtypes = [C.POINTER(MpxPdlpOptions)]
     L.mpx_device_count.argtypes = [C.POINTER(C.c_int32)]
+    L.mpx_device_synchronize.argtypes = [C.c_int32]
     L.mpx_pdlp_solve.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions), C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
     L.mpx_pdlp_solve_batch_shared.argtypes = [C.POINTER(MpxLp), C.c_int32, C.c_void_p,
@@ -53,6 +56,15 @@ def lib():
     L.mpx_pdhg_iterate.argtypes = ([C.POINTER(MpxLp), C.POINTER(MpxScaledProblem), C.c_int32,
                                     C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32]
                                    + [C.c_void_p] * 6)
+    L.mpx_batch_create_shared.argtypes = [C.POINTER(MpxLp), C.c_int32, C.c_void_p, C.c_void_p,
+                                          C.POINTER(MpxPdlpOptions), C.POINTER(C.c_void_p)]
+    L.mpx_batch_create.argtypes = [C.POINTER(MpxLp), C.c_int32, C.POINTER(MpxPdlpOptions),
+                                   C.POINTER(C.c_void_p)]
+    L.mpx_batch_run.argtypes = [C.c_void_p]
+    L.mpx_batch_results.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
+    L.mpx_batch_get_stats.argtypes = [C.c_void_p, C.POINTER(MpxBatchStats)]
+    L.mpx_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.POINTER(MpxBatchStats)]
+    L.mpx_batch_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -159,3 +171,74 @@ def spmv_device(rows, cols, rs, ci, val, x, y, group=0, stream=None):
         lib().mpx_spmv_csr_f32
     _check(f(rows, cols, ptr(rs), ptr(ci), ptr(val), ptr(x), ptr(y), group, stream),
            "mpx_spmv_csr")
+
+
+class ResidentBatch:
+    """A batch whose problem data stays in HBM between solves
+    (mpx_batch_create_shared / mpx_batch_create, then mpx_batch_run)."""
+
+    def __init__(self, lp=None, col_lower=None, col_upper=None, lps=None, **opts):
+        L = lib()
+        self._o = opts.pop("_options", None) or make_options(**opts)
+        self._h = C.c_void_p()
+        if lps is not None:
+            self.lps = list(lps)
+            self._clps = (MpxLp * len(self.lps))(*[x.as_c() for x in self.lps])
+            rc = L.mpx_batch_create(self._clps, len(self.lps), C.byref(self._o), C.byref(self._h))
+            self.count = len(self.lps)
+        else:
+            self.lps = [lp]
+            self._clp = lp.as_c()
+            self.col_lower = np.ascontiguousarray(col_lower, dtype=np.float64)
+            self.col_upper = np.ascontiguousarray(col_upper, dtype=np.float64)
+            self.count = self.col_lower.shape[0]
+            rc = L.mpx_batch_create_shared(C.byref(self._clp), self.count,
+                                           self.col_lower.ctypes.data, self.col_upper.ctypes.data,
+                                           C.byref(self._o), C.byref(self._h))
+        _check(rc, "mpx_batch_create")
+
+    def run(self):
+        _check(lib().mpx_batch_run(self._h), "mpx_batch_run")
+
+    def results(self, want_vectors=True):
+        res = (MpxPdlpResult * self.count)()
+        xs = ys = None
+        if want_vectors:
+            if len(self.lps) == 1:
+                xs = [np.zeros(self.lps[0].num_cols) for _ in range(self.count)]
+                ys = [np.zeros(self.lps[0].num_rows) for _ in range(self.count)]
+            else:
+                xs = [np.zeros(l.num_cols) for l in self.lps]
+                ys = [np.zeros(l.num_rows) for l in self.lps]
+            xp = (C.c_void_p * self.count)(*[x.ctypes.data for x in xs])
+            yp = (C.c_void_p * self.count)(*[y.ctypes.data for y in ys])
+            rc = lib().mpx_batch_results(self._h, C.cast(xp, C.c_void_p), C.cast(yp, C.c_void_p), res)
+        else:
+            rc = lib().mpx_batch_results(self._h, None, None, res)
+        _check(rc, "mpx_batch_results")
+        return [r.as_dict() for r in res], xs, ys
+
+    def stats(self):
+        st = MpxBatchStats()
+        _check(lib().mpx_batch_get_stats(self._h, C.byref(st)), "mpx_batch_get_stats")
+        return st.as_dict()
+
+    def profile(self, launches=50):
+        st = MpxBatchStats()
+        _check(lib().mpx_batch_profile(self._h, launches, C.byref(st)), "mpx_batch_profile")
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().mpx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize(device=0):
+    _check(lib().mpx_device_synchronize(device), "mpx_device_synchronize")
