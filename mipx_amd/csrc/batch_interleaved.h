@@ -217,9 +217,16 @@ __device__ inline T laneDot(const int* __restrict__ idx, const T* __restrict__ v
     return acc;
 }
 
-// Primal half step for one column j of 64 instances per wave (4 columns per
-// block).  Shared data: CSC structure/values, c, tau_base.
-template <typename T>
+// CW = columns (rows) per wave in the iteration kernels: their epilogue
+// operands and gathers are issued together so that several memory round trips
+// are in flight per wave; each column still sums its entries in storage
+// order, so every CW gives bit-identical results and the choice is left to
+// the online autotuner.
+
+// Primal half step: each wave handles kCW consecutive columns for the 64
+// instances of its group (4 waves per block).  Shared data: CSC structure and
+// values, c, tau_base.
+template <typename T, int kCW>
 __global__ __launch_bounds__(kBlock) void kILPrimal(
     const BInst* __restrict__ insts, const int* __restrict__ slots, const int* __restrict__ count,
     const ILane* __restrict__ lanes, int n, int m, const int* __restrict__ cs,
@@ -228,33 +235,66 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
     const T* __restrict__ IX0, T* __restrict__ IXR, const T* __restrict__ IL,
     const T* __restrict__ IU, int kiter) {
     const int nslots = *count;
-    const int per_group = (n + 3) / 4;
+    const int per_group = (n + 4 * kCW - 1) / (4 * kCW);
     forGroupItems(nslots, per_group, [&](int g, int jb) {
-        const int j = jb * 4 + static_cast<int>(threadIdx.x) / kLanes;
+        const int j0 = (jb * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW;
         const int lane = static_cast<int>(threadIdx.x) % kLanes;
-        if (j >= n) return;
+        if (j0 >= n) return;
         const int s = g * kLanes + lane;
         const bool live = s < nslots;
         const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
         if (kiter > ln.lim) return;  // lane idle this iteration
-        const T acc = laneDot(ri, valt, cs[j], cs[j + 1],
-                              IY + static_cast<size_t>(g) * m * kLanes + lane);
-        const size_t o = (static_cast<size_t>(g) * n + j) * kLanes + lane;
+        const T* Y = IY + static_cast<size_t>(g) * m * kLanes + lane;
+        const size_t base = (static_cast<size_t>(g) * n + j0) * kLanes + lane;
+        T xv[kCW], x0v[kCW], lv[kCW], uv[kCW], acc[kCW];
+        int qb[kCW], qe[kCW];
+        int maxlen = 0;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) {
+            const bool in = j0 + cc < n;
+            const size_t o = base + static_cast<size_t>(cc) * kLanes;
+            xv[cc] = in ? ldNT(IX + o) : T(0);
+            x0v[cc] = in ? ldNT(IX0 + o) : T(0);
+            lv[cc] = in ? ldNT(IL + o) : T(0);
+            uv[cc] = in ? ldNT(IU + o) : T(0);
+            qb[cc] = in ? cs[j0 + cc] : 0;
+            qe[cc] = in ? cs[j0 + cc + 1] : 0;
+            acc[cc] = T(0);
+            maxlen = max(maxlen, qe[cc] - qb[cc]);
+        }
+        if constexpr (kCW == 1) {
+            acc[0] = laneDot(ri, valt, qb[0], qe[0], Y);
+        } else {
+            for (int t = 0; t < maxlen; ++t) {
+#pragma unroll
+                for (int cc = 0; cc < kCW; ++cc) {
+                    const int q = qb[cc] + t;
+                    if (q < qe[cc]) acc[cc] += valt[q] * Y[static_cast<size_t>(ri[q]) * kLanes];
+                }
+            }
+        }
         const int kk = ln.inner + kiter;
         const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
         const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
-        const T tau = step * tau_base[j] / pw;
-        const T xj = ldNT(IX + o);
-        const T t = xj - tau * (c[j] + acc);
-        const T h = clampRef(t, ldNT(IL + o), ldNT(IU + o));
-        const T r = T(2) * h - xj;
-        IXR[o] = r;  // gathered by the dual step: keep it cacheable
-        stNT(IX + o, lam * r + (T(1) - lam) * ldNT(IX0 + o));
-        if (kiter == ln.lim) vecsOf<T>(insts[slots[s]]).xh[j] = h;
+        const bool last = kiter == ln.lim;
+        T* xh = last ? vecsOf<T>(insts[slots[s]]).xh : nullptr;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) {
+            const int j = j0 + cc;
+            if (j >= n) break;
+            const size_t o = base + static_cast<size_t>(cc) * kLanes;
+            const T tau = step * tau_base[j] / pw;
+            const T t = xv[cc] - tau * (c[j] + acc[cc]);
+            const T h = clampRef(t, lv[cc], uv[cc]);
+            const T r = T(2) * h - xv[cc];
+            IXR[o] = r;  // gathered by the dual step: keep it cacheable
+            stNT(IX + o, lam * r + (T(1) - lam) * x0v[cc]);
+            if (last) xh[j] = h;
+        }
     });
 }
 
-template <typename T>
+template <typename T, int kCW>
 __global__ __launch_bounds__(kBlock) void kILDual(
     const BInst* __restrict__ insts, const int* __restrict__ slots, const int* __restrict__ count,
     const ILane* __restrict__ lanes, int n, int m, const int* __restrict__ rs,
@@ -262,28 +302,59 @@ __global__ __launch_bounds__(kBlock) void kILDual(
     const T* __restrict__ rl, const T* __restrict__ ru, const T* __restrict__ IXR,
     T* __restrict__ IY, const T* __restrict__ IY0, int kiter) {
     const int nslots = *count;
-    const int per_group = (m + 3) / 4;
+    const int per_group = (m + 4 * kCW - 1) / (4 * kCW);
     forGroupItems(nslots, per_group, [&](int g, int ib) {
-        const int i = ib * 4 + static_cast<int>(threadIdx.x) / kLanes;
+        const int i0 = (ib * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW;
         const int lane = static_cast<int>(threadIdx.x) % kLanes;
-        if (i >= m) return;
+        if (i0 >= m) return;
         const int s = g * kLanes + lane;
         const bool live = s < nslots;
         const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
         if (kiter > ln.lim) return;
-        const T acc = laneDot(ci, val, rs[i], rs[i + 1],
-                              IXR + static_cast<size_t>(g) * n * kLanes + lane);
-        const size_t o = (static_cast<size_t>(g) * m + i) * kLanes + lane;
+        const T* X = IXR + static_cast<size_t>(g) * n * kLanes + lane;
+        const size_t base = (static_cast<size_t>(g) * m + i0) * kLanes + lane;
+        T yv[kCW], y0v[kCW], acc[kCW];
+        int qb[kCW], qe[kCW];
+        int maxlen = 0;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) {
+            const bool in = i0 + cc < m;
+            const size_t o = base + static_cast<size_t>(cc) * kLanes;
+            yv[cc] = in ? IY[o] : T(0);  // y is gathered by the next primal step
+            y0v[cc] = in ? ldNT(IY0 + o) : T(0);
+            qb[cc] = in ? rs[i0 + cc] : 0;
+            qe[cc] = in ? rs[i0 + cc + 1] : 0;
+            acc[cc] = T(0);
+            maxlen = max(maxlen, qe[cc] - qb[cc]);
+        }
+        if constexpr (kCW == 1) {
+            acc[0] = laneDot(ci, val, qb[0], qe[0], X);
+        } else {
+            for (int t = 0; t < maxlen; ++t) {
+#pragma unroll
+                for (int cc = 0; cc < kCW; ++cc) {
+                    const int q = qb[cc] + t;
+                    if (q < qe[cc]) acc[cc] += val[q] * X[static_cast<size_t>(ci[q]) * kLanes];
+                }
+            }
+        }
         const int kk = ln.inner + kiter;
         const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
         const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
-        const T sig = step * sigma_base[i] * pw;
-        const T yi = IY[o];  // y is gathered by the next primal step: cacheable
-        const T v = yi + sig * acc;
-        const T h = dualProx(v, sig, rl[i], ru[i]);
-        const T r = T(2) * h - yi;
-        IY[o] = lam * r + (T(1) - lam) * ldNT(IY0 + o);
-        if (kiter == ln.lim) vecsOf<T>(insts[slots[s]]).yh[i] = h;
+        const bool last = kiter == ln.lim;
+        T* yh = last ? vecsOf<T>(insts[slots[s]]).yh : nullptr;
+#pragma unroll
+        for (int cc = 0; cc < kCW; ++cc) {
+            const int i = i0 + cc;
+            if (i >= m) break;
+            const size_t o = base + static_cast<size_t>(cc) * kLanes;
+            const T sig = step * sigma_base[i] * pw;
+            const T v = yv[cc] + sig * acc[cc];
+            const T h = dualProx(v, sig, rl[i], ru[i]);
+            const T r = T(2) * h - yv[cc];
+            IY[o] = lam * r + (T(1) - lam) * y0v[cc];
+            if (last) yh[i] = h;
+        }
     });
 }
 

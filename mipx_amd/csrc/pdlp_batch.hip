@@ -1094,6 +1094,8 @@ private:
     unsigned long long* stamp_ = nullptr;  // device wall clock at the end of prepare()
     BCtrl* ctrl_all_ = nullptr;
     bool allow_interleaved_ = false;
+    int cw_ = 1;  // columns per wave of the lane-per-instance kernels
+    int tuned_cw_ = 0;
     // Lane-per-instance mode (every instance shares owner 0).
     bool interleaved_ = false;
     std::unique_ptr<DevBuf> d_il_buf_;
@@ -1384,7 +1386,17 @@ void BatchEngine::setup() {
     MPX_HIP(hipGetDevice(&dev));
     MPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     grid_ = std::max(kNumXcd, (cus * 8 / kNumXcd) * kNumXcd);
-    cur_ = {0, 0};
+    // Online autotuning of the lane-per-instance kernels: columns per wave.
+    if (interleaved_) {
+        if (tuned_cw_ > 0) {
+            cur_ = {tuned_cw_, tuned_cw_};
+        } else {
+            tune_left_ = {{4, 4}, {2, 2}, {1, 1}};
+            cur_ = {1, 1};
+        }
+    } else {
+        cur_ = {0, 0};
+    }
     d_owners_buf_ = std::make_unique<DevBuf>(sizeof(BOwner) * std::max(1, no));
     d_presets_buf_ = std::make_unique<DevBuf>(sizeof(int) * std::max(1, no));
     d_insts_buf_ = std::make_unique<DevBuf>(sizeof(BInst) * std::max(1, ni));
@@ -1490,6 +1502,15 @@ void BatchEngine::enqueueRound(const IterPreset& p, bool timed) {
     kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
 }
 
+template <typename F>
+void withCW(int cw, F&& f) {
+    switch (cw) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        default: f(std::integral_constant<int, 1>{}); break;
+    }
+}
+
 void BatchEngine::enqueueInterleavedRound(bool timed) {
     const int K = count();
     const BInst* di = static_cast<const BInst*>(d_insts_buf_->p);
@@ -1510,21 +1531,23 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
                                                    nullptr, nullptr);
     if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
     for (int k = 1; k <= bo_.N; ++k) {
-        if (mixed_) {
-            kILPrimal<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, lanes32_, n, m, w.cs,
-                                                        w.ri, w.valtf, w.cf, w.tauf, il32_.y,
-                                                        il32_.x, il32_.x0, il32_.xr, il32_.l,
-                                                        il32_.u, k);
-            kILDual<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, lanes32_, n, m, w.rs,
-                                                      w.ci, w.valf, w.sigf, w.rlf, w.ruf, il32_.xr,
-                                                      il32_.y, il32_.y0, k);
-        }
-        kILPrimal<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, lanes64_, n, m, w.cs, w.ri,
-                                                     w.valt, w.c, w.tau, il64_.y, il64_.x,
-                                                     il64_.x0, il64_.xr, il64_.l, il64_.u, k);
-        kILDual<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, lanes64_, n, m, w.rs, w.ci,
-                                                   w.val, w.sig, w.rl, w.ru, il64_.xr, il64_.y,
-                                                   il64_.y0, k);
+        withCW(cw_, [&](auto CWI) {
+            constexpr int CW = decltype(CWI)::value;
+            if (mixed_) {
+                kILPrimal<float, CW><<<grid_, kBlock, 0, st_>>>(
+                    di, slots32_, c32, lanes32_, n, m, w.cs, w.ri, w.valtf, w.cf, w.tauf, il32_.y,
+                    il32_.x, il32_.x0, il32_.xr, il32_.l, il32_.u, k);
+                kILDual<float, CW><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, lanes32_, n, m,
+                                                              w.rs, w.ci, w.valf, w.sigf, w.rlf,
+                                                              w.ruf, il32_.xr, il32_.y, il32_.y0, k);
+            }
+            kILPrimal<double, CW><<<grid_, kBlock, 0, st_>>>(
+                di, slots64_, c64, lanes64_, n, m, w.cs, w.ri, w.valt, w.c, w.tau, il64_.y,
+                il64_.x, il64_.x0, il64_.xr, il64_.l, il64_.u, k);
+            kILDual<double, CW><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, lanes64_, n, m, w.rs,
+                                                           w.ci, w.val, w.sig, w.rl, w.ru, il64_.xr,
+                                                           il64_.y, il64_.y0, k);
+        });
     }
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
     if (mixed_) {
@@ -1566,25 +1589,29 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
         }
         for (int k = 1; k <= launches; ++k) {
             MPX_HIP(hipEventRecord(e[0], st_));
-            if (f32)
-                kILPrimal<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.cs, w.ri,
-                                                            w.valtf, w.cf, w.tauf, il32_.y,
-                                                            il32_.x, il32_.x0, il32_.xr, il32_.l,
-                                                            il32_.u, k);
-            else
-                kILPrimal<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.cs, w.ri,
-                                                             w.valt, w.c, w.tau, il64_.y, il64_.x,
-                                                             il64_.x0, il64_.xr, il64_.l, il64_.u,
-                                                             k);
+            withCW(cw_, [&](auto CWI) {
+                constexpr int CW = decltype(CWI)::value;
+                if (f32)
+                    kILPrimal<float, CW><<<grid_, kBlock, 0, st_>>>(
+                        di, sl, cnt, ln, n, m, w.cs, w.ri, w.valtf, w.cf, w.tauf, il32_.y, il32_.x,
+                        il32_.x0, il32_.xr, il32_.l, il32_.u, k);
+                else
+                    kILPrimal<double, CW><<<grid_, kBlock, 0, st_>>>(
+                        di, sl, cnt, ln, n, m, w.cs, w.ri, w.valt, w.c, w.tau, il64_.y, il64_.x,
+                        il64_.x0, il64_.xr, il64_.l, il64_.u, k);
+            });
             MPX_HIP(hipEventRecord(e[1], st_));
-            if (f32)
-                kILDual<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs, w.ci, w.valf,
-                                                          w.sigf, w.rlf, w.ruf, il32_.xr, il32_.y,
-                                                          il32_.y0, k);
-            else
-                kILDual<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs, w.ci, w.val,
-                                                           w.sig, w.rl, w.ru, il64_.xr, il64_.y,
-                                                           il64_.y0, k);
+            withCW(cw_, [&](auto CWI) {
+                constexpr int CW = decltype(CWI)::value;
+                if (f32)
+                    kILDual<float, CW><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs, w.ci,
+                                                                  w.valf, w.sigf, w.rlf, w.ruf,
+                                                                  il32_.xr, il32_.y, il32_.y0, k);
+                else
+                    kILDual<double, CW><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs,
+                                                                   w.ci, w.val, w.sig, w.rl, w.ru,
+                                                                   il64_.xr, il64_.y, il64_.y0, k);
+            });
             MPX_HIP(hipEventRecord(e[2], st_));
             MPX_HIP(hipEventSynchronize(e[2]));
             float a = 0.0f, b = 0.0f;
@@ -1644,6 +1671,7 @@ std::vector<double> BatchEngine::finishTimesMs() {
 }
 
 void BatchEngine::launchRound(const IterPreset& p, bool timed) {
+    if (interleaved_) cw_ = p.gc > 0 ? p.gc : 1;
     if (timed || graph_failed_) {
         if (interleaved_) enqueueInterleavedRound(timed);
         else enqueueRound(p, timed);
@@ -1694,6 +1722,7 @@ int BatchEngine::step(int rounds) {
                 std::sort(tune_res_.begin(), tune_res_.end(),
                           [](const auto& a, const auto& b) { return a.first < b.first; });
                 cur_ = tune_res_.front().second;
+                if (interleaved_) tuned_cw_ = cur_.gc;
             }
             continue;
         }

@@ -143,8 +143,9 @@ def test_solve_mixed_precision(seed, m, n):
     import json, os
     from lp_cases import GOLDEN
     fixture = json.load(open(os.path.join(GOLDEN, "highs_optima.json")))["cases"]
-    opt = [c["objective"] for c in fixture if c["seed"] == seed][0]
-    lp = lpgen.c0_plain(seed=seed, m=m, n=n).lp
+    case = [c for c in fixture if c["seed"] == seed][0]
+    opt = case["objective"]
+    lp = lpgen.c0_plain(seed=seed, m=m, n=n, per_col=case["per_col"]).lp
     r, x, y = native.solve(lp, precision=1, **TIGHT)
     ro, _, _ = ob.solve(lp, **TIGHT)
     print(r)
