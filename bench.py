@@ -90,10 +90,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c2"])
     ap.add_argument("--count", type=int, default=4096)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="upload the workload and time --launches launches of the iteration "
+                         "kernels one by one (for rocprofv3), then a 1 GiB device copy")
+    ap.add_argument("--launches", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -107,6 +111,19 @@ def main():
     lp = g.lp
     log("uploading (resident batch)")
     batch = native.ResidentBatch(lp, lo, hi, precision=prec, **TOL)
+    if a.profile_only:
+        prof = batch.profile(launches=a.launches)
+        sv = 4 if prec == 1 else 8
+        bl = primal_kernel_bytes(lp, prof["profiled_slots"], sv)
+        print(json.dumps({"primal_kernel_us": prof["primal_kernel_us"],
+                          "dual_kernel_us": prof["dual_kernel_us"],
+                          "slots": prof["profiled_slots"], "launches": prof["profiled_launches"],
+                          "algorithmic_bytes_per_launch": bl,
+                          "achieved_GBs": bl / (prof["primal_kernel_us"] * 1e-6) / 1e9}), flush=True)
+        batch.close()
+        copied = native.copy_bench(1 << 30)
+        log(f"device copy of {copied} bytes done (counter calibration)")
+        return
     for w in range(a.warmup):
         t = time.perf_counter()
         batch.run()

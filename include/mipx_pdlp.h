@@ -129,14 +129,18 @@ typedef struct mpx_pdlp_options {
                                          within 10x of the tolerances (or FP32
                                          stalls); FP64 iterations then refine
                                          until primal, dual and gap are below
-                                         refine_tol (<= 0: the residual
-                                         tolerances and half the gap
-                                         tolerance) */
+                                         refine_tol (<= 0: the solve's own
+                                         tolerances) */
     int32_t spmv_group;               /* lanes per row/column in the SpMV
                                          kernels, 0 = autotuned.  1 gives the
                                          sequential per-row summation order of
                                          the reference's CPU SpMV. */
     int32_t device;                   /* HIP device ordinal, default 0 */
+    double mixed_handover;            /* mixed precision: FP32 hands over to
+                                         FP64 once max(primal/primal_tol,
+                                         dual/dual_tol, gap/optimality_tol),
+                                         evaluated in FP64, is at most this
+                                         (default 3), or when FP32 stalls */
 } mpx_pdlp_options;
 
 /* Mirrors `mipx::LpResult` (include/mipx/lp_solver.h:11-16) plus the final
@@ -165,6 +169,10 @@ const char* mpx_version(void);
 int mpx_device_count(int32_t* count);
 /* Waits for all work on the device (hipDeviceSynchronize). */
 int mpx_device_synchronize(int32_t device);
+/* Copies `bytes` bytes device-to-device with a plain streaming kernel
+ * (read `bytes`, write `bytes`); used to calibrate HBM counters.  Returns the
+ * kernel time in milliseconds in *ms. */
+int mpx_copy_bench(int64_t bytes, double* ms);
 
 /*
  * Solve one LP on the GPU.  Replaces `PdlpSolver::solveGpu()`

@@ -47,6 +47,7 @@ struct BOptions {
     double primal_tol, dual_tol, opt_tol, polish_p_tol, polish_d_tol;
     double refine_p_tol, refine_d_tol, refine_g_tol;  // FP64 phase of mixed precision
     int32_t mixed, pad;
+    double handover;  // FP32 -> FP64 at this KKT ratio
     double pid_kp, pid_ki, pid_kd, pid_i_smooth;
     double rs_suff, rs_nec, rs_art, precond_ratio;
 };

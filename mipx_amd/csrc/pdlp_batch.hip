@@ -103,7 +103,7 @@ enum BAction { kActNone = 0, kActRestart = 1, kActToFp64 = 2, kActConverged = 3,
 // Mixed precision: FP32 hands over to FP64 once the FP64-judged KKT ratio
 // (max of residual / tolerance) is within this factor, or after this many
 // evaluations without halving it.
-constexpr double kFp32ExitRatio = 10.0;
+constexpr double kFp32ExitRatio = 3.0;
 constexpr int kFp32Stall = 20;
 
 template <typename T>
@@ -380,10 +380,10 @@ __device__ void evalControl(const BInst& in, const BOptions& o) {
                 }
             }
         } else if (kFp32) {
-            // FP32 iterations judged in FP64: hand over to FP64 within a
-            // decade of the tolerances, or when FP32 stops making progress.
+            // FP32 iterations judged in FP64: hand over to FP64 once the KKT
+            // ratio is within o.handover, or when FP32 stops making progress.
             const double k = kktRatio(e, o);
-            bool handover = k <= kFp32ExitRatio;
+            bool handover = k <= o.handover;
             if (k < 0.5 * c.best_kkt) {
                 c.best_kkt = k;
                 c.stall = 0;
@@ -992,14 +992,12 @@ public:
         bo_.precond_ratio = o.preconditioner_refresh_ratio;
         mixed_ = o.precision == MPX_PREC_MIXED;
         bo_.mixed = mixed_;
+        bo_.handover = o.mixed_handover > 0.0 ? o.mixed_handover : kFp32ExitRatio;
         // FP64 phase of mixed precision: refine_tol, or the solve's own tolerances.
         bo_.refine_p_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.primal_tol) : o.primal_tol;
         bo_.refine_d_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.dual_tol) : o.dual_tol;
-        // The gap target is halved: the reference's relative gap divides by
-        // 1 + |pobj| + |dobj| (about twice |objective|), so gap <= tol/2 keeps
-        // the objective within tol of the optimum.
         bo_.refine_g_tol =
-            o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol) : 0.5 * o.optimality_tol;
+            o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol) : o.optimality_tol;
     }
     ~BatchEngine() {
         for (auto& kv : graphs_) kv.second.reset();
@@ -1096,6 +1094,7 @@ private:
     bool allow_interleaved_ = false;
     int cw_ = 1;  // columns per wave of the lane-per-instance kernels
     int tuned_cw_ = 0;
+    bool fp32_gone_ = false;  // no instance left in its FP32 phase
     // Lane-per-instance mode (every instance shares owner 0).
     bool interleaved_ = false;
     std::unique_ptr<DevBuf> d_il_buf_;
@@ -1425,6 +1424,7 @@ void BatchEngine::prepare() {
     for (auto& kv : graphs_) kv.second.reset();
     graphs_.clear();
     rounds_ = 0;
+    fp32_gone_ = false;
     if (no) {
         kOwnerScale<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p));
         kOwnerPower<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p),
@@ -1489,8 +1489,9 @@ void BatchEngine::enqueueRound(const IterPreset& p, bool timed) {
     kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
     kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
     if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
+    const bool f32 = mixed_ && !fp32_gone_;
     for (int k = 1; k <= bo_.N; ++k) {
-        if (mixed_) {
+        if (f32) {
             kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
             kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, tr32_, totals_ + 1, k);
         }
@@ -1519,7 +1520,10 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     kSlotLists<<<1, 1024, 0, st_>>>(di, K, slots32_, slots64_, lanes32_, lanes64_, slot_counts_);
     const int* c32 = slot_counts_;
     const int* c64 = slot_counts_ + 1;
-    if (mixed_) {
+    // FP32 kernels only while some instance is still in its FP32 phase
+    // (instances never return to FP32, so this is decided on the host).
+    const bool f32 = mixed_ && !fp32_gone_;
+    if (f32) {
         kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, n, 0, il32_.x, il32_.x0,
                                                       il32_.l, il32_.u);
         kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, m, 1, il32_.y, il32_.y0,
@@ -1533,7 +1537,7 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     for (int k = 1; k <= bo_.N; ++k) {
         withCW(cw_, [&](auto CWI) {
             constexpr int CW = decltype(CWI)::value;
-            if (mixed_) {
+            if (f32) {
                 kILPrimal<float, CW><<<grid_, kBlock, 0, st_>>>(
                     di, slots32_, c32, lanes32_, n, m, w.cs, w.ri, w.valtf, w.cf, w.tauf, il32_.y,
                     il32_.x, il32_.x0, il32_.xr, il32_.l, il32_.u, k);
@@ -1550,13 +1554,13 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
         });
     }
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
-    if (mixed_) {
+    if (f32) {
         kDeinterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, n, 0, il32_.x);
         kDeinterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, m, 1, il32_.y);
     }
     kDeinterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, n, 0, il64_.x);
     kDeinterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, m, 1, il64_.y);
-    if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    if (f32) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
     kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
 }
 
@@ -1678,7 +1682,7 @@ void BatchEngine::launchRound(const IterPreset& p, bool timed) {
         MPX_HIP(hipGetLastError());
         return;
     }
-    G& g = graphs_[{p.gc, p.gr}];
+    G& g = graphs_[{p.gc, p.gr * 2 + (fp32_gone_ ? 1 : 0)}];
     if (g.e == nullptr) {
         bool ok = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
@@ -1734,7 +1738,17 @@ int BatchEngine::step(int rounds) {
                                                         K, dcount);
     MPX_HIP(hipGetLastError());
     int active = 0;
+    int n32 = 1;
     MPX_HIP(hipMemcpyAsync(&active, dcount, sizeof(int), hipMemcpyDeviceToHost, st_));
+    if (mixed_ && !fp32_gone_) {
+        // FP32 instances at the start of the last round (0 stays 0).
+        const int* src = interleaved_ ? slot_counts_ : totals_;
+        int c2[2] = {1, 1};
+        MPX_HIP(hipMemcpyAsync(c2, src, sizeof(c2), hipMemcpyDeviceToHost, st_));
+        MPX_HIP(hipStreamSynchronize(st_));
+        n32 = interleaved_ ? c2[0] : c2[0] + c2[1];
+        if (rounds_ > 0 && n32 == 0) fp32_gone_ = true;
+    }
     MPX_HIP(hipStreamSynchronize(st_));
     return active;
 }

@@ -880,11 +880,8 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
     const double rfac = 1.0;
     const double rtol_p = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.primal_tol) : rfac * o.primal_tol;
     const double rtol_d = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.dual_tol) : rfac * o.dual_tol;
-    // Gap target halved: the relative gap divides by 1 + |pobj| + |dobj|
-    // (about twice |objective|), so gap <= tol/2 keeps the objective within
-    // tol of the optimum.
-    const double rtol_g = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol)
-                                             : 0.5 * rfac * o.optimality_tol;
+    const double rtol_g =
+        o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol) : rfac * o.optimality_tol;
     auto kktOf = [&](const EvalOut& ev) {
         const double tp = o.optimality_norm == 1 ? ev.rel_p_inf : ev.rel_p;
         const double td = o.optimality_norm == 1 ? ev.rel_d_inf : ev.rel_d;
@@ -950,10 +947,10 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
                         total, e.pobj, tp, td, e.gap, e.fpe, pw_, step_, fp32_phase ? " fp32" : "");
         if (fp32_phase) {
             // FP32 iterations judged in FP64: hand over to FP64 (restarted at
-            // the FP32 point) within a decade of the tolerances, or when FP32
+            // the FP32 point) once the KKT ratio is within mixed_handover, or when FP32
             // stops making progress.
             const double k = kktOf(e);
-            bool handover = k <= 10.0;
+            bool handover = k <= o.mixed_handover;
             if (k < 0.5 * best_kkt) {
                 best_kkt = k;
                 stall = 0;
@@ -1223,6 +1220,7 @@ void mpx_pdlp_default_options(mpx_pdlp_options* o) {
     o->refine_tol = -1.0;
     o->spmv_group = 0;
     o->device = 0;
+    o->mixed_handover = 3.0;
 }
 
 const char* mpx_last_error(void) { return mpx::lastError(); }
@@ -1233,6 +1231,40 @@ int mpx_device_count(int32_t* count) {
     if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
     if (count) *count = c;
     return MPX_OK;
+}
+
+__global__ void kCopyBench(long long n4, const int4* __restrict__ src, int4* __restrict__ dst) {
+    for (long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; t < n4;
+         t += static_cast<long long>(gridDim.x) * kBlock)
+        dst[t] = src[t];
+}
+
+int mpx_copy_bench(int64_t bytes, double* ms) {
+    if (bytes <= 0 || bytes % 16 != 0) return MPX_ERR_INVALID;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MPX_OK;
+    try {
+        MPX_HIP(hipMalloc(&a, bytes));
+        MPX_HIP(hipMalloc(&b, bytes));
+        MPX_HIP(hipMemset(a, 1, bytes));
+        MPX_HIP(hipEventCreate(&e0));
+        MPX_HIP(hipEventCreate(&e1));
+        MPX_HIP(hipEventRecord(e0, nullptr));
+        kCopyBench<<<4096, kBlock>>>(bytes / 16, static_cast<const int4*>(a), static_cast<int4*>(b));
+        MPX_HIP(hipEventRecord(e1, nullptr));
+        MPX_HIP(hipEventSynchronize(e1));
+        float t = 0.0f;
+        MPX_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (ms) *ms = t;
+    } catch (const DeviceError&) {
+        rc = MPX_ERR_DEVICE;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int mpx_device_synchronize(int32_t device) {

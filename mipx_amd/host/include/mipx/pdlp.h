@@ -68,6 +68,7 @@ struct PdlpOptions {
     PdlpPrecision precision = PdlpPrecision::FP64;
     Int spmv_group = 0;
     Int device = 0;
+    Real mixed_handover = 3.0;  // FP32 -> FP64 at this KKT ratio (see mipx_pdlp.h)
 };
 
 class PdlpSolver : public LpSolver {

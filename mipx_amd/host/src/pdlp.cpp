@@ -72,6 +72,7 @@ mpx_pdlp_options toC(const PdlpOptions& p) {
     o.precision = p.precision == PdlpPrecision::Mixed ? MPX_PREC_MIXED : MPX_PREC_FP64;
     o.spmv_group = p.spmv_group;
     o.device = p.device;
+    o.mixed_handover = p.mixed_handover;
     return o;
 }
 

@@ -15,7 +15,7 @@ _lib = None
 
 EXPORTED = [
     "mpx_pdlp_default_options", "mpx_last_error", "mpx_version", "mpx_device_count",
-    "mpx_device_synchronize",
+    "mpx_device_synchronize", "mpx_copy_bench",
     "mpx_pdlp_solve", "mpx_pdlp_solve_batch_shared", "mpx_pdlp_solve_batch",
     "mpx_server_create", "mpx_server_submit", "mpx_server_step", "mpx_server_result",
     "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
@@ -34,6 +34,7 @@ def lib():
     L.mpx_pdlp_default_options.argtypes = [C.POINTER(MpxPdlpOptions)]
     L.mpx_device_count.argtypes = [C.POINTER(C.c_int32)]
     L.mpx_device_synchronize.argtypes = [C.c_int32]
+    L.mpx_copy_bench.argtypes = [C.c_int64, C.POINTER(C.c_double)]
     L.mpx_pdlp_solve.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions), C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
     L.mpx_pdlp_solve_batch_shared.argtypes = [C.POINTER(MpxLp), C.c_int32, C.c_void_p,
@@ -242,3 +243,13 @@ class ResidentBatch:
 
 def synchronize(device=0):
     _check(lib().mpx_device_synchronize(device), "mpx_device_synchronize")
+
+
+def copy_bench(nbytes):
+    """Device-to-device copy of nbytes with a plain streaming kernel; returns
+    nbytes (the kernel time is available as copy_bench_ms)."""
+    ms = C.c_double(0.0)
+    _check(lib().mpx_copy_bench(nbytes, C.byref(ms)), "mpx_copy_bench")
+    global copy_bench_ms
+    copy_bench_ms = ms.value
+    return nbytes
