@@ -124,13 +124,11 @@ typedef struct mpx_pdlp_options {
     void* stop_user;                  /* NULL */
     /* ---- MI355X extensions ---- */
     int32_t precision;                /* MPX_PREC_FP64 */
-    double refine_tol;                /* mixed precision: FP32 iterations run
-                                         until the FP64-evaluated KKT ratio is
-                                         within 10x of the tolerances (or FP32
-                                         stalls); FP64 iterations then refine
-                                         until primal, dual and gap are below
-                                         refine_tol (<= 0: the solve's own
-                                         tolerances) */
+    double refine_tol;                /* mixed precision: after the handover
+                                         (see mixed_handover) FP64 iterations
+                                         refine until primal, dual and gap are
+                                         below refine_tol (<= 0: the solve's
+                                         own tolerances) */
     int32_t spmv_group;               /* lanes per row/column in the SpMV
                                          kernels, 0 = autotuned.  1 gives the
                                          sequential per-row summation order of
@@ -261,24 +259,66 @@ int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats);
 int mpx_batch_destroy(mpx_batch* b);
 
 /* ---- serving loop (continuous batching) ---------------------------------
- * A server owns a device context and an HBM budget.  Submitted requests are
- * admitted into device slots as memory and slots free up; every admitted LP
- * iterates in the same persistent launch as the others; finished slots are
- * recycled while others keep running.  Results are identical to a solo
- * mpx_pdlp_solve_batch() of the same LP with the same options.
+ * No single counterpart in the reference: its GPU path allocates, solves and
+ * frees one LP per PdlpSolver::solveGpu() call (src/lp/pdlp.cpp:1767-1897,
+ * 2751-2784), and concurrent solves are separate threads with separate
+ * streams (src/mip/mip_solver.cpp:4499-4501).  A server instead keeps one
+ * device context, a pool of device memory of `hbm_budget_bytes` (<= 0: half
+ * of the free device memory) and `max_slots` instance slots (<= 0: 1024).
+ *
+ * Requests are admitted first-in first-out into free slots while the pool
+ * has room; a fresh LP's scaling and power iteration run on a side stream
+ * while the running instances keep iterating; each mpx_server_step() runs
+ * one round (termination_eval_frequency iterations, then the per-instance
+ * evaluation / restart / termination logic) of every running instance in
+ * one launch sequence, retires the finished ones and recycles their slots
+ * and memory.  Every instance's arithmetic depends only on its own data, so
+ * a request's result is bit-identical to mpx_pdlp_solve_batch() of the same
+ * LP alone with the same options (a bounds variant: of the base LP with
+ * those column bounds), whatever else is in flight.
+ *
+ * Per request, max_solve_seconds (if >= 0) counts from its activation;
+ * stop_fn is not polled.  Calls on one server must not overlap.
  */
 typedef struct mpx_server mpx_server;
+
+typedef struct mpx_server_stats {
+    int64_t submitted;
+    int64_t finished;
+    int32_t queued;            /* waiting for a slot / memory */
+    int32_t preparing;         /* scaling + power iteration in flight */
+    int32_t running;           /* in slots, iterating */
+    int32_t slots;             /* max_slots */
+    int64_t rounds;            /* rounds launched */
+    double pool_bytes;         /* device memory pool (the HBM budget) */
+    double pool_used_bytes;    /* in use now (bases + requests in flight) */
+    double pool_peak_bytes;
+    double iterate_ms;         /* host wall time spent in rounds (launch + wait) */
+    int64_t instance_iterations; /* PDHG iterations of finished requests */
+} mpx_server_stats;
+
 int mpx_server_create(const mpx_pdlp_options* opts, double hbm_budget_bytes,
                       int32_t max_slots, mpx_server** out);
+/* Uploads a base LP whose scaled matrix, objective, row bounds and power
+ * iteration are shared by the bounds variants submitted against it (the
+ * branch-and-bound node relaxations of one MIP).  Kept until destroy. */
+int mpx_server_add_base(mpx_server* srv, const mpx_lp* base, int32_t* base_id);
 /* Copies the LP; returns a ticket >= 0 in *ticket. */
 int mpx_server_submit(mpx_server* srv, const mpx_lp* lp, int64_t* ticket);
-/* Runs one scheduling round: admit, iterate one block, retire.  Returns the
- * number of requests still pending or running in *active. */
+/* A variant of base `base_id` with its own column bounds (copied). */
+int mpx_server_submit_bounds(mpx_server* srv, int32_t base_id, const double* col_lower,
+                             const double* col_upper, int64_t* ticket);
+/* Runs one scheduling round: activate prepared requests, admit queued ones,
+ * iterate one round, retire finished ones.  Returns the number of requests
+ * still queued, preparing or running in *active. */
 int mpx_server_step(mpx_server* srv, int32_t* active);
-/* Fetches a finished result (x_out/y_out may be NULL).  *done = 0 if the
- * ticket is not finished yet. */
+/* Fetches a finished result once (x_out[n] / y_out[m] may be NULL); the
+ * server then forgets the ticket.  *done = 0 if it is not finished yet.
+ * res->seconds is the time from submit to completion, res->setup_seconds
+ * the part before the request started iterating (queue + scaling). */
 int mpx_server_result(mpx_server* srv, int64_t ticket, int32_t* done,
                       double* x_out, double* y_out, mpx_pdlp_result* res);
+int mpx_server_get_stats(mpx_server* srv, mpx_server_stats* stats);
 int mpx_server_destroy(mpx_server* srv);
 
 /* ---- device-pointer kernels (lower-level entry points) -------------------

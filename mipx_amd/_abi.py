@@ -219,3 +219,23 @@ class MpxBatchStats(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class MpxServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64),
+        ("finished", C.c_int64),
+        ("queued", C.c_int32),
+        ("preparing", C.c_int32),
+        ("running", C.c_int32),
+        ("slots", C.c_int32),
+        ("rounds", C.c_int64),
+        ("pool_bytes", C.c_double),
+        ("pool_used_bytes", C.c_double),
+        ("pool_peak_bytes", C.c_double),
+        ("iterate_ms", C.c_double),
+        ("instance_iterations", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}

@@ -956,11 +956,261 @@ size_t instBytes(int m, int n, bool own_bounds) {
     return b;
 }
 
+// Host-side data of one distinct matrix that its device setup needs besides
+// the LP: A' in CSR (= A in CSC) with the CSR position of every entry, and
+// the power-iteration start vector.
+struct OwnerHost {
+    std::vector<int> cs, ri, perm;
+    std::vector<double> start;
+};
+
+OwnerHost ownerHostData(const mpx_lp& lp) {
+    OwnerHost h;
+    hostTranspose(lp.num_rows, lp.num_cols, lp.row_starts, lp.col_indices, h.cs, h.ri, h.perm);
+    // Power-iteration start vector, exactly as the reference (pdlp.cpp:756-764).
+    std::mt19937 rng(42);
+    std::normal_distribution<double> dist(0.0, 1.0);
+    h.start.resize(static_cast<size_t>(lp.num_cols));
+    for (double& v : h.start) v = dist(rng);
+    double s = 0.0;
+    for (double v : h.start) s += v * v;
+    double xn = std::sqrt(s);
+    if (xn < 1e-15) xn = 1.0;
+    for (double& v : h.start) v /= xn;
+    return h;
+}
+
+// Lane-group preset of a matrix: the heuristic, or the one with
+// `spmv_group` lanes per column when the caller forces a group size.
+int ownerPreset(const mpx_pdlp_options& o, int nnz, int m, int n) {
+    if (o.spmv_group <= 0) return heuristicPreset(nnz, m, n);
+    if (o.spmv_group == 1) return 0;
+    int best = 0;
+    for (int p = 0; p < kNumPresets; ++p)
+        if (presetGC(p) <= o.spmv_group) best = p;
+    return best;
+}
+
+// Carves the device arrays of one distinct matrix from `cv`, queues the
+// uploads of `lp` (and of `h`) on `st` and fills `b`.
+void carveOwner(const mpx_lp& lp, const OwnerHost& h, const mpx_pdlp_options& o, Carve& cv,
+                hipStream_t st, BOwner& b) {
+    const int m = lp.num_rows, n = lp.num_cols, nnz = lp.row_starts[m];
+    std::memset(&b, 0, sizeof(b));
+    b.m = m;
+    b.n = n;
+    b.nnz = nnz;
+    b.do_ruiz = o.do_ruiz_scaling;
+    b.ruiz_iters = o.ruiz_iterations;
+    b.do_pc = o.do_pock_chambolle_scaling;
+    b.do_rescale = o.do_bound_obj_rescaling;
+    b.norm_inf = o.optimality_norm == 1;
+    b.sense_max = lp.sense != 0;
+    b.offset = lp.obj_offset;
+    b.sv_max_iter = o.sv_max_iter;
+    b.sv_tol = o.sv_tol;
+    int* rs = cv.take<int>(m + 1);
+    int* ci = cv.take<int>(nnz);
+    int* ri = cv.take<int>(nnz);
+    int* perm = cv.take<int>(nnz);
+    int* cs = cv.take<int>(n + 1);
+    double* val_o = cv.take<double>(nnz);
+    b.val = cv.take<double>(nnz);
+    b.valt = cv.take<double>(nnz);
+    double* obj_o = cv.take<double>(n);
+    double* lo_o = cv.take<double>(n);
+    double* hi_o = cv.take<double>(n);
+    b.c = cv.take<double>(n);
+    b.l = cv.take<double>(n);
+    b.u = cv.take<double>(n);
+    b.cmax = cv.take<double>(n);
+    b.col_scale = cv.take<double>(n);
+    b.tau = cv.take<double>(n);
+    b.px = cv.take<double>(n);
+    b.pxn = cv.take<double>(n);
+    double* rl_o = cv.take<double>(m);
+    double* ru_o = cv.take<double>(m);
+    b.rl = cv.take<double>(m);
+    b.ru = cv.take<double>(m);
+    b.row_scale = cv.take<double>(m);
+    b.sig = cv.take<double>(m);
+    b.py = cv.take<double>(m);
+    b.valf = cv.take<float>(nnz);
+    b.valtf = cv.take<float>(nnz);
+    b.cf = cv.take<float>(n);
+    b.tauf = cv.take<float>(n);
+    b.sigf = cv.take<float>(m);
+    b.rlf = cv.take<float>(m);
+    b.ruf = cv.take<float>(m);
+    b.scal = cv.take<double>(8);
+    MPX_HIP(hipMemcpyAsync(rs, lp.row_starts, (m + 1) * 4ull, hipMemcpyHostToDevice, st));
+    if (nnz) {
+        MPX_HIP(hipMemcpyAsync(ci, lp.col_indices, nnz * 4ull, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(ri, h.ri.data(), nnz * 4ull, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(perm, h.perm.data(), nnz * 4ull, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(val_o, lp.values, nnz * 8ull, hipMemcpyHostToDevice, st));
+    }
+    MPX_HIP(hipMemcpyAsync(cs, h.cs.data(), (n + 1) * 4ull, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(obj_o, lp.obj, n * 8ull, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(lo_o, lp.col_lower, n * 8ull, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(hi_o, lp.col_upper, n * 8ull, hipMemcpyHostToDevice, st));
+    if (m) {
+        MPX_HIP(hipMemcpyAsync(rl_o, lp.row_lower, m * 8ull, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(ru_o, lp.row_upper, m * 8ull, hipMemcpyHostToDevice, st));
+    }
+    MPX_HIP(hipMemcpyAsync(b.px, h.start.data(), n * 8ull, hipMemcpyHostToDevice, st));
+    b.rs = rs;
+    b.ci = ci;
+    b.cs = cs;
+    b.ri = ri;
+    b.perm = perm;
+    b.val_o = val_o;
+    b.obj_o = obj_o;
+    b.lo_o = lo_o;
+    b.hi_o = hi_o;
+    b.rl_o = rl_o;
+    b.ru_o = ru_o;
+}
+
+// Carves the device arrays of one instance of matrix `w` (its own column
+// bounds when lo/hi are given, queued for upload on `st`) and fills the
+// instance descriptor and its initialisation arguments.
+void carveInstance(const BOwner& w, const mpx_lp& lp, int owner_id, int preset, const double* lo,
+                   const double* hi, bool mixed, Carve& cv, hipStream_t st, BCtrl* ctrl,
+                   BInst& in, BInitArgs& a) {
+    const int m = w.m, n = w.n;
+    std::memset(&in, 0, sizeof(in));
+    std::memset(&a, 0, sizeof(a));
+    in.m = m;
+    in.n = n;
+    in.nnz = w.nnz;
+    in.sense_max = w.sense_max;
+    in.owner = owner_id;
+    in.preset = preset;
+    in.rs = w.rs;
+    in.ci = w.ci;
+    in.cs = w.cs;
+    in.ri = w.ri;
+    in.rl_o = w.rl_o;
+    in.ru_o = w.ru_o;
+    in.obj_o = w.obj_o;
+    in.val = w.val;
+    in.valt = w.valt;
+    in.c = w.c;
+    in.rl = w.rl;
+    in.ru = w.ru;
+    in.row_scale = w.row_scale;
+    in.col_scale = w.col_scale;
+    in.sig = w.sig;
+    in.tau = w.tau;
+    a.n = n;
+    a.m = m;
+    a.col_scale = w.col_scale;
+    a.scal = w.scal;
+    a.mixed = mixed;
+    a.offset = lp.obj_offset;
+    a.sgn = lp.sense ? -1.0 : 1.0;
+    if (lo) {
+        double* lo_o = cv.take<double>(n);
+        double* hi_o = cv.take<double>(n);
+        MPX_HIP(hipMemcpyAsync(lo_o, lo, n * 8ull, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(hi_o, hi, n * 8ull, hipMemcpyHostToDevice, st));
+        in.lo_o = lo_o;
+        in.hi_o = hi_o;
+        double* l = cv.take<double>(n);
+        double* u = cv.take<double>(n);
+        in.l = l;
+        in.u = u;
+        a.own_bounds = 1;
+        a.lo_o = lo_o;
+        a.hi_o = hi_o;
+        a.l = l;
+        a.u = u;
+    } else {
+        in.lo_o = w.lo_o;
+        in.hi_o = w.hi_o;
+        in.l = w.l;
+        in.u = w.u;
+        a.own_bounds = 0;
+        a.l = w.l;
+        a.u = w.u;
+    }
+    float* lf = cv.take<float>(n);
+    float* uf = cv.take<float>(n);
+    a.lf = lf;
+    a.uf = uf;
+    auto fillVecs = [&](auto& v, auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        v.x = cv.take<T>(n);
+        v.x0 = cv.take<T>(n);
+        v.xr = cv.take<T>(n);
+        v.xh = cv.take<T>(n);
+        v.y = cv.take<T>(m);
+        v.y0 = cv.take<T>(m);
+        v.yh = cv.take<T>(m);
+    };
+    fillVecs(in.d, static_cast<double*>(nullptr));
+    fillVecs(in.f, static_cast<float*>(nullptr));
+    in.d.val = w.val;
+    in.d.valt = w.valt;
+    in.d.c = w.c;
+    in.d.tau = w.tau;
+    in.d.l = in.l;
+    in.d.u = in.u;
+    in.d.sig = w.sig;
+    in.d.rl = w.rl;
+    in.d.ru = w.ru;
+    in.f.val = w.valf;
+    in.f.valt = w.valtf;
+    in.f.c = w.cf;
+    in.f.tau = w.tauf;
+    in.f.l = lf;
+    in.f.u = uf;
+    in.f.sig = w.sigf;
+    in.f.rl = w.rlf;
+    in.f.ru = w.ruf;
+    in.ctrl = ctrl;
+}
+
 // Iteration presets the online autotuner tries: (lanes per column, lanes per
 // row) of the packed iteration kernels.
 struct IterPreset {
     int gc, gr;
 };
+
+// Device copy of the options the per-instance control logic reads.
+BOptions makeBOptions(const mpx_pdlp_options& o) {
+    BOptions b{};
+    b.max_iter = o.max_iter;
+    b.N = o.termination_eval_frequency;
+    b.polish_max_iter = o.polish_max_iter;
+    b.norm_inf = o.optimality_norm == 1;
+    b.update_pw = o.update_primal_weight;
+    b.enable_polish = o.enable_polishing;
+    b.precond_refresh = o.preconditioner_refresh;
+    b.do_pc = o.do_pock_chambolle_scaling;
+    b.primal_tol = o.primal_tol;
+    b.dual_tol = o.dual_tol;
+    b.opt_tol = o.optimality_tol;
+    b.polish_p_tol = o.polish_primal_tol;
+    b.polish_d_tol = o.polish_dual_tol;
+    b.pid_kp = o.pid_kp;
+    b.pid_ki = o.pid_ki;
+    b.pid_kd = o.pid_kd;
+    b.pid_i_smooth = o.pid_i_smooth;
+    b.rs_suff = o.restart_sufficient_decay;
+    b.rs_nec = o.restart_necessary_decay;
+    b.rs_art = o.restart_artificial_fraction;
+    b.precond_ratio = o.preconditioner_refresh_ratio;
+    b.mixed = o.precision == MPX_PREC_MIXED;
+    b.handover = o.mixed_handover > 0.0 ? o.mixed_handover : kFp32ExitRatio;
+    // FP64 phase of mixed precision: refine_tol, or the solve's own tolerances.
+    b.refine_p_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.primal_tol) : o.primal_tol;
+    b.refine_d_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.dual_tol) : o.dual_tol;
+    b.refine_g_tol =
+        o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol) : o.optimality_tol;
+    return b;
+}
 
 class BatchEngine {
 public:
@@ -969,35 +1219,8 @@ public:
         MPX_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         MPX_HIP(hipEventCreate(&ev0_));
         MPX_HIP(hipEventCreate(&ev1_));
-        bo_.max_iter = o.max_iter;
-        bo_.N = o.termination_eval_frequency;
-        bo_.polish_max_iter = o.polish_max_iter;
-        bo_.norm_inf = o.optimality_norm == 1;
-        bo_.update_pw = o.update_primal_weight;
-        bo_.enable_polish = o.enable_polishing;
-        bo_.precond_refresh = o.preconditioner_refresh;
-        bo_.do_pc = o.do_pock_chambolle_scaling;
-        bo_.primal_tol = o.primal_tol;
-        bo_.dual_tol = o.dual_tol;
-        bo_.opt_tol = o.optimality_tol;
-        bo_.polish_p_tol = o.polish_primal_tol;
-        bo_.polish_d_tol = o.polish_dual_tol;
-        bo_.pid_kp = o.pid_kp;
-        bo_.pid_ki = o.pid_ki;
-        bo_.pid_kd = o.pid_kd;
-        bo_.pid_i_smooth = o.pid_i_smooth;
-        bo_.rs_suff = o.restart_sufficient_decay;
-        bo_.rs_nec = o.restart_necessary_decay;
-        bo_.rs_art = o.restart_artificial_fraction;
-        bo_.precond_ratio = o.preconditioner_refresh_ratio;
+        bo_ = makeBOptions(o);
         mixed_ = o.precision == MPX_PREC_MIXED;
-        bo_.mixed = mixed_;
-        bo_.handover = o.mixed_handover > 0.0 ? o.mixed_handover : kFp32ExitRatio;
-        // FP64 phase of mixed precision: refine_tol, or the solve's own tolerances.
-        bo_.refine_p_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.primal_tol) : o.primal_tol;
-        bo_.refine_d_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.dual_tol) : o.dual_tol;
-        bo_.refine_g_tol =
-            o.refine_tol > 0.0 ? std::min(o.refine_tol, o.optimality_tol) : o.optimality_tol;
     }
     ~BatchEngine() {
         for (auto& kv : graphs_) kv.second.reset();
@@ -1114,7 +1337,6 @@ void BatchEngine::setup() {
     const auto t0 = std::chrono::steady_clock::now();
     const int no = static_cast<int>(owners_host_.size());
     const int ni = count();
-    // Which owners have per-instance bound overrides.
     size_t bytes = 0;
     for (int w = 0; w < no; ++w) {
         const mpx_lp& lp = *owners_host_[w];
@@ -1128,211 +1350,20 @@ void BatchEngine::setup() {
     Carve cv{static_cast<char*>(arena_->p), 0, bytes};
     owners_.resize(static_cast<size_t>(no));
     owner_preset_.resize(static_cast<size_t>(no));
-    std::vector<std::vector<double>> starts(static_cast<size_t>(no));
-    int max_need = 0;
+    std::vector<OwnerHost> hosts(static_cast<size_t>(no));  // alive until the stream syncs
     for (int w = 0; w < no; ++w) {
         const mpx_lp& lp = *owners_host_[w];
-        const int m = lp.num_rows, n = lp.num_cols, nnz = lp.row_starts[m];
-        BOwner& b = owners_[w];
-        std::memset(&b, 0, sizeof(b));
-        b.m = m;
-        b.n = n;
-        b.nnz = nnz;
-        b.do_ruiz = o_.do_ruiz_scaling;
-        b.ruiz_iters = o_.ruiz_iterations;
-        b.do_pc = o_.do_pock_chambolle_scaling;
-        b.do_rescale = o_.do_bound_obj_rescaling;
-        b.norm_inf = o_.optimality_norm == 1;
-        b.sense_max = lp.sense != 0;
-        b.offset = lp.obj_offset;
-        b.sv_max_iter = o_.sv_max_iter;
-        b.sv_tol = o_.sv_tol;
-        int* rs = cv.take<int>(m + 1);
-        int* ci = cv.take<int>(nnz);
-        int* ri = cv.take<int>(nnz);
-        int* perm = cv.take<int>(nnz);
-        int* cs = cv.take<int>(n + 1);
-        double* val_o = cv.take<double>(nnz);
-        b.val = cv.take<double>(nnz);
-        b.valt = cv.take<double>(nnz);
-        double* obj_o = cv.take<double>(n);
-        double* lo_o = cv.take<double>(n);
-        double* hi_o = cv.take<double>(n);
-        b.c = cv.take<double>(n);
-        b.l = cv.take<double>(n);
-        b.u = cv.take<double>(n);
-        b.cmax = cv.take<double>(n);
-        b.col_scale = cv.take<double>(n);
-        b.tau = cv.take<double>(n);
-        b.px = cv.take<double>(n);
-        b.pxn = cv.take<double>(n);
-        double* rl_o = cv.take<double>(m);
-        double* ru_o = cv.take<double>(m);
-        b.rl = cv.take<double>(m);
-        b.ru = cv.take<double>(m);
-        b.row_scale = cv.take<double>(m);
-        b.sig = cv.take<double>(m);
-        b.py = cv.take<double>(m);
-        b.valf = cv.take<float>(nnz);
-        b.valtf = cv.take<float>(nnz);
-        b.cf = cv.take<float>(n);
-        b.tauf = cv.take<float>(n);
-        b.sigf = cv.take<float>(m);
-        b.rlf = cv.take<float>(m);
-        b.ruf = cv.take<float>(m);
-        b.scal = cv.take<double>(8);
-        std::vector<int> hcs, hri, hperm;
-        hostTranspose(m, n, lp.row_starts, lp.col_indices, hcs, hri, hperm);
-        MPX_HIP(hipMemcpyAsync(rs, lp.row_starts, (m + 1) * 4ull, hipMemcpyHostToDevice, st_));
-        if (nnz) {
-            MPX_HIP(hipMemcpyAsync(ci, lp.col_indices, nnz * 4ull, hipMemcpyHostToDevice, st_));
-            MPX_HIP(hipMemcpyAsync(ri, hri.data(), nnz * 4ull, hipMemcpyHostToDevice, st_));
-            MPX_HIP(hipMemcpyAsync(perm, hperm.data(), nnz * 4ull, hipMemcpyHostToDevice, st_));
-            MPX_HIP(hipMemcpyAsync(val_o, lp.values, nnz * 8ull, hipMemcpyHostToDevice, st_));
-        }
-        MPX_HIP(hipMemcpyAsync(cs, hcs.data(), (n + 1) * 4ull, hipMemcpyHostToDevice, st_));
-        MPX_HIP(hipMemcpyAsync(obj_o, lp.obj, n * 8ull, hipMemcpyHostToDevice, st_));
-        MPX_HIP(hipMemcpyAsync(lo_o, lp.col_lower, n * 8ull, hipMemcpyHostToDevice, st_));
-        MPX_HIP(hipMemcpyAsync(hi_o, lp.col_upper, n * 8ull, hipMemcpyHostToDevice, st_));
-        if (m) {
-            MPX_HIP(hipMemcpyAsync(rl_o, lp.row_lower, m * 8ull, hipMemcpyHostToDevice, st_));
-            MPX_HIP(hipMemcpyAsync(ru_o, lp.row_upper, m * 8ull, hipMemcpyHostToDevice, st_));
-        }
-        b.rs = rs;
-        b.ci = ci;
-        b.cs = cs;
-        b.ri = ri;
-        b.perm = perm;
-        b.val_o = val_o;
-        b.obj_o = obj_o;
-        b.lo_o = lo_o;
-        b.hi_o = hi_o;
-        b.rl_o = rl_o;
-        b.ru_o = ru_o;
-        owner_preset_[w] = o_.spmv_group > 0 ? 0 : heuristicPreset(nnz, m, n);
-        if (o_.spmv_group > 0) {
-            // Forced group: the preset with that many lanes per column.
-            int best = 0;
-            for (int p = 0; p < kNumPresets; ++p)
-                if (presetGC(p) <= o_.spmv_group) best = p;
-            owner_preset_[w] = o_.spmv_group == 1 ? 0 : best;
-        }
-        // Power-iteration start vector, exactly as the reference (pdlp.cpp:756-764).
-        std::mt19937 rng(42);
-        std::normal_distribution<double> dist(0.0, 1.0);
-        std::vector<double>& xs = starts[w];
-        xs.resize(static_cast<size_t>(n));
-        for (double& v : xs) v = dist(rng);
-        double s = 0.0;
-        for (double v : xs) s += v * v;
-        double xn = std::sqrt(s);
-        if (xn < 1e-15) xn = 1.0;
-        for (double& v : xs) v /= xn;
-        MPX_HIP(hipMemcpyAsync(b.px, xs.data(), n * 8ull, hipMemcpyHostToDevice, st_));
-        const int need = (m + n) * (mixed_ ? 4 : 8);
-        max_need = std::max(max_need, need);
+        hosts[w] = ownerHostData(lp);
+        carveOwner(lp, hosts[w], o_, cv, st_, owners_[w]);
+        owner_preset_[w] = ownerPreset(o_, owners_[w].nnz, owners_[w].m, owners_[w].n);
     }
-    // Instances.
     insts_.resize(static_cast<size_t>(ni));
     ctrl_all_ = cv.take<BCtrl>(static_cast<size_t>(std::max(1, ni)));
-    BCtrl* ctrl_all = ctrl_all_;
     std::vector<BInitArgs> args(static_cast<size_t>(ni));
     for (int i = 0; i < ni; ++i) {
-        const BOwner& w = owners_[inst_owner_[i]];
-        const mpx_lp& lp = *owners_host_[inst_owner_[i]];
-        const int m = w.m, n = w.n;
-        BInst& in = insts_[i];
-        std::memset(&in, 0, sizeof(in));
-        in.m = m;
-        in.n = n;
-        in.nnz = w.nnz;
-        in.sense_max = w.sense_max;
-        in.owner = inst_owner_[i];
-        in.preset = owner_preset_[inst_owner_[i]];
-        in.rs = w.rs;
-        in.ci = w.ci;
-        in.cs = w.cs;
-        in.ri = w.ri;
-        in.rl_o = w.rl_o;
-        in.ru_o = w.ru_o;
-        in.obj_o = w.obj_o;
-        in.val = w.val;
-        in.valt = w.valt;
-        in.c = w.c;
-        in.rl = w.rl;
-        in.ru = w.ru;
-        in.row_scale = w.row_scale;
-        in.col_scale = w.col_scale;
-        in.sig = w.sig;
-        in.tau = w.tau;
-        BInitArgs& a = args[i];
-        a.n = n;
-        a.m = m;
-        a.col_scale = w.col_scale;
-        a.scal = w.scal;
-        a.mixed = mixed_;
-        a.offset = lp.obj_offset;
-        a.sgn = lp.sense ? -1.0 : 1.0;
-        if (inst_lo_[i]) {
-            double* lo_o = cv.take<double>(n);
-            double* hi_o = cv.take<double>(n);
-            MPX_HIP(hipMemcpyAsync(lo_o, inst_lo_[i], n * 8ull, hipMemcpyHostToDevice, st_));
-            MPX_HIP(hipMemcpyAsync(hi_o, inst_hi_[i], n * 8ull, hipMemcpyHostToDevice, st_));
-            in.lo_o = lo_o;
-            in.hi_o = hi_o;
-            double* l = cv.take<double>(n);
-            double* u = cv.take<double>(n);
-            in.l = l;
-            in.u = u;
-            a.own_bounds = 1;
-            a.lo_o = lo_o;
-            a.hi_o = hi_o;
-            a.l = l;
-            a.u = u;
-        } else {
-            in.lo_o = w.lo_o;
-            in.hi_o = w.hi_o;
-            in.l = w.l;
-            in.u = w.u;
-            a.own_bounds = 0;
-            a.l = w.l;
-            a.u = w.u;
-        }
-        float* lf = cv.take<float>(n);
-        float* uf = cv.take<float>(n);
-        a.lf = lf;
-        a.uf = uf;
-        auto fillVecs = [&](auto& v, auto* tag) {
-            using T = std::remove_pointer_t<decltype(tag)>;
-            v.x = cv.take<T>(n);
-            v.x0 = cv.take<T>(n);
-            v.xr = cv.take<T>(n);
-            v.xh = cv.take<T>(n);
-            v.y = cv.take<T>(m);
-            v.y0 = cv.take<T>(m);
-            v.yh = cv.take<T>(m);
-        };
-        fillVecs(in.d, static_cast<double*>(nullptr));
-        fillVecs(in.f, static_cast<float*>(nullptr));
-        in.d.val = w.val;
-        in.d.valt = w.valt;
-        in.d.c = w.c;
-        in.d.tau = w.tau;
-        in.d.l = in.l;
-        in.d.u = in.u;
-        in.d.sig = w.sig;
-        in.d.rl = w.rl;
-        in.d.ru = w.ru;
-        in.f.val = w.valf;
-        in.f.valt = w.valtf;
-        in.f.c = w.cf;
-        in.f.tau = w.tauf;
-        in.f.l = lf;
-        in.f.u = uf;
-        in.f.sig = w.sigf;
-        in.f.rl = w.rlf;
-        in.f.ru = w.ruf;
-        in.ctrl = ctrl_all + i;
+        const int ow = inst_owner_[i];
+        carveInstance(owners_[ow], *owners_host_[ow], ow, owner_preset_[ow], inst_lo_[i],
+                      inst_hi_[i], mixed_, cv, st_, ctrl_all_ + i, insts_[i], args[i]);
     }
     // Tile lists for the packed kernels, sized for the smallest tiles (16
     // columns or rows per tile at 16 lanes each).
@@ -1872,6 +1903,632 @@ bool batchOptionsOk(const mpx_pdlp_options* o) {
     return true;
 }
 
+// ---- serving loop (continuous batching) ---------------------------------------
+// One device context with a fixed table of instance slots.  The packed
+// iteration kernels, the tile lists and the per-instance evaluation / control
+// kernel are the batched engine's; only the table rows change between
+// rounds: activation writes a new instance into a free row (and initialises
+// it), retirement recovers the finished ones and leaves their rows with
+// phase == done, which every kernel skips.
+
+// First-fit allocator with coalescing over one device allocation.
+class DevicePool {
+public:
+    explicit DevicePool(size_t bytes) : buf_(bytes), cap_(bytes) {
+        if (bytes) free_[0] = bytes;
+    }
+    // nullptr when no free block is large enough.
+    void* alloc(size_t bytes) {
+        bytes = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, sz = it->second;
+            free_.erase(it);
+            if (sz > bytes) free_[off + bytes] = sz - bytes;
+            live_[off] = bytes;
+            used_ += bytes;
+            peak_ = std::max(peak_, used_);
+            return static_cast<char*>(buf_.p) + off;
+        }
+        return nullptr;
+    }
+    void release(void* p) {
+        if (!p) return;
+        const size_t off =
+            static_cast<size_t>(static_cast<char*>(p) - static_cast<char*>(buf_.p));
+        auto lv = live_.find(off);
+        if (lv == live_.end()) return;
+        size_t start = off, sz = lv->second;
+        used_ -= sz;
+        live_.erase(lv);
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.end() && off + sz == nx->first) {
+            sz += nx->second;
+            nx = free_.erase(nx);
+        }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == start) {
+                start = pv->first;
+                sz += pv->second;
+                free_.erase(pv);
+            }
+        }
+        free_[start] = sz;
+    }
+    size_t capacity() const { return cap_; }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+
+private:
+    DevBuf buf_;
+    size_t cap_, used_ = 0, peak_ = 0;
+    std::map<size_t, size_t> free_, live_;
+};
+
+__global__ void kPlaceInsts(const BInst* __restrict__ src, const int* __restrict__ slots,
+                            int count, BInst* __restrict__ table) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < count) table[slots[t]] = src[t];
+}
+
+__global__ void kAbortSlot(BCtrl* __restrict__ c, int status) {
+    if (c->phase != kPhaseDone) {
+        c->phase = kPhaseDone;
+        c->status = status;
+        c->round_iters = 0;
+        stampDone(*c);
+    }
+}
+
+// An LP the server holds a copy of: a registered base, or a fresh request.
+struct ServeLp {
+    std::vector<int> rs, ci;
+    std::vector<double> val, obj, lo, hi, rl, ru;
+    mpx_lp lp{};
+    BOwner owner{};
+    BOwner* d_owner = nullptr;
+    int* d_preset = nullptr;
+    int preset = 0;
+
+    explicit ServeLp(const mpx_lp& src) {
+        const int m = src.num_rows, n = src.num_cols, nnz = src.row_starts[m];
+        rs.assign(src.row_starts, src.row_starts + m + 1);
+        ci.assign(src.col_indices, src.col_indices + nnz);
+        val.assign(src.values, src.values + nnz);
+        obj.assign(src.obj, src.obj + n);
+        lo.assign(src.col_lower, src.col_lower + n);
+        hi.assign(src.col_upper, src.col_upper + n);
+        rl.assign(src.row_lower, src.row_lower + m);
+        ru.assign(src.row_upper, src.row_upper + m);
+        lp = src;
+        lp.row_starts = rs.data();
+        lp.col_indices = ci.data();
+        lp.values = val.data();
+        lp.obj = obj.data();
+        lp.col_lower = lo.data();
+        lp.col_upper = hi.data();
+        lp.row_lower = rl.data();
+        lp.row_upper = ru.data();
+    }
+    size_t bytes() const {
+        return ownerBytes(lp.num_rows, lp.num_cols, lp.row_starts[lp.num_rows]) +
+               pad(1, sizeof(BOwner)) + pad(1, sizeof(int));
+    }
+    // Carves and uploads the matrix, then queues its scaling and power
+    // iteration on `st` (one workgroup, like the batched engine).
+    void setup(const mpx_pdlp_options& o, const OwnerHost& h, Carve& cv, hipStream_t st) {
+        carveOwner(lp, h, o, cv, st, owner);
+        preset = ownerPreset(o, owner.nnz, owner.m, owner.n);
+        d_owner = cv.take<BOwner>(1);
+        d_preset = cv.take<int>(1);
+        MPX_HIP(hipMemcpyAsync(d_owner, &owner, sizeof(BOwner), hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(d_preset, &preset, sizeof(int), hipMemcpyHostToDevice, st));
+        kOwnerScale<<<1, kBlock, 0, st>>>(d_owner);
+        kOwnerPower<<<1, kBlock, 0, st>>>(d_owner, d_preset);
+        MPX_HIP(hipGetLastError());
+    }
+};
+
+struct ServeReq {
+    int64_t ticket = 0;
+    int base = -1;                  // registered base of a bounds variant
+    std::unique_ptr<ServeLp> own;   // a fresh LP
+    OwnerHost oh;                   // its transpose and power start vector
+    std::vector<double> vlo, vhi;   // a variant's column bounds
+    std::chrono::steady_clock::time_point t_submit, t_active;
+    void* mem = nullptr;
+    BInst inst{};
+    BInitArgs args{};
+    double *xo = nullptr, *yo = nullptr;
+    hipEvent_t ready = nullptr;
+    int slot = -1;
+};
+
+struct ServeDone {
+    mpx_pdlp_result res{};
+    std::vector<double> x, y;
+};
+
+template <typename T>
+struct PinnedArray {
+    T* p = nullptr;
+    explicit PinnedArray(size_t count) {
+        MPX_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * std::max<size_t>(1, count),
+                              hipHostMallocDefault));
+    }
+    PinnedArray(const PinnedArray&) = delete;
+    PinnedArray& operator=(const PinnedArray&) = delete;
+    ~PinnedArray() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+class Server {
+public:
+    static constexpr int kSetupStreams = 4;
+
+    Server(const mpx_pdlp_options& o, double budget, int slots)
+        : o_(o), S_(slots > 0 ? slots : 1024), h_ctrl_(S_), h_new_(S_), h_args_(S_),
+          h_slots_(S_), h_ptrs_(2 * static_cast<size_t>(S_)), h_totals_(8) {
+        MPX_HIP(hipSetDevice(o.device));
+        MPX_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        for (auto& s : setup_st_) MPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        bo_ = makeBOptions(o);
+        mixed_ = o.precision == MPX_PREC_MIXED;
+        size_t freeb = 0, total = 0;
+        MPX_HIP(hipMemGetInfo(&freeb, &total));
+        const size_t want = budget > 0.0 ? static_cast<size_t>(budget) : freeb / 2;
+        pool_ = std::make_unique<DevicePool>(std::max<size_t>(want, 1 << 20));
+        const size_t S = static_cast<size_t>(S_);
+        tab_ = std::make_unique<DevBuf>(sizeof(BInst) * S);
+        ctrl_ = std::make_unique<DevBuf>(sizeof(BCtrl) * S);
+        cnt_ = std::make_unique<DevBuf>(sizeof(int) * 4 * S);
+        newi_ = std::make_unique<DevBuf>(sizeof(BInst) * S);
+        newa_ = std::make_unique<DevBuf>(sizeof(BInitArgs) * S);
+        news_ = std::make_unique<DevBuf>(sizeof(int) * S);
+        ptrs_ = std::make_unique<DevBuf>(sizeof(double*) * 2 * S);
+        // Empty rows: no data, control state done.
+        BCtrl* dc = static_cast<BCtrl*>(ctrl_->p);
+        for (int s = 0; s < S_; ++s) {
+            std::memset(&h_new_.p[s], 0, sizeof(BInst));
+            h_new_.p[s].ctrl = dc + s;
+            std::memset(&h_ctrl_.p[s], 0, sizeof(BCtrl));
+            h_ctrl_.p[s].phase = kPhaseDone;
+        }
+        MPX_HIP(hipMemcpyAsync(tab_->p, h_new_.p, sizeof(BInst) * S, hipMemcpyHostToDevice, st_));
+        MPX_HIP(hipMemcpyAsync(ctrl_->p, h_ctrl_.p, sizeof(BCtrl) * S, hipMemcpyHostToDevice, st_));
+        MPX_HIP(hipStreamSynchronize(st_));
+        for (int s = S_ - 1; s >= 0; --s) free_slots_.push_back(s);
+        slot_req_.resize(S);
+        int dev = 0, cus = 256;
+        MPX_HIP(hipGetDevice(&dev));
+        MPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        grid_ = std::max(kNumXcd, (cus * 8 / kNumXcd) * kNumXcd);
+        growTiles(1 << 14, 1 << 14);
+    }
+
+    ~Server() {
+        (void)hipStreamSynchronize(st_);
+        for (auto& s : setup_st_) (void)hipStreamSynchronize(s);
+        for (auto& r : preparing_)
+            if (r->ready) (void)hipEventDestroy(r->ready);
+        for (auto& kv : graphs_) kv.second.reset();
+        for (auto& s : setup_st_) (void)hipStreamDestroy(s);
+        (void)hipStreamDestroy(st_);
+    }
+
+    int addBase(const mpx_lp& lp) {
+        auto b = std::make_unique<ServeLp>(lp);
+        const OwnerHost h = ownerHostData(b->lp);
+        const size_t need = b->bytes();
+        void* mem = pool_->alloc(need);
+        if (!mem) {
+            setError("mpx_server_add_base: the base LP does not fit the server's memory pool");
+            throw OutOfMemory();
+        }
+        Carve cv{static_cast<char*>(mem), 0, need};
+        b->setup(o_, h, cv, st_);
+        MPX_HIP(hipStreamSynchronize(st_));
+        bases_.push_back(std::move(b));
+        return static_cast<int>(bases_.size()) - 1;
+    }
+    int numBases() const { return static_cast<int>(bases_.size()); }
+    int numCols(int base) const { return bases_[base]->lp.num_cols; }
+
+    int64_t submit(const mpx_lp& lp) {
+        auto r = std::make_unique<ServeReq>();
+        r->own = std::make_unique<ServeLp>(lp);
+        r->oh = ownerHostData(r->own->lp);
+        return enqueue(std::move(r));
+    }
+    int64_t submitBounds(int base, const double* lo, const double* hi) {
+        auto r = std::make_unique<ServeReq>();
+        const int n = bases_[base]->lp.num_cols;
+        r->base = base;
+        r->vlo.assign(lo, lo + n);
+        r->vhi.assign(hi, hi + n);
+        return enqueue(std::move(r));
+    }
+
+    int step() {
+        activate();
+        admit();
+        iterate();
+        retire();
+        enforceTimeLimit();
+        return static_cast<int>(queued_.size() + preparing_.size()) + running_;
+    }
+
+    // 1: done (and forgotten), 0: not finished, -1: unknown ticket.
+    int result(int64_t ticket, double* x, double* y, mpx_pdlp_result* res) {
+        auto it = done_.find(ticket);
+        if (it == done_.end()) return ticket >= 0 && ticket < next_ticket_ ? 0 : -1;
+        if (res) *res = it->second.res;
+        if (x && !it->second.x.empty())
+            std::memcpy(x, it->second.x.data(), it->second.x.size() * sizeof(double));
+        if (y && !it->second.y.empty())
+            std::memcpy(y, it->second.y.data(), it->second.y.size() * sizeof(double));
+        done_.erase(it);
+        return 1;
+    }
+
+    void stats(mpx_server_stats* s) const {
+        std::memset(s, 0, sizeof(*s));
+        s->submitted = next_ticket_;
+        s->finished = finished_;
+        s->queued = static_cast<int32_t>(queued_.size());
+        s->preparing = static_cast<int32_t>(preparing_.size());
+        s->running = running_;
+        s->slots = S_;
+        s->rounds = rounds_;
+        s->pool_bytes = static_cast<double>(pool_->capacity());
+        s->pool_used_bytes = static_cast<double>(pool_->used());
+        s->pool_peak_bytes = static_cast<double>(pool_->peak());
+        s->iterate_ms = iterate_ms_;
+        s->instance_iterations = inst_iters_;
+    }
+
+private:
+    struct OutOfMemory {};
+    struct G {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        void reset() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+            e = nullptr;
+            g = nullptr;
+        }
+    };
+
+    const mpx_lp& lpOf(const ServeReq& r) const {
+        return r.own ? r.own->lp : bases_[r.base]->lp;
+    }
+
+    int64_t enqueue(std::unique_ptr<ServeReq> r) {
+        r->ticket = next_ticket_++;
+        r->t_submit = std::chrono::steady_clock::now();
+        const int64_t t = r->ticket;
+        queued_.push_back(std::move(r));
+        return t;
+    }
+
+    size_t bytesFor(const ServeReq& r) const {
+        const mpx_lp& lp = lpOf(r);
+        const size_t m = lp.num_rows, n = lp.num_cols;
+        size_t b = instBytes(lp.num_rows, lp.num_cols, !r.own) + pad(n, 8) + pad(m, 8);
+        if (r.own) b += r.own->bytes();
+        return b;
+    }
+
+    void finishNow(std::unique_ptr<ServeReq> r, int status) {
+        ServeDone d;
+        d.res.status = status;
+        d.res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                      r->t_submit).count();
+        d.res.setup_seconds = d.res.seconds;
+        done_[r->ticket] = std::move(d);
+        ++finished_;
+    }
+
+    // Queued -> preparing: memory, a slot, uploads and (fresh LPs) the
+    // scaling and power iteration on a side stream.
+    void admit() {
+        while (!queued_.empty() && !free_slots_.empty()) {
+            ServeReq& r = *queued_.front();
+            const size_t need = bytesFor(r);
+            if (need > pool_->capacity()) {
+                setError("serving loop: request larger than the memory pool");
+                std::unique_ptr<ServeReq> q = std::move(queued_.front());
+                queued_.pop_front();
+                finishNow(std::move(q), MPX_STATUS_ERROR);
+                continue;
+            }
+            void* mem = pool_->alloc(need);
+            if (!mem) break;  // first in, first out: wait for memory
+            std::unique_ptr<ServeReq> q = std::move(queued_.front());
+            queued_.pop_front();
+            q->mem = mem;
+            q->slot = free_slots_.back();
+            free_slots_.pop_back();
+            hipStream_t st = setup_st_[next_setup_++ % kSetupStreams];
+            Carve cv{static_cast<char*>(mem), 0, need};
+            BCtrl* ctrl = static_cast<BCtrl*>(ctrl_->p) + q->slot;
+            if (q->own) {
+                q->own->setup(o_, q->oh, cv, st);
+                carveInstance(q->own->owner, q->own->lp, 0, q->own->preset, nullptr, nullptr,
+                              mixed_, cv, st, ctrl, q->inst, q->args);
+            } else {
+                const ServeLp& b = *bases_[q->base];
+                carveInstance(b.owner, b.lp, q->base, b.preset, q->vlo.data(), q->vhi.data(),
+                              mixed_, cv, st, ctrl, q->inst, q->args);
+            }
+            q->xo = cv.take<double>(static_cast<size_t>(q->inst.n));
+            q->yo = cv.take<double>(static_cast<size_t>(q->inst.m));
+            MPX_HIP(hipEventCreateWithFlags(&q->ready, hipEventDisableTiming));
+            MPX_HIP(hipEventRecord(q->ready, st));
+            preparing_.push_back(std::move(q));
+        }
+    }
+
+    // Preparing -> running for every request whose setup has completed:
+    // table rows written and instances initialised on the main stream.
+    void activate() {
+        if (preparing_.empty()) return;
+        if (running_ == 0) MPX_HIP(hipEventSynchronize(preparing_.front()->ready));
+        int k = 0;
+        size_t need_c = 0, need_r = 0;
+        std::vector<std::unique_ptr<ServeReq>> still;
+        const auto now = std::chrono::steady_clock::now();
+        for (auto& q : preparing_) {
+            const hipError_t e = hipEventQuery(q->ready);
+            if (e == hipErrorNotReady) {
+                still.push_back(std::move(q));
+                continue;
+            }
+            MPX_HIP(e);
+            (void)hipEventDestroy(q->ready);
+            q->ready = nullptr;
+            // The pageable uploads have been consumed: drop the host copies.
+            q->oh = OwnerHost();
+            h_new_.p[k] = q->inst;
+            h_args_.p[k] = q->args;
+            h_slots_.p[k] = q->slot;
+            ++k;
+            q->t_active = now;
+            need_c += static_cast<size_t>((q->inst.n + 15) / 16);
+            need_r += static_cast<size_t>((q->inst.m + 15) / 16);
+            const int s = q->slot;
+            slot_req_[s] = std::move(q);
+            ++running_;
+        }
+        preparing_ = std::move(still);
+        if (k == 0) return;
+        tiles_c_ += need_c;
+        tiles_r_ += need_r;
+        if (tiles_c_ > cap_c_ || tiles_r_ > cap_r_) growTiles(tiles_c_ * 3 / 2, tiles_r_ * 3 / 2);
+        MPX_HIP(hipMemcpyAsync(newi_->p, h_new_.p, sizeof(BInst) * k, hipMemcpyHostToDevice, st_));
+        MPX_HIP(hipMemcpyAsync(newa_->p, h_args_.p, sizeof(BInitArgs) * k, hipMemcpyHostToDevice,
+                               st_));
+        MPX_HIP(hipMemcpyAsync(news_->p, h_slots_.p, sizeof(int) * k, hipMemcpyHostToDevice, st_));
+        kPlaceInsts<<<ceilDiv(k, kBlock), kBlock, 0, st_>>>(static_cast<const BInst*>(newi_->p),
+                                                           static_cast<const int*>(news_->p), k,
+                                                           static_cast<BInst*>(tab_->p));
+        kInstInit<<<k, kBlock, 0, st_>>>(static_cast<const BInst*>(newi_->p),
+                                         static_cast<const BInitArgs*>(newa_->p), bo_,
+                                         o_.primal_weight);
+        MPX_HIP(hipGetLastError());
+        // The staging arrays are reused next step, after this stream's sync.
+        fp32_gone_ = false;
+    }
+
+    void growTiles(size_t c, size_t r) {
+        MPX_HIP(hipStreamSynchronize(st_));
+        for (auto& kv : graphs_) kv.second.reset();
+        graphs_.clear();
+        cap_c_ = std::max(c, cap_c_);
+        cap_r_ = std::max(r, cap_r_);
+        tiles_ = std::make_unique<DevBuf>(2 * (cap_c_ + cap_r_) * sizeof(Tile) + 16 * sizeof(int));
+        Tile* t = static_cast<Tile*>(tiles_->p);
+        tc32_ = t;
+        tr32_ = tc32_ + cap_c_;
+        tc64_ = tr32_ + cap_r_;
+        tr64_ = tc64_ + cap_c_;
+        totals_ = reinterpret_cast<int*>(tr64_ + cap_r_);
+    }
+
+    void enqueueRound() {
+        const int K = S_;
+        const BInst* di = static_cast<const BInst*>(tab_->p);
+        int* cnt = static_cast<int*>(cnt_->p);
+        const int gk = ceilDiv(K, kBlock);
+        kTileCounts<<<gk, kBlock, 0, st_>>>(di, K, cnt);
+        kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
+        kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
+        const bool f32 = mixed_ && !fp32_gone_;
+        for (int k = 1; k <= bo_.N; ++k) {
+            if (f32) {
+                kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
+                kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, tr32_, totals_ + 1, k);
+            }
+            kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, tc64_, totals_ + 2, k);
+            kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, tr64_, totals_ + 3, k);
+        }
+        if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+        kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    }
+
+    void launchRound() {
+        if (graph_failed_) {
+            enqueueRound();
+            MPX_HIP(hipGetLastError());
+            return;
+        }
+        G& g = graphs_[fp32_gone_ ? 1 : 0];
+        if (g.e == nullptr) {
+            bool ok = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                enqueueRound();
+                ok = hipStreamEndCapture(st_, &g.g) == hipSuccess && g.g != nullptr &&
+                     hipGraphInstantiate(&g.e, g.g, nullptr, nullptr, 0) == hipSuccess;
+            }
+            if (!ok) {
+                (void)hipGetLastError();
+                g.reset();
+                graph_failed_ = true;
+                enqueueRound();
+                MPX_HIP(hipGetLastError());
+                return;
+            }
+        }
+        MPX_HIP(hipGraphLaunch(g.e, st_));
+    }
+
+    // One round of every running instance, then the control rows back.
+    void iterate() {
+        if (running_ == 0) return;
+        const auto t0 = std::chrono::steady_clock::now();
+        launchRound();
+        ++rounds_;
+        MPX_HIP(hipMemcpyAsync(h_ctrl_.p, ctrl_->p, sizeof(BCtrl) * S_, hipMemcpyDeviceToHost, st_));
+        MPX_HIP(hipMemcpyAsync(h_totals_.p, totals_, sizeof(int) * 2, hipMemcpyDeviceToHost, st_));
+        MPX_HIP(hipStreamSynchronize(st_));
+        if (mixed_ && !fp32_gone_ && h_totals_.p[0] + h_totals_.p[1] == 0) {
+            // No FP32 instance was active in this round (new ones reset it).
+            fp32_gone_ = true;
+        }
+        iterate_ms_ += 1000.0 * std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                              t0).count();
+    }
+
+    // Running instances that finished: recovery into their own output
+    // arrays, copies out, memory and slot recycled.
+    void retire() {
+        if (running_ == 0) return;
+        std::vector<int> fin;
+        for (int s = 0; s < S_; ++s)
+            if (slot_req_[s] && h_ctrl_.p[s].phase == kPhaseDone) fin.push_back(s);
+        if (fin.empty()) return;
+        const int k = static_cast<int>(fin.size());
+        for (int i = 0; i < k; ++i) {
+            const ServeReq& q = *slot_req_[fin[i]];
+            h_new_.p[i] = q.inst;
+            h_ptrs_.p[i] = q.xo;
+            h_ptrs_.p[k + i] = q.yo;
+        }
+        MPX_HIP(hipMemcpyAsync(newi_->p, h_new_.p, sizeof(BInst) * k, hipMemcpyHostToDevice, st_));
+        MPX_HIP(hipMemcpyAsync(ptrs_->p, h_ptrs_.p, sizeof(double*) * 2 * k, hipMemcpyHostToDevice,
+                               st_));
+        double* const* xp = static_cast<double* const*>(ptrs_->p);
+        kInstRecover<<<k, kBlock, 0, st_>>>(static_cast<const BInst*>(newi_->p), xp, xp + k);
+        MPX_HIP(hipGetLastError());
+        std::vector<ServeDone> out(static_cast<size_t>(k));
+        for (int i = 0; i < k; ++i) {
+            const ServeReq& q = *slot_req_[fin[i]];
+            out[i].x.resize(static_cast<size_t>(q.inst.n));
+            out[i].y.resize(static_cast<size_t>(q.inst.m));
+            MPX_HIP(hipMemcpyAsync(out[i].x.data(), q.xo, q.inst.n * 8ull, hipMemcpyDeviceToHost,
+                                   st_));
+            MPX_HIP(hipMemcpyAsync(out[i].y.data(), q.yo, q.inst.m * 8ull, hipMemcpyDeviceToHost,
+                                   st_));
+        }
+        MPX_HIP(hipStreamSynchronize(st_));
+        const auto now = std::chrono::steady_clock::now();
+        for (int i = 0; i < k; ++i) {
+            const int s = fin[i];
+            std::unique_ptr<ServeReq> q = std::move(slot_req_[s]);
+            const BCtrl& c = h_ctrl_.p[s];
+            const mpx_lp& lp = lpOf(*q);
+            const int n = q->inst.n, m = q->inst.m;
+            ServeDone& d = out[i];
+            mpx_pdlp_result& r = d.res;
+            r.status = c.status;
+            r.iterations = c.total;
+            r.work_units = static_cast<double>(c.total) *
+                           (4.0 * static_cast<double>(q->inst.nnz) + 2.0 * static_cast<double>(m + n));
+            r.rel_primal = o_.optimality_norm == 1 ? c.rel_p_inf : c.rel_p;
+            r.rel_dual = o_.optimality_norm == 1 ? c.rel_d_inf : c.rel_d;
+            r.rel_gap = c.gap;
+            r.primal_obj = c.pobj;
+            r.dual_obj = c.dobj;
+            r.step = c.step;
+            r.primal_weight = c.pw;
+            r.restarts = c.restarts;
+            r.iterations_fp32 = c.fp32_iters;
+            r.seconds = std::chrono::duration<double>(now - q->t_submit).count();
+            r.setup_seconds = std::chrono::duration<double>(q->t_active - q->t_submit).count();
+            // Same conventions as the batched results: Optimal keeps both,
+            // Infeasible the Farkas ray in y, Unbounded the ray in x.
+            const bool keepX = c.status == MPX_STATUS_OPTIMAL || c.status == MPX_STATUS_UNBOUNDED;
+            const bool keepY = c.status == MPX_STATUS_OPTIMAL || c.status == MPX_STATUS_INFEASIBLE;
+            if (c.status == MPX_STATUS_OPTIMAL) {
+                double objv = lp.obj_offset;
+                for (int j = 0; j < n; ++j) objv += lp.obj[j] * d.x[j];
+                r.objective = objv;
+            }
+            if (!keepX) std::fill(d.x.begin(), d.x.end(), 0.0);
+            if (!keepY) std::fill(d.y.begin(), d.y.end(), 0.0);
+            inst_iters_ += c.total;
+            tiles_c_ -= static_cast<size_t>((n + 15) / 16);
+            tiles_r_ -= static_cast<size_t>((m + 15) / 16);
+            pool_->release(q->mem);
+            free_slots_.push_back(s);
+            --running_;
+            done_[q->ticket] = std::move(d);
+            ++finished_;
+        }
+    }
+
+    void enforceTimeLimit() {
+        if (o_.max_solve_seconds < 0.0 || running_ == 0) return;
+        const auto now = std::chrono::steady_clock::now();
+        BCtrl* dc = static_cast<BCtrl*>(ctrl_->p);
+        for (int s = 0; s < S_; ++s) {
+            const ServeReq* q = slot_req_[s].get();
+            if (!q || h_ctrl_.p[s].phase == kPhaseDone) continue;
+            if (std::chrono::duration<double>(now - q->t_active).count() >= o_.max_solve_seconds)
+                kAbortSlot<<<1, 1, 0, st_>>>(dc + s, MPX_STATUS_TIME_LIMIT);
+        }
+        MPX_HIP(hipGetLastError());
+    }
+
+    mpx_pdlp_options o_;
+    BOptions bo_{};
+    bool mixed_ = false;
+    int S_;
+    hipStream_t st_ = nullptr;
+    hipStream_t setup_st_[kSetupStreams] = {};
+    int next_setup_ = 0;
+    std::unique_ptr<DevicePool> pool_;
+    std::unique_ptr<DevBuf> tab_, ctrl_, cnt_, newi_, newa_, news_, ptrs_, tiles_;
+    PinnedArray<BCtrl> h_ctrl_;
+    PinnedArray<BInst> h_new_;
+    PinnedArray<BInitArgs> h_args_;
+    PinnedArray<int> h_slots_;
+    PinnedArray<double*> h_ptrs_;
+    PinnedArray<int> h_totals_;
+    Tile *tc32_ = nullptr, *tr32_ = nullptr, *tc64_ = nullptr, *tr64_ = nullptr;
+    int* totals_ = nullptr;
+    size_t cap_c_ = 0, cap_r_ = 0, tiles_c_ = 0, tiles_r_ = 0;
+    int grid_ = 2048;
+    std::map<int, G> graphs_;
+    bool graph_failed_ = false;
+    bool fp32_gone_ = false;
+    std::vector<std::unique_ptr<ServeLp>> bases_;
+    std::deque<std::unique_ptr<ServeReq>> queued_;
+    std::vector<std::unique_ptr<ServeReq>> preparing_;
+    std::vector<std::unique_ptr<ServeReq>> slot_req_;
+    std::vector<int> free_slots_;
+    int running_ = 0;
+    std::map<int64_t, ServeDone> done_;
+    int64_t next_ticket_ = 0, finished_ = 0, rounds_ = 0, inst_iters_ = 0;
+    double iterate_ms_ = 0.0;
+
+public:
+    using OOM = OutOfMemory;
+};
+
 }  // namespace mpx
 
 using namespace mpx;
@@ -1946,6 +2603,11 @@ int mpx_pdlp_solve_batch(const mpx_lp* lps, int32_t count, const mpx_pdlp_option
 }
 
 }  // extern "C"
+
+struct mpx_server {
+    std::unique_ptr<mpx::Server> s;
+    bool broken = false;
+};
 
 struct mpx_batch {
     mpx_pdlp_options opts;
@@ -2113,16 +2775,97 @@ int mpx_batch_destroy(mpx_batch* b) {
     return MPX_OK;
 }
 
-int mpx_server_create(const mpx_pdlp_options*, double, int32_t, mpx_server** out) {
-    if (out) *out = nullptr;
-    mpx::setError("mpx_server_create: not implemented yet");
-    return MPX_ERR_UNSUPPORTED;
+int mpx_server_create(const mpx_pdlp_options* opts, double hbm_budget_bytes, int32_t max_slots,
+                      mpx_server** out) {
+    if (!out) {
+        setError("mpx_server_create: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (!batchOptionsOk(opts)) return MPX_ERR_UNSUPPORTED;
+    try {
+        auto s = std::make_unique<mpx_server>();
+        s->s = std::make_unique<Server>(*opts, hbm_budget_bytes, max_slots);
+        *out = s.release();
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
 }
-int mpx_server_submit(mpx_server*, const mpx_lp*, int64_t*) { return MPX_ERR_UNSUPPORTED; }
-int mpx_server_step(mpx_server*, int32_t*) { return MPX_ERR_UNSUPPORTED; }
-int mpx_server_result(mpx_server*, int64_t, int32_t*, double*, double*, mpx_pdlp_result*) {
-    return MPX_ERR_UNSUPPORTED;
+
+int mpx_server_add_base(mpx_server* srv, const mpx_lp* base, int32_t* base_id) {
+    if (!srv || !base_id || !validLpB(base)) {
+        setError("mpx_server_add_base: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    if (srv->broken) return MPX_ERR_DEVICE;
+    try {
+        *base_id = srv->s->addBase(*base);
+    } catch (const Server::OOM&) {
+        return MPX_ERR_OUT_OF_MEMORY;
+    } catch (const DeviceError&) {
+        srv->broken = true;
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
 }
-int mpx_server_destroy(mpx_server*) { return MPX_OK; }
+
+int mpx_server_submit(mpx_server* srv, const mpx_lp* lp, int64_t* ticket) {
+    if (!srv || !ticket || !validLpB(lp)) {
+        setError("mpx_server_submit: invalid arguments (served LPs need m > 0 and n > 0)");
+        return MPX_ERR_INVALID;
+    }
+    if (srv->broken) return MPX_ERR_DEVICE;
+    *ticket = srv->s->submit(*lp);
+    return MPX_OK;
+}
+
+int mpx_server_submit_bounds(mpx_server* srv, int32_t base_id, const double* col_lower,
+                             const double* col_upper, int64_t* ticket) {
+    if (!srv || !ticket || !col_lower || !col_upper || base_id < 0 ||
+        base_id >= srv->s->numBases()) {
+        setError("mpx_server_submit_bounds: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    if (srv->broken) return MPX_ERR_DEVICE;
+    *ticket = srv->s->submitBounds(base_id, col_lower, col_upper);
+    return MPX_OK;
+}
+
+int mpx_server_step(mpx_server* srv, int32_t* active) {
+    if (!srv) return MPX_ERR_INVALID;
+    if (srv->broken) return MPX_ERR_DEVICE;
+    try {
+        const int a = srv->s->step();
+        if (active) *active = a;
+    } catch (const DeviceError&) {
+        srv->broken = true;
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
+
+int mpx_server_result(mpx_server* srv, int64_t ticket, int32_t* done, double* x_out,
+                      double* y_out, mpx_pdlp_result* res) {
+    if (!srv || !done) return MPX_ERR_INVALID;
+    const int r = srv->s->result(ticket, x_out, y_out, res);
+    if (r < 0) {
+        setError("mpx_server_result: unknown or already fetched ticket");
+        return MPX_ERR_INVALID;
+    }
+    *done = r;
+    return MPX_OK;
+}
+
+int mpx_server_get_stats(mpx_server* srv, mpx_server_stats* stats) {
+    if (!srv || !stats) return MPX_ERR_INVALID;
+    srv->s->stats(stats);
+    return MPX_OK;
+}
+
+int mpx_server_destroy(mpx_server* srv) {
+    delete srv;
+    return MPX_OK;
+}
 
 }  // extern "C"

@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 
 from ._abi import (MpxBatchStats, MpxLp, MpxPdlpOptions, MpxPdlpResult, MpxScaledProblem,
-                   alloc_scaled, make_options, scaled_from_arrays, c_double_p)
+                   MpxServerStats, alloc_scaled, make_options, scaled_from_arrays, c_double_p)
 from ._loader import load_library
 
 _lib = None
@@ -17,8 +17,8 @@ EXPORTED = [
     "mpx_pdlp_default_options", "mpx_last_error", "mpx_version", "mpx_device_count",
     "mpx_device_synchronize", "mpx_copy_bench",
     "mpx_pdlp_solve", "mpx_pdlp_solve_batch_shared", "mpx_pdlp_solve_batch",
-    "mpx_server_create", "mpx_server_submit", "mpx_server_step", "mpx_server_result",
-    "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
+    "mpx_server_create", "mpx_server_add_base", "mpx_server_submit", "mpx_server_submit_bounds",
+    "mpx_server_step", "mpx_server_result", "mpx_server_get_stats", "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
     "mpx_pdhg_iterate", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
     "mpx_batch_results", "mpx_batch_get_stats", "mpx_batch_profile", "mpx_batch_destroy",
 ]
@@ -44,7 +44,11 @@ def lib():
                                        C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
     L.mpx_server_create.argtypes = [C.POINTER(MpxPdlpOptions), C.c_double, C.c_int32,
                                     C.POINTER(C.c_void_p)]
+    L.mpx_server_add_base.argtypes = [C.c_void_p, C.POINTER(MpxLp), C.POINTER(C.c_int32)]
     L.mpx_server_submit.argtypes = [C.c_void_p, C.POINTER(MpxLp), C.POINTER(C.c_int64)]
+    L.mpx_server_submit_bounds.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                           C.POINTER(C.c_int64)]
+    L.mpx_server_get_stats.argtypes = [C.c_void_p, C.POINTER(MpxServerStats)]
     L.mpx_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.mpx_server_result.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.c_void_p,
                                     C.c_void_p, C.POINTER(MpxPdlpResult)]
@@ -232,6 +236,81 @@ class ResidentBatch:
     def close(self):
         if self._h:
             lib().mpx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching serving loop (mpx_server_*): submit LPs or bounds
+    variants of a registered base at any time, call step() to advance every
+    admitted request by one round, collect finished results by ticket."""
+
+    def __init__(self, hbm_budget_bytes=0.0, max_slots=0, **opts):
+        self._o = opts.pop("_options", None) or make_options(**opts)
+        self._h = C.c_void_p()
+        _check(lib().mpx_server_create(C.byref(self._o), float(hbm_budget_bytes), int(max_slots),
+                                       C.byref(self._h)), "mpx_server_create")
+        self._dims = {}   # ticket -> (n, m)
+        self._base_dims = []
+
+    def add_base(self, lp):
+        bid = C.c_int32(-1)
+        clp = lp.as_c()
+        _check(lib().mpx_server_add_base(self._h, C.byref(clp), C.byref(bid)),
+               "mpx_server_add_base")
+        self._base_dims.append((lp.num_cols, lp.num_rows))
+        return bid.value
+
+    def submit(self, lp):
+        t = C.c_int64(-1)
+        clp = lp.as_c()
+        _check(lib().mpx_server_submit(self._h, C.byref(clp), C.byref(t)), "mpx_server_submit")
+        self._dims[t.value] = (lp.num_cols, lp.num_rows)
+        return t.value
+
+    def submit_bounds(self, base_id, col_lower, col_upper):
+        lo = np.ascontiguousarray(col_lower, dtype=np.float64)
+        hi = np.ascontiguousarray(col_upper, dtype=np.float64)
+        t = C.c_int64(-1)
+        _check(lib().mpx_server_submit_bounds(self._h, base_id, lo.ctypes.data, hi.ctypes.data,
+                                              C.byref(t)), "mpx_server_submit_bounds")
+        self._dims[t.value] = self._base_dims[base_id]
+        return t.value
+
+    def step(self):
+        a = C.c_int32(0)
+        _check(lib().mpx_server_step(self._h, C.byref(a)), "mpx_server_step")
+        return a.value
+
+    def result(self, ticket, want_vectors=True):
+        """(result dict, x, y) once finished (then forgotten), else None."""
+        n, m = self._dims[ticket]
+        x = np.zeros(n) if want_vectors else None
+        y = np.zeros(m) if want_vectors else None
+        done = C.c_int32(0)
+        res = MpxPdlpResult()
+        _check(lib().mpx_server_result(self._h, ticket, C.byref(done),
+                                       None if x is None else x.ctypes.data,
+                                       None if y is None else y.ctypes.data, C.byref(res)),
+               "mpx_server_result")
+        if not done.value:
+            return None
+        del self._dims[ticket]
+        return res.as_dict(), x, y
+
+    def stats(self):
+        st = MpxServerStats()
+        _check(lib().mpx_server_get_stats(self._h, C.byref(st)), "mpx_server_get_stats")
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().mpx_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -43,3 +43,15 @@ def test_invalid_arguments_rejected_without_gpu():
     o = _abi.make_options()
     assert lib.mpx_pdlp_solve(None, C.byref(o), None, None, None, None, C.byref(res)) == 1
     assert "invalid" in native.last_error()
+
+
+def test_server_rejects_invalid_arguments_without_gpu():
+    lib = native.lib()
+    o = _abi.make_options()
+    assert lib.mpx_server_create(C.byref(o), 0.0, 0, None) == 1
+    assert lib.mpx_server_step(None, None) == 1
+    assert lib.mpx_server_submit(None, None, None) == 1
+    o.adaptive_step = 1
+    h = C.c_void_p()
+    assert lib.mpx_server_create(C.byref(o), 0.0, 0, C.byref(h)) == 4
+    assert not h.value

@@ -1,0 +1,132 @@
+"""GPU tests of the continuous-batching serving loop (mpx_server_*), through
+the C-ABI: requests admitted at different times, sharing the device with
+whatever else is in flight, give bit-identical results to the same LP solved
+alone by the batched entry point (itself checked against the oracle in
+test_gpu_batch.py); admission respects the memory budget and the slot count."""
+import numpy as np
+import pytest
+
+from conftest import TIGHT
+from lp_cases import closed_form_cases
+from mipx_amd import lpgen, native
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+MAX_STEPS = 20000
+
+
+def _drain(srv, tickets, submit_at=None):
+    """Steps the server until every ticket is finished; `submit_at` maps a
+    step number to a callable that submits more requests (returns tickets)."""
+    out = {}
+    submit_at = dict(submit_at or {})
+    for k in range(MAX_STEPS):
+        if k in submit_at:
+            tickets += submit_at.pop(k)()
+        active = srv.step()
+        for t in tickets:
+            if t not in out:
+                r = srv.result(t)
+                if r is not None:
+                    out[t] = r
+        if active == 0 and not submit_at and len(out) == len(tickets):
+            return out
+    raise AssertionError(f"server did not drain in {MAX_STEPS} steps: {srv.stats()}")
+
+
+def _same(a, b, tag):
+    ra, xa, ya = a
+    rb, xb, yb = b
+    assert ra["status"] == rb["status"], (tag, ra, rb)
+    assert ra["iterations"] == rb["iterations"], (tag, ra["iterations"], rb["iterations"])
+    assert ra["objective"] == rb["objective"], (tag, ra["objective"], rb["objective"])
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb), tag
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_server_results_identical_to_solo_batch(precision):
+    opts = dict(precision=precision, max_iter=200000, **TIGHT)
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    fresh = [lpgen.c3_instance(100 + i, m).lp for i, m in enumerate([60, 140, 90, 200, 75])]
+    bounds = [g.branch_bounds(lpgen.c2_instance_seed(i)) for i in range(8)]
+    srv = native.Server(max_slots=16, **opts)
+    base = srv.add_base(g.lp)
+    kinds = {}
+
+    def sub_variants(idx):
+        ts = []
+        for i in idx:
+            t = srv.submit_bounds(base, *bounds[i])
+            kinds[t] = ("variant", i)
+            ts.append(t)
+        return ts
+
+    def sub_fresh(idx):
+        ts = []
+        for i in idx:
+            t = srv.submit(fresh[i])
+            kinds[t] = ("fresh", i)
+            ts.append(t)
+        return ts
+
+    tickets = sub_variants([0, 1, 2]) + sub_fresh([0])
+    got = _drain(srv, tickets, submit_at={
+        3: lambda: sub_fresh([1, 2]) + sub_variants([3, 4, 5]),
+        20: lambda: sub_variants([6, 7]) + sub_fresh([3, 4]),
+    })
+    st = srv.stats()
+    print("server stats", st)
+    assert st["finished"] == len(tickets) and st["running"] == 0 and st["pool_used_bytes"] > 0
+    srv.close()
+    for t, (kind, i) in kinds.items():
+        lp = g.lp.with_col_bounds(*bounds[i]) if kind == "variant" else fresh[i]
+        res, xs, ys = native.solve_batch([lp], **opts)
+        _same(got[t], (res[0], xs[0], ys[0]), f"{kind}{i}")
+        r, x, y = got[t]
+        if r["status"] == 0:
+            k = ob.kkt_check(lp, x, y)
+            assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, (kind, i, k)
+
+
+def test_server_budget_and_slot_admission():
+    """Two slots and a pool that holds only a few instances: requests queue,
+    are admitted as memory frees up, and still match the solo results; a
+    request larger than the whole pool finishes with status Error."""
+    opts = dict(max_iter=200000, **TIGHT)
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    bounds = [g.branch_bounds(lpgen.c2_instance_seed(40 + i)) for i in range(6)]
+    budget = 1.5e6
+    srv = native.Server(hbm_budget_bytes=budget, max_slots=2, **opts)
+    base = srv.add_base(g.lp)
+    big = lpgen.c0_plain(seed=7, m=4000, n=10000).lp  # ~10 MB of device state
+    tickets = [srv.submit_bounds(base, *b) for b in bounds]
+    t_big = srv.submit(big)
+    srv.step()
+    st = srv.stats()
+    assert st["running"] + st["preparing"] <= 2 and st["queued"] >= 3, st
+    got = _drain(srv, tickets + [t_big])
+    st = srv.stats()
+    assert st["pool_peak_bytes"] <= st["pool_bytes"] <= budget + 1, st
+    srv.close()
+    assert got[t_big][0]["status"] == 6
+    for t, b in zip(tickets, bounds):
+        res, xs, ys = native.solve_batch([g.lp.with_col_bounds(*b)], **opts)
+        _same(got[t], (res[0], xs[0], ys[0]), f"variant{t}")
+
+
+def test_server_statuses_and_time_limit():
+    cases = {c["name"]: lp for c, lp in closed_form_cases()}
+    srv = native.Server(max_slots=8, max_iter=100000, verbose=0)
+    ts = [srv.submit(cases[k]) for k in ("infeasible", "unbounded_constrained", "simple")]
+    got = _drain(srv, ts)
+    assert [got[t][0]["status"] for t in ts] == [1, 2, 0]
+    assert abs(got[ts[2]][0]["objective"] + 4.0) <= 1e-4
+    srv.close()
+    # A zero time limit stops a request after its first round.
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    srv = native.Server(max_slots=4, max_solve_seconds=0.0, **TIGHT)
+    t = srv.submit(g.lp)
+    r = _drain(srv, [t])[t][0]
+    srv.close()
+    assert r["status"] == 4 and 0 < r["iterations"] <= 2 * 200, r
