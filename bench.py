@@ -15,11 +15,21 @@ batch's stream) and the reference CPU PDLP on the host cores.
 
   python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload c2]
                   [--count 4096] [--precision fp64|mixed]
+
+Workload `c4` drives the continuous-batching server instead: Poisson
+arrivals (--rate per second for --duration seconds), 70% branch-and-bound
+bounds variants of the c2 matrix and 30% fresh LPs with lognormal nnz
+(median 5e4); it reports requests/s served and p50 / p99 latency from each
+request's scheduled arrival to its completion.
+
+  python bench.py --workload c4 [--rate 200] [--duration 60] [--slots 2048]
 """
 import argparse
 import json
 import os
+import queue
 import sys
+import threading
 import time
 
 import numpy as np
@@ -83,12 +93,126 @@ def cpu_baseline(lp, lo, hi, budget_s):
                 sample=f"{done} C2 instances solved back to back on one host core in {el:.1f} s")
 
 
+def run_c4(a, prec):
+    """Serving benchmark: Poisson arrivals into mpx_server_*; a producer
+    thread generates requests ahead of their arrival times."""
+    from mipx_amd import lpgen, native
+    sched = lpgen.c4_schedule(a.rate, a.duration)
+    g = lpgen.c2_shared()
+    srv = native.Server(hbm_budget_bytes=a.budget_gb * 1e9, max_slots=a.slots,
+                        precision=prec, max_iter=a.max_iter, **TOL)
+    base = srv.add_base(g.lp)
+    q = queue.Queue(maxsize=512)
+
+    def produce():
+        for i, (t, variant, nnz) in enumerate(sched):
+            if variant:
+                q.put((i, t, None, g.branch_bounds(lpgen.c4_variant_seed(i))))
+            else:
+                q.put((i, t, lpgen.c4_fresh(i, nnz).lp, None))
+        q.put(None)
+
+    threading.Thread(target=produce, daemon=True).start()
+    info = {}
+    lat = {"variant": [], "fresh": []}
+    status = [0] * 7
+    iters = []
+    worst_kkt = 0.0
+    pending, exhausted = None, False
+    late = 0.0
+    t0 = time.perf_counter()
+    last_log = t0
+    while True:
+        now = time.perf_counter() - t0
+        while not exhausted:
+            if pending is None:
+                try:
+                    pending = q.get_nowait()
+                except queue.Empty:
+                    break
+                if pending is None:
+                    exhausted = True
+                    break
+            i, t_arr, lp, bnd = pending
+            if t_arr > now:
+                break
+            late = max(late, now - t_arr)
+            tk = srv.submit(lp) if lp is not None else srv.submit_bounds(base, *bnd)
+            info[tk] = (t_arr, "fresh" if lp is not None else "variant")
+            pending = None
+        active = srv.step()
+        t_done = time.perf_counter() - t0
+        for tk in srv.poll():
+            r = srv.result(tk, want_vectors=False)[0]
+            t_arr, kind = info.pop(tk)
+            lat[kind].append(1000.0 * (t_done - t_arr))
+            status[r["status"]] += 1
+            iters.append(r["iterations"])
+            if r["status"] == 0:
+                worst_kkt = max(worst_kkt, r["rel_primal"], r["rel_dual"], r["rel_gap"])
+        if time.perf_counter() - last_log > 5.0:
+            last_log = time.perf_counter()
+            st = srv.stats()
+            log(f"t={t_done:.1f}s submitted {st['submitted']} finished {st['finished']} "
+                f"queued {st['queued']} preparing {st['preparing']} running {st['running']}")
+        if exhausted and pending is None and active == 0 and not info:
+            break
+        if t_done > a.duration + a.drain_seconds:
+            log("drain limit reached")
+            break
+        if active == 0:
+            time.sleep(0.0005)
+    elapsed = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    all_lat = lat["variant"] + lat["fresh"]
+    pc = lambda v, p: round(float(np.percentile(v, p)), 2) if v else None
+    done = sum(status)
+    return {
+        "metric": "LP requests/s served at 1e-6 rel. KKT (Poisson arrivals); p50/p99 latency",
+        "value": round(done / elapsed, 3),
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(1000.0 * elapsed, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if prec == 1 else "f64",
+        "data": "synthetic (seeded generator: mipx_amd/csrc/lpgen.cpp, Poisson stream seed 0xC4)",
+        "config": {"workload": f"c4: Poisson {a.rate:g} requests/s for {a.duration:g} s, 70% "
+                               "bounds variants of the c2 matrix (m=3000 n=8000), 30% fresh LPs "
+                               "nnz~lognormal(5e4, 1) in [1e3, 5e6], m=nnz/15, n=2.5m",
+                   "offered_requests": len(sched), "precision": a.precision, "slots": a.slots,
+                   "hbm_budget_gb": a.budget_gb},
+        "offered_rate": round(len(sched) / a.duration, 2),
+        "finished": done,
+        "unfinished": len(sched) - done,
+        "latency_ms": {"p50": pc(all_lat, 50), "p99": pc(all_lat, 99), "max": pc(all_lat, 100),
+                       "variant_p50": pc(lat["variant"], 50), "variant_p99": pc(lat["variant"], 99),
+                       "fresh_p50": pc(lat["fresh"], 50), "fresh_p99": pc(lat["fresh"], 99)},
+        "status_counts": status,
+        "mean_iterations": float(np.mean(iters)) if iters else None,
+        "max_rel_kkt": worst_kkt,
+        "max_submit_lag_s": round(late, 3),
+        "server": {"rounds": st["rounds"], "iterate_ms": round(st["iterate_ms"], 1),
+                   "pool_peak_gb": round(st["pool_peak_bytes"] / 1e9, 3)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4"])
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--slots", type=int, default=2048)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--drain-seconds", type=float, default=120.0)
+    ap.add_argument("--max-iter", type=int, default=400000)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--plain-sample", type=int, default=8)
@@ -106,6 +230,9 @@ def main():
 
     from mipx_amd import native
     prec = 1 if a.precision == "mixed" else 0
+    if a.workload == "c4":
+        print(json.dumps(run_c4(a, prec)), flush=True)
+        return
     log(f"building workload {a.workload} count={a.count}")
     g, lo, hi = c2_workload(a.count)
     lp = g.lp

@@ -272,10 +272,12 @@ int mpx_batch_destroy(mpx_batch* b);
  * one round (termination_eval_frequency iterations, then the per-instance
  * evaluation / restart / termination logic) of every running instance in
  * one launch sequence, retires the finished ones and recycles their slots
- * and memory.  Every instance's arithmetic depends only on its own data, so
- * a request's result is bit-identical to mpx_pdlp_solve_batch() of the same
- * LP alone with the same options (a bounds variant: of the base LP with
- * those column bounds), whatever else is in flight.
+ * and memory.  Fresh LPs iterate in the packed kernels, bounds variants of a
+ * base in the lane-per-instance kernels of that base.  Every instance's
+ * arithmetic depends only on its own data, so a fresh request's result is
+ * bit-identical to mpx_pdlp_solve_batch() of the same LP alone, and a bounds
+ * variant's to mpx_pdlp_solve_batch_shared() of its base with those column
+ * bounds alone (same options), whatever else is in flight.
  *
  * Per request, max_solve_seconds (if >= 0) counts from its activation;
  * stop_fn is not polled.  Calls on one server must not overlap.
@@ -318,6 +320,9 @@ int mpx_server_step(mpx_server* srv, int32_t* active);
  * the part before the request started iterating (queue + scaling). */
 int mpx_server_result(mpx_server* srv, int64_t ticket, int32_t* done,
                       double* x_out, double* y_out, mpx_pdlp_result* res);
+/* Tickets finished since the last poll, in completion order: up to
+ * `capacity` of them into tickets[], their number in *count. */
+int mpx_server_poll(mpx_server* srv, int64_t* tickets, int32_t capacity, int32_t* count);
 int mpx_server_get_stats(mpx_server* srv, mpx_server_stats* stats);
 int mpx_server_destroy(mpx_server* srv);
 

@@ -30,11 +30,19 @@ struct ILane {
     int32_t inner, lim;
 };
 
+// Interleaved iterate storage of one precision: V[group][element][lane].
+template <typename T>
+struct IlBufs {
+    T *x = nullptr, *x0 = nullptr, *xr = nullptr, *l = nullptr, *u = nullptr, *y = nullptr,
+      *y0 = nullptr;
+};
+
 // Slot lists: slots[p][s] = instance id for precision p (0 = FP32, 1 = FP64),
-// counts[p] = number of slots, lanes[p][s] = its scalars.  One block, in
-// instance order (deterministic).
+// counts[p] = number of slots, lanes[p][s] = its scalars.  Only instances
+// whose `owner` equals `owner` take part (all of them when owner < 0).  One
+// block, in instance order (deterministic).
 __global__ __launch_bounds__(1024) void kSlotLists(const BInst* __restrict__ insts, int K,
-                                                   int* __restrict__ slots32,
+                                                   int owner, int* __restrict__ slots32,
                                                    int* __restrict__ slots64,
                                                    ILane* __restrict__ lanes32,
                                                    ILane* __restrict__ lanes64,
@@ -48,7 +56,8 @@ __global__ __launch_bounds__(1024) void kSlotLists(const BInst* __restrict__ ins
         int p32 = 0, p64 = 0;
         if (k < K) {
             const BCtrl& c = *insts[k].ctrl;
-            const bool act = c.phase != kPhaseDone && c.round_iters > 0;
+            const bool mine = owner < 0 || insts[k].owner == owner;
+            const bool act = mine && c.phase != kPhaseDone && c.round_iters > 0;
             p32 = act && c.phase == kPhaseFp32;
             p64 = act && c.phase != kPhaseFp32;
         }

@@ -60,7 +60,8 @@ struct BVecs {
 
 struct BInst {
     int32_t m, n, nnz, sense_max;
-    int32_t preset, use_lds, owner, pad;
+    int32_t preset, use_lds, owner;
+    int32_t il;  // iterated by the lane-per-instance kernels of its shared matrix
     const int *rs, *ci, *cs, *ri;
     // original bounds / objective (certificates)
     const double *lo_o, *hi_o, *rl_o, *ru_o, *obj_o;

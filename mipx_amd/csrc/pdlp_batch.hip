@@ -242,7 +242,7 @@ __global__ void kTileCounts(const BInst* __restrict__ insts, int K, int* __restr
     const BInst& in = insts[k];
     const int cpt = kBlock / presetGC(in.preset), rpt = kBlock / presetGR(in.preset);
     const BCtrl& c = *in.ctrl;
-    const bool act = c.phase != kPhaseDone && c.round_iters > 0;
+    const bool act = !in.il && c.phase != kPhaseDone && c.round_iters > 0;
     const bool p32 = act && c.phase == kPhaseFp32, p64 = act && c.phase != kPhaseFp32;
     const int ct = (in.n + cpt - 1) / cpt, rt = (in.m + rpt - 1) / rpt;
     cnt[0 * K + k] = p32 ? ct : 0;
@@ -301,7 +301,7 @@ __global__ void kTileFill(const BInst* __restrict__ insts, int K, const int* __r
     const BInst& in = insts[k];
     const int cpt = kBlock / presetGC(in.preset), rpt = kBlock / presetGR(in.preset);
     const BCtrl& c = *in.ctrl;
-    const bool act = c.phase != kPhaseDone && c.round_iters > 0;
+    const bool act = !in.il && c.phase != kPhaseDone && c.round_iters > 0;
     if (!act) return;
     const bool p32 = c.phase == kPhaseFp32;
     Tile* cl = p32 ? c32 : c64;
@@ -1172,6 +1172,108 @@ void carveInstance(const BOwner& w, const mpx_lp& lp, int owner_id, int preset, 
     in.ctrl = ctrl;
 }
 
+// ---- lane-per-instance rounds of one shared matrix ----------------------------
+// State of the interleaved kernels for up to `cap` instances of one matrix.
+struct IlSet {
+    IlBufs<float> f;
+    IlBufs<double> d;
+    int *slots32 = nullptr, *slots64 = nullptr, *counts = nullptr;
+    ILane *lanes32 = nullptr, *lanes64 = nullptr;
+    int owner = -1;  // instances taking part (-1: every instance of the table)
+
+    static size_t bytes(int cap, int n, int m, bool mixed) {
+        const size_t ng = static_cast<size_t>((cap + kLanes - 1) / kLanes);
+        const size_t en = ng * kLanes * static_cast<size_t>(n);
+        const size_t em = ng * kLanes * static_cast<size_t>(m);
+        size_t b = 5 * pad(en, 8) + 2 * pad(em, 8);
+        if (mixed) b += 5 * pad(en, 4) + 2 * pad(em, 4);
+        const size_t c = static_cast<size_t>(cap);
+        return b + 2 * pad(c, 4) + 2 * pad(c, sizeof(ILane)) + pad(4, 4);
+    }
+    void carve(Carve& c, int cap, int n, int m, bool mixed) {
+        const size_t ng = static_cast<size_t>((cap + kLanes - 1) / kLanes);
+        const size_t N = static_cast<size_t>(n), M = static_cast<size_t>(m);
+        auto take = [&](auto& b, auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            b.x = c.take<T>(ng * kLanes * N);
+            b.x0 = c.take<T>(ng * kLanes * N);
+            b.xr = c.take<T>(ng * kLanes * N);
+            b.l = c.take<T>(ng * kLanes * N);
+            b.u = c.take<T>(ng * kLanes * N);
+            b.y = c.take<T>(ng * kLanes * M);
+            b.y0 = c.take<T>(ng * kLanes * M);
+        };
+        take(d, static_cast<double*>(nullptr));
+        if (mixed) take(f, static_cast<float*>(nullptr));
+        slots32 = c.take<int>(static_cast<size_t>(cap));
+        slots64 = c.take<int>(static_cast<size_t>(cap));
+        lanes32 = c.take<ILane>(static_cast<size_t>(cap));
+        lanes64 = c.take<ILane>(static_cast<size_t>(cap));
+        counts = c.take<int>(4);
+    }
+};
+
+template <typename F>
+void withCW(int cw, F&& f) {
+    switch (cw) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        default: f(std::integral_constant<int, 1>{}); break;
+    }
+}
+
+// Slot lists of the coming round and per-instance -> interleaved copies.
+void ilGather(hipStream_t st, int grid, const BInst* di, int K, const BOwner& w, IlSet& s,
+              bool f32) {
+    const int n = w.n, m = w.m;
+    kSlotLists<<<1, 1024, 0, st>>>(di, K, s.owner, s.slots32, s.slots64, s.lanes32, s.lanes64,
+                                   s.counts);
+    if (f32) {
+        kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, n, 0, s.f.x, s.f.x0,
+                                                    s.f.l, s.f.u);
+        kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, m, 1, s.f.y, s.f.y0,
+                                                    nullptr, nullptr);
+    }
+    kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x, s.d.x0,
+                                                 s.d.l, s.d.u);
+    kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y, s.d.y0,
+                                                 nullptr, nullptr);
+}
+
+// Iteration k (primal then dual; FP32 instances first).
+void ilIterate(hipStream_t st, int grid, const BInst* di, const BOwner& w, IlSet& s, bool f32,
+               int cw, int k) {
+    const int n = w.n, m = w.m;
+    withCW(cw, [&](auto CWI) {
+        constexpr int CW = decltype(CWI)::value;
+        if (f32) {
+            kILPrimal<float, CW><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, s.lanes32, n, m,
+                                                          w.cs, w.ri, w.valtf, w.cf, w.tauf, s.f.y,
+                                                          s.f.x, s.f.x0, s.f.xr, s.f.l, s.f.u, k);
+            kILDual<float, CW><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, s.lanes32, n, m,
+                                                        w.rs, w.ci, w.valf, w.sigf, w.rlf, w.ruf,
+                                                        s.f.xr, s.f.y, s.f.y0, k);
+        }
+        kILPrimal<double, CW><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, s.lanes64, n,
+                                                       m, w.cs, w.ri, w.valt, w.c, w.tau, s.d.y,
+                                                       s.d.x, s.d.x0, s.d.xr, s.d.l, s.d.u, k);
+        kILDual<double, CW><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, s.lanes64, n, m,
+                                                     w.rs, w.ci, w.val, w.sig, w.rl, w.ru, s.d.xr,
+                                                     s.d.y, s.d.y0, k);
+    });
+}
+
+// Interleaved -> per-instance x and y after the round's last iteration.
+void ilScatter(hipStream_t st, int grid, const BInst* di, const BOwner& w, IlSet& s, bool f32) {
+    const int n = w.n, m = w.m;
+    if (f32) {
+        kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, n, 0, s.f.x);
+        kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, m, 1, s.f.y);
+    }
+    kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x);
+    kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y);
+}
+
 // Iteration presets the online autotuner tries: (lanes per column, lanes per
 // row) of the packed iteration kernels.
 struct IterPreset {
@@ -1321,15 +1423,7 @@ private:
     // Lane-per-instance mode (every instance shares owner 0).
     bool interleaved_ = false;
     std::unique_ptr<DevBuf> d_il_buf_;
-    template <typename T>
-    struct IlBufs {
-        T *x = nullptr, *x0 = nullptr, *xr = nullptr, *l = nullptr, *u = nullptr, *y = nullptr,
-          *y0 = nullptr;
-    };
-    IlBufs<float> il32_;
-    IlBufs<double> il64_;
-    int *slots32_ = nullptr, *slots64_ = nullptr, *slot_counts_ = nullptr;
-    ILane *lanes32_ = nullptr, *lanes64_ = nullptr;
+    IlSet il_;
     void enqueueInterleavedRound(bool timed);
 };
 
@@ -1387,30 +1481,11 @@ void BatchEngine::setup() {
     // Lane-per-instance mode when every instance shares one matrix.
     interleaved_ = allow_interleaved_ && no == 1 && ni >= 1;
     if (interleaved_) {
-        const size_t ng = static_cast<size_t>((ni + kLanes - 1) / kLanes);
-        const size_t N = static_cast<size_t>(owners_[0].n), M = static_cast<size_t>(owners_[0].m);
-        const size_t e64 = ng * kLanes * (5 * N + 2 * M);
-        const size_t bytes_il = e64 * sizeof(double) + (mixed_ ? e64 * sizeof(float) : 0) +
-                                2 * static_cast<size_t>(ni) * (sizeof(int) + sizeof(ILane)) + 4096;
+        const size_t bytes_il = IlSet::bytes(ni, owners_[0].n, owners_[0].m, mixed_);
         d_il_buf_ = std::make_unique<DevBuf>(bytes_il);
         Carve c{static_cast<char*>(d_il_buf_->p), 0, bytes_il};
-        auto carve = [&](auto& b, auto* tag) {
-            using T = std::remove_pointer_t<decltype(tag)>;
-            b.x = c.take<T>(ng * kLanes * N);
-            b.x0 = c.take<T>(ng * kLanes * N);
-            b.xr = c.take<T>(ng * kLanes * N);
-            b.l = c.take<T>(ng * kLanes * N);
-            b.u = c.take<T>(ng * kLanes * N);
-            b.y = c.take<T>(ng * kLanes * M);
-            b.y0 = c.take<T>(ng * kLanes * M);
-        };
-        carve(il64_, static_cast<double*>(nullptr));
-        if (mixed_) carve(il32_, static_cast<float*>(nullptr));
-        slots32_ = c.take<int>(ni);
-        slots64_ = c.take<int>(ni);
-        lanes32_ = c.take<ILane>(ni);
-        lanes64_ = c.take<ILane>(ni);
-        slot_counts_ = c.take<int>(4);
+        il_.carve(c, ni, owners_[0].n, owners_[0].m, mixed_);
+        il_.owner = -1;
     }
     int dev = 0, cus = 256;
     MPX_HIP(hipGetDevice(&dev));
@@ -1534,63 +1609,18 @@ void BatchEngine::enqueueRound(const IterPreset& p, bool timed) {
     kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
 }
 
-template <typename F>
-void withCW(int cw, F&& f) {
-    switch (cw) {
-        case 2: f(std::integral_constant<int, 2>{}); break;
-        case 4: f(std::integral_constant<int, 4>{}); break;
-        default: f(std::integral_constant<int, 1>{}); break;
-    }
-}
-
 void BatchEngine::enqueueInterleavedRound(bool timed) {
     const int K = count();
     const BInst* di = static_cast<const BInst*>(d_insts_buf_->p);
     const BOwner& w = owners_[0];
-    const int n = w.n, m = w.m;
-    kSlotLists<<<1, 1024, 0, st_>>>(di, K, slots32_, slots64_, lanes32_, lanes64_, slot_counts_);
-    const int* c32 = slot_counts_;
-    const int* c64 = slot_counts_ + 1;
     // FP32 kernels only while some instance is still in its FP32 phase
     // (instances never return to FP32, so this is decided on the host).
     const bool f32 = mixed_ && !fp32_gone_;
-    if (f32) {
-        kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, n, 0, il32_.x, il32_.x0,
-                                                      il32_.l, il32_.u);
-        kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, m, 1, il32_.y, il32_.y0,
-                                                      nullptr, nullptr);
-    }
-    kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, n, 0, il64_.x, il64_.x0,
-                                                   il64_.l, il64_.u);
-    kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, m, 1, il64_.y, il64_.y0,
-                                                   nullptr, nullptr);
+    ilGather(st_, grid_, di, K, w, il_, f32);
     if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
-    for (int k = 1; k <= bo_.N; ++k) {
-        withCW(cw_, [&](auto CWI) {
-            constexpr int CW = decltype(CWI)::value;
-            if (f32) {
-                kILPrimal<float, CW><<<grid_, kBlock, 0, st_>>>(
-                    di, slots32_, c32, lanes32_, n, m, w.cs, w.ri, w.valtf, w.cf, w.tauf, il32_.y,
-                    il32_.x, il32_.x0, il32_.xr, il32_.l, il32_.u, k);
-                kILDual<float, CW><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, lanes32_, n, m,
-                                                              w.rs, w.ci, w.valf, w.sigf, w.rlf,
-                                                              w.ruf, il32_.xr, il32_.y, il32_.y0, k);
-            }
-            kILPrimal<double, CW><<<grid_, kBlock, 0, st_>>>(
-                di, slots64_, c64, lanes64_, n, m, w.cs, w.ri, w.valt, w.c, w.tau, il64_.y,
-                il64_.x, il64_.x0, il64_.xr, il64_.l, il64_.u, k);
-            kILDual<double, CW><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, lanes64_, n, m, w.rs,
-                                                           w.ci, w.val, w.sig, w.rl, w.ru, il64_.xr,
-                                                           il64_.y, il64_.y0, k);
-        });
-    }
+    for (int k = 1; k <= bo_.N; ++k) ilIterate(st_, grid_, di, w, il_, f32, cw_, k);
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
-    if (f32) {
-        kDeinterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, n, 0, il32_.x);
-        kDeinterleave<float><<<grid_, kBlock, 0, st_>>>(di, slots32_, c32, m, 1, il32_.y);
-    }
-    kDeinterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, n, 0, il64_.x);
-    kDeinterleave<double><<<grid_, kBlock, 0, st_>>>(di, slots64_, c64, m, 1, il64_.y);
+    ilScatter(st_, grid_, di, w, il_, f32);
     if (f32) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
     kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
 }
@@ -1606,46 +1636,32 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
     const bool f32 = mixed_;  // the FP32 kernels carry the work in mixed mode
     if (interleaved_) {
         const BOwner& w = owners_[0];
-        const int n = w.n, m = w.m;
-        kSlotLists<<<1, 1024, 0, st_>>>(di, K, slots32_, slots64_, lanes32_, lanes64_, slot_counts_);
-        const int* cnt = slot_counts_ + (f32 ? 0 : 1);
-        const int* sl = f32 ? slots32_ : slots64_;
-        const ILane* ln = f32 ? lanes32_ : lanes64_;
-        if (f32) {
-            kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, n, 0, il32_.x, il32_.x0,
-                                                          il32_.l, il32_.u);
-            kInterleave<float><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, m, 1, il32_.y, il32_.y0,
-                                                          nullptr, nullptr);
-        } else {
-            kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, n, 0, il64_.x, il64_.x0,
-                                                           il64_.l, il64_.u);
-            kInterleave<double><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, m, 1, il64_.y, il64_.y0,
-                                                           nullptr, nullptr);
-        }
+        // Only the precision that carries the work is gathered and timed.
+        ilGather(st_, grid_, di, K, w, il_, f32);
         for (int k = 1; k <= launches; ++k) {
             MPX_HIP(hipEventRecord(e[0], st_));
             withCW(cw_, [&](auto CWI) {
                 constexpr int CW = decltype(CWI)::value;
                 if (f32)
                     kILPrimal<float, CW><<<grid_, kBlock, 0, st_>>>(
-                        di, sl, cnt, ln, n, m, w.cs, w.ri, w.valtf, w.cf, w.tauf, il32_.y, il32_.x,
-                        il32_.x0, il32_.xr, il32_.l, il32_.u, k);
+                        di, il_.slots32, il_.counts, il_.lanes32, w.n, w.m, w.cs, w.ri, w.valtf,
+                        w.cf, w.tauf, il_.f.y, il_.f.x, il_.f.x0, il_.f.xr, il_.f.l, il_.f.u, k);
                 else
                     kILPrimal<double, CW><<<grid_, kBlock, 0, st_>>>(
-                        di, sl, cnt, ln, n, m, w.cs, w.ri, w.valt, w.c, w.tau, il64_.y, il64_.x,
-                        il64_.x0, il64_.xr, il64_.l, il64_.u, k);
+                        di, il_.slots64, il_.counts + 1, il_.lanes64, w.n, w.m, w.cs, w.ri, w.valt,
+                        w.c, w.tau, il_.d.y, il_.d.x, il_.d.x0, il_.d.xr, il_.d.l, il_.d.u, k);
             });
             MPX_HIP(hipEventRecord(e[1], st_));
             withCW(cw_, [&](auto CWI) {
                 constexpr int CW = decltype(CWI)::value;
                 if (f32)
-                    kILDual<float, CW><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs, w.ci,
-                                                                  w.valf, w.sigf, w.rlf, w.ruf,
-                                                                  il32_.xr, il32_.y, il32_.y0, k);
+                    kILDual<float, CW><<<grid_, kBlock, 0, st_>>>(
+                        di, il_.slots32, il_.counts, il_.lanes32, w.n, w.m, w.rs, w.ci, w.valf,
+                        w.sigf, w.rlf, w.ruf, il_.f.xr, il_.f.y, il_.f.y0, k);
                 else
-                    kILDual<double, CW><<<grid_, kBlock, 0, st_>>>(di, sl, cnt, ln, n, m, w.rs,
-                                                                   w.ci, w.val, w.sig, w.rl, w.ru,
-                                                                   il64_.xr, il64_.y, il64_.y0, k);
+                    kILDual<double, CW><<<grid_, kBlock, 0, st_>>>(
+                        di, il_.slots64, il_.counts + 1, il_.lanes64, w.n, w.m, w.rs, w.ci, w.val,
+                        w.sig, w.rl, w.ru, il_.d.xr, il_.d.y, il_.d.y0, k);
             });
             MPX_HIP(hipEventRecord(e[2], st_));
             MPX_HIP(hipEventSynchronize(e[2]));
@@ -1656,7 +1672,7 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
             td += b;
         }
         int c2[2];
-        MPX_HIP(hipMemcpy(c2, slot_counts_, sizeof(c2), hipMemcpyDeviceToHost));
+        MPX_HIP(hipMemcpy(c2, il_.counts, sizeof(c2), hipMemcpyDeviceToHost));
         active = f32 ? c2[0] : c2[1];
     } else {
         int* cnt = static_cast<int*>(d_cnt_buf_->p);
@@ -1773,7 +1789,7 @@ int BatchEngine::step(int rounds) {
     MPX_HIP(hipMemcpyAsync(&active, dcount, sizeof(int), hipMemcpyDeviceToHost, st_));
     if (mixed_ && !fp32_gone_) {
         // FP32 instances at the start of the last round (0 stays 0).
-        const int* src = interleaved_ ? slot_counts_ : totals_;
+        const int* src = interleaved_ ? il_.counts : totals_;
         int c2[2] = {1, 1};
         MPX_HIP(hipMemcpyAsync(c2, src, sizeof(c2), hipMemcpyDeviceToHost, st_));
         MPX_HIP(hipStreamSynchronize(st_));
@@ -2015,19 +2031,23 @@ struct ServeLp {
         return ownerBytes(lp.num_rows, lp.num_cols, lp.row_starts[lp.num_rows]) +
                pad(1, sizeof(BOwner)) + pad(1, sizeof(int));
     }
-    // Carves and uploads the matrix, then queues its scaling and power
-    // iteration on `st` (one workgroup, like the batched engine).
-    void setup(const mpx_pdlp_options& o, const OwnerHost& h, Carve& cv, hipStream_t st) {
+    // Carves the matrix arrays and queues their uploads on `st`.
+    void upload(const mpx_pdlp_options& o, const OwnerHost& h, Carve& cv, hipStream_t st) {
         carveOwner(lp, h, o, cv, st, owner);
         preset = ownerPreset(o, owner.nnz, owner.m, owner.n);
         d_owner = cv.take<BOwner>(1);
         d_preset = cv.take<int>(1);
         MPX_HIP(hipMemcpyAsync(d_owner, &owner, sizeof(BOwner), hipMemcpyHostToDevice, st));
         MPX_HIP(hipMemcpyAsync(d_preset, &preset, sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    // Scaling and power iteration (one workgroup, like the batched engine).
+    void prepare(hipStream_t st) {
         kOwnerScale<<<1, kBlock, 0, st>>>(d_owner);
         kOwnerPower<<<1, kBlock, 0, st>>>(d_owner, d_preset);
         MPX_HIP(hipGetLastError());
     }
+    // Lane-per-instance state of a base's variants.
+    IlSet il;
 };
 
 struct ServeReq {
@@ -2066,14 +2086,19 @@ struct PinnedArray {
 
 class Server {
 public:
-    static constexpr int kSetupStreams = 4;
+    // Side streams for the setup of fresh LPs (with the main stream this is
+    // the hardware queue count); matrices above kBigSetupNnz entries use the
+    // last one.
+    static constexpr int kSetupStreams = 3;
+    static constexpr int kBigSetupNnz = 200000;
 
     Server(const mpx_pdlp_options& o, double budget, int slots)
         : o_(o), S_(slots > 0 ? slots : 1024), h_ctrl_(S_), h_new_(S_), h_args_(S_),
-          h_slots_(S_), h_ptrs_(2 * static_cast<size_t>(S_)), h_totals_(8) {
+          h_slots_(S_), h_ptrs_(2 * static_cast<size_t>(S_)) {
         MPX_HIP(hipSetDevice(o.device));
         MPX_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         for (auto& s : setup_st_) MPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        MPX_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming));
         bo_ = makeBOptions(o);
         mixed_ = o.precision == MPX_PREC_MIXED;
         size_t freeb = 0, total = 0;
@@ -2115,22 +2140,30 @@ public:
             if (r->ready) (void)hipEventDestroy(r->ready);
         for (auto& kv : graphs_) kv.second.reset();
         for (auto& s : setup_st_) (void)hipStreamDestroy(s);
+        (void)hipEventDestroy(uploaded_);
         (void)hipStreamDestroy(st_);
     }
 
     int addBase(const mpx_lp& lp) {
         auto b = std::make_unique<ServeLp>(lp);
         const OwnerHost h = ownerHostData(b->lp);
-        const size_t need = b->bytes();
+        const size_t il_bytes = IlSet::bytes(S_, b->lp.num_cols, b->lp.num_rows, mixed_);
+        const size_t need = b->bytes() + il_bytes;
         void* mem = pool_->alloc(need);
         if (!mem) {
             setError("mpx_server_add_base: the base LP does not fit the server's memory pool");
             throw OutOfMemory();
         }
         Carve cv{static_cast<char*>(mem), 0, need};
-        b->setup(o_, h, cv, st_);
+        b->upload(o_, h, cv, st_);
+        b->prepare(st_);
+        b->il.carve(cv, S_, b->lp.num_cols, b->lp.num_rows, mixed_);
+        b->il.owner = static_cast<int>(bases_.size());
         MPX_HIP(hipStreamSynchronize(st_));
         bases_.push_back(std::move(b));
+        // The round graphs include every base's lane-per-instance kernels.
+        for (auto& kv : graphs_) kv.second.reset();
+        graphs_.clear();
         return static_cast<int>(bases_.size()) - 1;
     }
     int numBases() const { return static_cast<int>(bases_.size()); }
@@ -2152,8 +2185,8 @@ public:
     }
 
     int step() {
-        activate();
         admit();
+        activate();
         iterate();
         retire();
         enforceTimeLimit();
@@ -2171,6 +2204,15 @@ public:
             std::memcpy(y, it->second.y.data(), it->second.y.size() * sizeof(double));
         done_.erase(it);
         return 1;
+    }
+
+    int poll(int64_t* out, int cap) {
+        int k = 0;
+        while (k < cap && !newly_done_.empty()) {
+            out[k++] = newly_done_.front();
+            newly_done_.pop_front();
+        }
+        return k;
     }
 
     void stats(mpx_server_stats* s) const {
@@ -2229,12 +2271,16 @@ private:
                                                       r->t_submit).count();
         d.res.setup_seconds = d.res.seconds;
         done_[r->ticket] = std::move(d);
+        newly_done_.push_back(r->ticket);
         ++finished_;
     }
 
-    // Queued -> preparing: memory, a slot, uploads and (fresh LPs) the
-    // scaling and power iteration on a side stream.
+    // Queued -> preparing: memory, a slot, uploads on the main stream (idle
+    // between rounds, so the pageable copies do not wait behind kernels),
+    // then for fresh LPs the scaling and power iteration on a side stream
+    // that waits for the uploads.  Bounds variants need no setup.
     void admit() {
+        std::vector<ServeReq*> fresh;
         while (!queued_.empty() && !free_slots_.empty()) {
             ServeReq& r = *queued_.front();
             const size_t need = bytesFor(r);
@@ -2252,23 +2298,35 @@ private:
             q->mem = mem;
             q->slot = free_slots_.back();
             free_slots_.pop_back();
-            hipStream_t st = setup_st_[next_setup_++ % kSetupStreams];
             Carve cv{static_cast<char*>(mem), 0, need};
             BCtrl* ctrl = static_cast<BCtrl*>(ctrl_->p) + q->slot;
             if (q->own) {
-                q->own->setup(o_, q->oh, cv, st);
-                carveInstance(q->own->owner, q->own->lp, 0, q->own->preset, nullptr, nullptr,
-                              mixed_, cv, st, ctrl, q->inst, q->args);
+                q->own->upload(o_, q->oh, cv, st_);
+                carveInstance(q->own->owner, q->own->lp, -1, q->own->preset, nullptr, nullptr,
+                              mixed_, cv, st_, ctrl, q->inst, q->args);
+                fresh.push_back(q.get());
             } else {
                 const ServeLp& b = *bases_[q->base];
                 carveInstance(b.owner, b.lp, q->base, b.preset, q->vlo.data(), q->vhi.data(),
-                              mixed_, cv, st, ctrl, q->inst, q->args);
+                              mixed_, cv, st_, ctrl, q->inst, q->args);
+                q->inst.il = 1;
             }
             q->xo = cv.take<double>(static_cast<size_t>(q->inst.n));
             q->yo = cv.take<double>(static_cast<size_t>(q->inst.m));
+            preparing_.push_back(std::move(q));
+        }
+        if (fresh.empty()) return;
+        MPX_HIP(hipEventRecord(uploaded_, st_));
+        for (ServeReq* q : fresh) {
+            // Large matrices get the last side stream so that they do not
+            // hold up the small ones behind them.
+            const bool big = q->inst.nnz > kBigSetupNnz;
+            hipStream_t st = big ? setup_st_[kSetupStreams - 1]
+                                 : setup_st_[next_setup_++ % (kSetupStreams - 1)];
+            MPX_HIP(hipStreamWaitEvent(st, uploaded_, 0));
+            q->own->prepare(st);
             MPX_HIP(hipEventCreateWithFlags(&q->ready, hipEventDisableTiming));
             MPX_HIP(hipEventRecord(q->ready, st));
-            preparing_.push_back(std::move(q));
         }
     }
 
@@ -2276,20 +2334,27 @@ private:
     // table rows written and instances initialised on the main stream.
     void activate() {
         if (preparing_.empty()) return;
-        if (running_ == 0) MPX_HIP(hipEventSynchronize(preparing_.front()->ready));
+        if (running_ == 0) {
+            // Nothing to iterate meanwhile: wait for the first setup.
+            bool any = false;
+            for (auto& q : preparing_) any = any || q->ready == nullptr;
+            if (!any) MPX_HIP(hipEventSynchronize(preparing_.front()->ready));
+        }
         int k = 0;
         size_t need_c = 0, need_r = 0;
         std::vector<std::unique_ptr<ServeReq>> still;
         const auto now = std::chrono::steady_clock::now();
         for (auto& q : preparing_) {
-            const hipError_t e = hipEventQuery(q->ready);
-            if (e == hipErrorNotReady) {
-                still.push_back(std::move(q));
-                continue;
+            if (q->ready) {
+                const hipError_t e = hipEventQuery(q->ready);
+                if (e == hipErrorNotReady) {
+                    still.push_back(std::move(q));
+                    continue;
+                }
+                MPX_HIP(e);
+                (void)hipEventDestroy(q->ready);
+                q->ready = nullptr;
             }
-            MPX_HIP(e);
-            (void)hipEventDestroy(q->ready);
-            q->ready = nullptr;
             // The pageable uploads have been consumed: drop the host copies.
             q->oh = OwnerHost();
             h_new_.p[k] = q->inst;
@@ -2297,8 +2362,10 @@ private:
             h_slots_.p[k] = q->slot;
             ++k;
             q->t_active = now;
-            need_c += static_cast<size_t>((q->inst.n + 15) / 16);
-            need_r += static_cast<size_t>((q->inst.m + 15) / 16);
+            if (!q->inst.il) {
+                need_c += static_cast<size_t>((q->inst.n + 15) / 16);
+                need_r += static_cast<size_t>((q->inst.m + 15) / 16);
+            }
             const int s = q->slot;
             slot_req_[s] = std::move(q);
             ++running_;
@@ -2347,6 +2414,7 @@ private:
         kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
         kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
         const bool f32 = mixed_ && !fp32_gone_;
+        for (auto& b : bases_) ilGather(st_, grid_, di, K, b->owner, b->il, f32);
         for (int k = 1; k <= bo_.N; ++k) {
             if (f32) {
                 kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
@@ -2354,7 +2422,9 @@ private:
             }
             kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, tc64_, totals_ + 2, k);
             kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, tr64_, totals_ + 3, k);
+            for (auto& b : bases_) ilIterate(st_, grid_, di, b->owner, b->il, f32, 1, k);
         }
+        for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, f32);
         if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
         kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
     }
@@ -2392,11 +2462,14 @@ private:
         launchRound();
         ++rounds_;
         MPX_HIP(hipMemcpyAsync(h_ctrl_.p, ctrl_->p, sizeof(BCtrl) * S_, hipMemcpyDeviceToHost, st_));
-        MPX_HIP(hipMemcpyAsync(h_totals_.p, totals_, sizeof(int) * 2, hipMemcpyDeviceToHost, st_));
         MPX_HIP(hipStreamSynchronize(st_));
-        if (mixed_ && !fp32_gone_ && h_totals_.p[0] + h_totals_.p[1] == 0) {
-            // No FP32 instance was active in this round (new ones reset it).
-            fp32_gone_ = true;
+        if (mixed_ && !fp32_gone_) {
+            // FP32 kernels stay in the round graph while a running instance
+            // is in its FP32 phase (activation of new ones resets this).
+            bool any = false;
+            for (int s = 0; s < S_ && !any; ++s)
+                any = slot_req_[s] && h_ctrl_.p[s].phase == kPhaseFp32;
+            fp32_gone_ = !any;
         }
         iterate_ms_ += 1000.0 * std::chrono::duration<double>(std::chrono::steady_clock::now() -
                                                               t0).count();
@@ -2470,12 +2543,15 @@ private:
             if (!keepX) std::fill(d.x.begin(), d.x.end(), 0.0);
             if (!keepY) std::fill(d.y.begin(), d.y.end(), 0.0);
             inst_iters_ += c.total;
-            tiles_c_ -= static_cast<size_t>((n + 15) / 16);
-            tiles_r_ -= static_cast<size_t>((m + 15) / 16);
+            if (!q->inst.il) {
+                tiles_c_ -= static_cast<size_t>((n + 15) / 16);
+                tiles_r_ -= static_cast<size_t>((m + 15) / 16);
+            }
             pool_->release(q->mem);
             free_slots_.push_back(s);
             --running_;
             done_[q->ticket] = std::move(d);
+            newly_done_.push_back(q->ticket);
             ++finished_;
         }
     }
@@ -2507,7 +2583,7 @@ private:
     PinnedArray<BInitArgs> h_args_;
     PinnedArray<int> h_slots_;
     PinnedArray<double*> h_ptrs_;
-    PinnedArray<int> h_totals_;
+    hipEvent_t uploaded_ = nullptr;
     Tile *tc32_ = nullptr, *tr32_ = nullptr, *tc64_ = nullptr, *tr64_ = nullptr;
     int* totals_ = nullptr;
     size_t cap_c_ = 0, cap_r_ = 0, tiles_c_ = 0, tiles_r_ = 0;
@@ -2522,6 +2598,7 @@ private:
     std::vector<int> free_slots_;
     int running_ = 0;
     std::map<int64_t, ServeDone> done_;
+    std::deque<int64_t> newly_done_;
     int64_t next_ticket_ = 0, finished_ = 0, rounds_ = 0, inst_iters_ = 0;
     double iterate_ms_ = 0.0;
 
@@ -2854,6 +2931,12 @@ int mpx_server_result(mpx_server* srv, int64_t ticket, int32_t* done, double* x_
         return MPX_ERR_INVALID;
     }
     *done = r;
+    return MPX_OK;
+}
+
+int mpx_server_poll(mpx_server* srv, int64_t* tickets, int32_t capacity, int32_t* count) {
+    if (!srv || !count || capacity < 0 || (capacity > 0 && !tickets)) return MPX_ERR_INVALID;
+    *count = srv->s->poll(tickets, capacity);
     return MPX_OK;
 }
 

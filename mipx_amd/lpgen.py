@@ -10,6 +10,9 @@ feasible primal (x0, b = A x0 plus <= slack) and dual (c = A' y0 + s0):
                 per instance a B&B branching of 2-10% of the 40% integer cols
   c3_hetero     m = exp(U[ln 200, ln 5e4]), n = 2.5 m, U{4..8} nnz/col,
                 values +-10^U[-3, 3]
+  c4 serving    Poisson arrivals; 70% bounds variants of the c2 matrix, 30%
+                fresh c3-style LPs with nnz ~ lognormal(median 5e4, sigma 1)
+                clipped to [1e3, 5e6] and m = nnz / 15
 """
 import ctypes as C
 import math
@@ -130,3 +133,30 @@ def c3_instance(i, m, base=0xC3):
     m = int(m)
     return Generated(params(m, int(round(2.5 * m)), 1, 4, 8, value_kind=1, vlo=-3.0, vhi=3.0,
                             seed=(base << 32) + int(i)))
+
+
+def c4_schedule(rate=200.0, duration=60.0, seed=0xC4, variant_fraction=0.7,
+                nnz_median=5e4, nnz_sigma=1.0, nnz_lo=1e3, nnz_hi=5e6):
+    """[(arrival seconds, is_variant, nnz)] of a Poisson stream of requests,
+    from a numpy PCG stream (every request draws all three numbers)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        variant = bool(rng.uniform() < variant_fraction)
+        nnz = float(np.clip(rng.lognormal(math.log(nnz_median), nnz_sigma), nnz_lo, nnz_hi))
+        if t >= duration:
+            return out
+        out.append((t, variant, nnz))
+
+
+def c4_fresh(i, nnz, base=0xC4):
+    """Fresh request i: a c3-style LP with about `nnz` entries (m = nnz / 15)."""
+    m = max(20, int(round(nnz / 15.0)))
+    return Generated(params(m, int(round(2.5 * m)), 1, 4, 8, value_kind=1, vlo=-3.0, vhi=3.0,
+                            seed=(base << 32) + int(i)))
+
+
+def c4_variant_seed(i, base=0xC4):
+    return (base << 32) + int(i)

@@ -18,7 +18,8 @@ EXPORTED = [
     "mpx_device_synchronize", "mpx_copy_bench",
     "mpx_pdlp_solve", "mpx_pdlp_solve_batch_shared", "mpx_pdlp_solve_batch",
     "mpx_server_create", "mpx_server_add_base", "mpx_server_submit", "mpx_server_submit_bounds",
-    "mpx_server_step", "mpx_server_result", "mpx_server_get_stats", "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
+    "mpx_server_step", "mpx_server_result", "mpx_server_poll", "mpx_server_get_stats",
+    "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
     "mpx_pdhg_iterate", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
     "mpx_batch_results", "mpx_batch_get_stats", "mpx_batch_profile", "mpx_batch_destroy",
 ]
@@ -49,6 +50,7 @@ def lib():
     L.mpx_server_submit_bounds.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                            C.POINTER(C.c_int64)]
     L.mpx_server_get_stats.argtypes = [C.c_void_p, C.POINTER(MpxServerStats)]
+    L.mpx_server_poll.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
     L.mpx_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     L.mpx_server_result.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.c_void_p,
                                     C.c_void_p, C.POINTER(MpxPdlpResult)]
@@ -302,6 +304,14 @@ class Server:
             return None
         del self._dims[ticket]
         return res.as_dict(), x, y
+
+    def poll(self, capacity=4096):
+        """Tickets finished since the last poll (completion order)."""
+        buf = np.empty(capacity, np.int64)
+        cnt = C.c_int32(0)
+        _check(lib().mpx_server_poll(self._h, buf.ctypes.data, capacity, C.byref(cnt)),
+               "mpx_server_poll")
+        return buf[:cnt.value].tolist()
 
     def stats(self):
         st = MpxServerStats()

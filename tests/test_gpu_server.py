@@ -1,8 +1,10 @@
 """GPU tests of the continuous-batching serving loop (mpx_server_*), through
 the C-ABI: requests admitted at different times, sharing the device with
 whatever else is in flight, give bit-identical results to the same LP solved
-alone by the batched entry point (itself checked against the oracle in
-test_gpu_batch.py); admission respects the memory budget and the slot count."""
+alone by the batched entry points (themselves checked against the oracle in
+test_gpu_batch.py): a fresh LP as mpx_pdlp_solve_batch of that LP, a bounds
+variant as mpx_pdlp_solve_batch_shared of its base with those bounds.
+Admission respects the memory budget and the slot count."""
 import numpy as np
 import pytest
 
@@ -25,11 +27,10 @@ def _drain(srv, tickets, submit_at=None):
         if k in submit_at:
             tickets += submit_at.pop(k)()
         active = srv.step()
-        for t in tickets:
-            if t not in out:
-                r = srv.result(t)
-                if r is not None:
-                    out[t] = r
+        for t in srv.poll():
+            r = srv.result(t)
+            assert r is not None, t
+            out[t] = r
         if active == 0 and not submit_at and len(out) == len(tickets):
             return out
     raise AssertionError(f"server did not drain in {MAX_STEPS} steps: {srv.stats()}")
@@ -80,8 +81,13 @@ def test_server_results_identical_to_solo_batch(precision):
     assert st["finished"] == len(tickets) and st["running"] == 0 and st["pool_used_bytes"] > 0
     srv.close()
     for t, (kind, i) in kinds.items():
-        lp = g.lp.with_col_bounds(*bounds[i]) if kind == "variant" else fresh[i]
-        res, xs, ys = native.solve_batch([lp], **opts)
+        if kind == "variant":
+            lp = g.lp.with_col_bounds(*bounds[i])
+            res, xs, ys = native.solve_batch_shared(g.lp, bounds[i][0][None], bounds[i][1][None],
+                                                    **opts)
+        else:
+            lp = fresh[i]
+            res, xs, ys = native.solve_batch([lp], **opts)
         _same(got[t], (res[0], xs[0], ys[0]), f"{kind}{i}")
         r, x, y = got[t]
         if r["status"] == 0:
@@ -89,30 +95,59 @@ def test_server_results_identical_to_solo_batch(precision):
             assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, (kind, i, k)
 
 
-def test_server_budget_and_slot_admission():
-    """Two slots and a pool that holds only a few instances: requests queue,
-    are admitted as memory frees up, and still match the solo results; a
-    request larger than the whole pool finishes with status Error."""
+def _alone_shared(g, b, opts):
+    res, xs, ys = native.solve_batch_shared(g.lp, b[0][None], b[1][None], **opts)
+    return res[0], xs[0], ys[0]
+
+
+def test_server_slot_admission():
+    """Two slots: requests queue and are admitted as slots free up."""
     opts = dict(max_iter=200000, **TIGHT)
     g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
     bounds = [g.branch_bounds(lpgen.c2_instance_seed(40 + i)) for i in range(6)]
-    budget = 1.5e6
-    srv = native.Server(hbm_budget_bytes=budget, max_slots=2, **opts)
+    srv = native.Server(max_slots=2, **opts)
     base = srv.add_base(g.lp)
-    big = lpgen.c0_plain(seed=7, m=4000, n=10000).lp  # ~10 MB of device state
+    tickets = [srv.submit_bounds(base, *b) for b in bounds]
+    srv.step()
+    st = srv.stats()
+    assert st["running"] + st["preparing"] <= 2 and st["queued"] >= 4, st
+    got = _drain(srv, tickets)
+    srv.close()
+    for t, b in zip(tickets, bounds):
+        _same(got[t], _alone_shared(g, b, opts), f"variant{t}")
+
+
+def test_server_memory_admission():
+    """A pool that holds the base and about two requests: the rest wait for
+    memory (first in, first out); a request larger than the whole pool
+    finishes with status Error."""
+    opts = dict(max_iter=200000, **TIGHT)
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    bounds = [g.branch_bounds(lpgen.c2_instance_seed(60 + i)) for i in range(5)]
+    probe = native.Server(hbm_budget_bytes=64e6, max_slots=64, **opts)
+    base = probe.add_base(g.lp)
+    b0 = probe.stats()["pool_used_bytes"]
+    probe.submit_bounds(base, *bounds[0])
+    probe.step()
+    per = probe.stats()["pool_used_bytes"] - b0
+    probe.close()
+    assert per > 0
+    budget = b0 + 2.5 * per
+    srv = native.Server(hbm_budget_bytes=budget, max_slots=64, **opts)
+    base = srv.add_base(g.lp)
+    big = lpgen.c0_plain(seed=7, m=4000, n=10000).lp  # far more than the pool
     tickets = [srv.submit_bounds(base, *b) for b in bounds]
     t_big = srv.submit(big)
     srv.step()
     st = srv.stats()
-    assert st["running"] + st["preparing"] <= 2 and st["queued"] >= 3, st
+    assert st["running"] + st["preparing"] == 2 and st["queued"] >= 3, st
     got = _drain(srv, tickets + [t_big])
     st = srv.stats()
     assert st["pool_peak_bytes"] <= st["pool_bytes"] <= budget + 1, st
     srv.close()
     assert got[t_big][0]["status"] == 6
     for t, b in zip(tickets, bounds):
-        res, xs, ys = native.solve_batch([g.lp.with_col_bounds(*b)], **opts)
-        _same(got[t], (res[0], xs[0], ys[0]), f"variant{t}")
+        _same(got[t], _alone_shared(g, b, opts), f"variant{t}")
 
 
 def test_server_statuses_and_time_limit():
