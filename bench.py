@@ -120,6 +120,7 @@ def run_c4(a, prec):
     worst_kkt = 0.0
     pending, exhausted = None, False
     late = 0.0
+    t_submit = t_result = 0.0
     t0 = time.perf_counter()
     last_log = t0
     while True:
@@ -137,11 +138,14 @@ def run_c4(a, prec):
             if t_arr > now:
                 break
             late = max(late, now - t_arr)
+            ts = time.perf_counter()
             tk = srv.submit(lp) if lp is not None else srv.submit_bounds(base, *bnd)
+            t_submit += time.perf_counter() - ts
             info[tk] = (t_arr, "fresh" if lp is not None else "variant")
             pending = None
         active = srv.step()
         t_done = time.perf_counter() - t0
+        ts = time.perf_counter()
         for tk in srv.poll():
             r = srv.result(tk, want_vectors=False)[0]
             t_arr, kind = info.pop(tk)
@@ -150,11 +154,15 @@ def run_c4(a, prec):
             iters.append(r["iterations"])
             if r["status"] == 0:
                 worst_kkt = max(worst_kkt, r["rel_primal"], r["rel_dual"], r["rel_gap"])
+        t_result += time.perf_counter() - ts
         if time.perf_counter() - last_log > 5.0:
             last_log = time.perf_counter()
             st = srv.stats()
             log(f"t={t_done:.1f}s submitted {st['submitted']} finished {st['finished']} "
-                f"queued {st['queued']} preparing {st['preparing']} running {st['running']}")
+                f"queued {st['queued']} preparing {st['preparing']} running {st['running']} "
+                f"rounds {st['rounds']} iterate {st['iterate_ms']:.0f} admit {st['admit_ms']:.0f} "
+                f"activate {st['activate_ms']:.0f} retire {st['retire_ms']:.0f} ms, "
+                f"max step {st['max_step_ms']:.0f} ms, submit {1000 * t_submit:.0f} ms")
         if exhausted and pending is None and active == 0 and not info:
             break
         if t_done > a.duration + a.drain_seconds:
@@ -197,6 +205,12 @@ def run_c4(a, prec):
         "max_rel_kkt": worst_kkt,
         "max_submit_lag_s": round(late, 3),
         "server": {"rounds": st["rounds"], "iterate_ms": round(st["iterate_ms"], 1),
+                   "admit_ms": round(st["admit_ms"], 1), "activate_ms": round(st["activate_ms"], 1),
+                   "retire_ms": round(st["retire_ms"], 1), "max_step_ms": round(st["max_step_ms"], 1),
+                   "setup_query_ms": round(st["setup_query_ms"], 1),
+                   "setup_wait_ms": round(st["setup_wait_ms"], 1),
+                   "host_submit_ms": round(1000 * t_submit, 1),
+                   "host_result_ms": round(1000 * t_result, 1),
                    "pool_peak_gb": round(st["pool_peak_bytes"] / 1e9, 3)},
     }
 

@@ -297,6 +297,12 @@ typedef struct mpx_server_stats {
     double pool_peak_bytes;
     double iterate_ms;         /* host wall time spent in rounds (launch + wait) */
     int64_t instance_iterations; /* PDHG iterations of finished requests */
+    double admit_ms;           /* host wall time in admission (uploads, setup launch) */
+    double activate_ms;        /* ... in activation (incl. waiting for a setup) */
+    double retire_ms;          /* ... in retirement (recovery, copies out) */
+    double max_step_ms;        /* longest single mpx_server_step */
+    double setup_query_ms;     /* ... of it in polling setup completion */
+    double setup_wait_ms;      /* ... of it waiting for a setup (nothing running) */
 } mpx_server_stats;
 
 int mpx_server_create(const mpx_pdlp_options* opts, double hbm_budget_bytes,
@@ -372,6 +378,12 @@ typedef struct mpx_scaled_problem {
 
 int mpx_pdlp_scale_problem(const mpx_lp* lp, const mpx_pdlp_options* opts,
                            mpx_scaled_problem* out);
+
+/* The same outputs from the batched engines' device setup (multi-workgroup
+ * passes over a cohort of matrices, pdlp.cpp:469-669 and 748-795), which the
+ * batched solves and the serving loop use. */
+int mpx_pdlp_scale_problem_batched(const mpx_lp* lp, const mpx_pdlp_options* opts,
+                                   mpx_scaled_problem* out);
 
 /*
  * Runs `iters` Halpern PDHG iterations (reference loop body,

@@ -235,6 +235,12 @@ class MpxServerStats(C.Structure):
         ("pool_peak_bytes", C.c_double),
         ("iterate_ms", C.c_double),
         ("instance_iterations", C.c_int64),
+        ("admit_ms", C.c_double),
+        ("activate_ms", C.c_double),
+        ("retire_ms", C.c_double),
+        ("max_step_ms", C.c_double),
+        ("setup_query_ms", C.c_double),
+        ("setup_wait_ms", C.c_double),
     ]
 
     def as_dict(self):

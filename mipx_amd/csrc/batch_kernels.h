@@ -23,6 +23,15 @@
 
 namespace mpx {
 
+// (lanes per column, lanes per row) presets the autotuner chooses from.
+constexpr int kNumPresets = 8;
+__host__ __device__ constexpr int presetGC(int p) {
+    return p == 0 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : p <= 6 ? 8 : 16;
+}
+__host__ __device__ constexpr int presetGR(int p) {
+    return p == 0 ? 1 : p == 1 ? 2 : p <= 3 ? 4 : p <= 5 ? 8 : 16;
+}
+
 // Per-instance control state (FP64 scalars of the reference's host loop).
 struct BCtrl {
     double step, pw;
@@ -70,6 +79,24 @@ struct BInst {
     BVecs<float> f;
     BVecs<double> d;
     BCtrl* ctrl;
+};
+
+// A distinct constraint matrix of a batch (an "owner"): its original data,
+// the scaled problem shared by its instances, FP32 copies and the
+// power-iteration scratch.
+struct BOwner {
+    int32_t m, n, nnz, do_ruiz, ruiz_iters, do_pc, do_rescale, norm_inf, sense_max, pad;
+    const int *rs, *ci, *cs, *ri, *perm;
+    const double *val_o, *obj_o, *lo_o, *hi_o, *rl_o, *ru_o;
+    double *val, *valt, *c, *l, *u, *rl, *ru, *row_scale, *col_scale, *cmax, *sig, *tau;
+    float *valf, *valtf, *cf, *tauf, *sigf, *rlf, *ruf;
+    double* scal;  // [os, cs, cnorm, bnorm, sigma_sq, power_iters]
+    // power iteration: start vector (kept, so setup can be re-run) and
+    // scratch (n, m, n)
+    double *px0, *px, *py, *pxn;
+    double offset;
+    int32_t sv_max_iter;
+    double sv_tol;
 };
 
 // ---- block helpers -----------------------------------------------------------

@@ -18,8 +18,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -27,23 +29,17 @@
 #include <memory>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mipx_pdlp.h"
 #include "batch_interleaved.h"
 #include "batch_kernels.h"
+#include "batch_setup.h"
 #include "device_common.h"
 
 namespace mpx {
 
-// (lanes per column, lanes per row) presets the autotuner chooses from.
-constexpr int kNumPresets = 8;
-__host__ __device__ constexpr int presetGC(int p) {
-    return p == 0 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : p <= 6 ? 8 : 16;
-}
-__host__ __device__ constexpr int presetGR(int p) {
-    return p == 0 ? 1 : p == 1 ? 2 : p <= 3 ? 4 : p <= 5 ? 8 : 16;
-}
 
 // Heuristic preset from mean entries per column / row (about two entries per
 // lane); the measured autotuner (mpx_batch_autotune) can override it.
@@ -561,228 +557,6 @@ __global__ void kAbortActive(const BInst* __restrict__ insts, int count, int sta
     }
 }
 
-// ---- setup kernels (one workgroup per distinct matrix) ---------------------
-struct BOwner {
-    int32_t m, n, nnz, do_ruiz, ruiz_iters, do_pc, do_rescale, norm_inf, sense_max, pad;
-    const int *rs, *ci, *cs, *ri, *perm;
-    const double *val_o, *obj_o, *lo_o, *hi_o, *rl_o, *ru_o;
-    double *val, *valt, *c, *l, *u, *rl, *ru, *row_scale, *col_scale, *cmax, *sig, *tau;
-    float *valf, *valtf, *cf, *tauf, *sigf, *rlf, *ruf;
-    double* scal;  // [os, cs, cnorm, bnorm, sigma_sq, power_iters]
-    // power iteration scratch (n, m, n)
-    double *px, *py, *pxn;
-    double offset;
-    int32_t sv_max_iter;
-    double sv_tol;
-};
-
-__global__ __launch_bounds__(kBlock) void kOwnerScale(const BOwner* __restrict__ owners) {
-    const BOwner w = owners[blockIdx.x];
-    const int tid = threadIdx.x;
-    const double sgn = w.sense_max ? -1.0 : 1.0;
-    for (long long k = tid; k < w.nnz; k += kBlock) w.val[k] = w.val_o[k];
-    for (int i = tid; i < w.m; i += kBlock) w.row_scale[i] = 1.0;
-    for (int j = tid; j < w.n; j += kBlock) w.col_scale[j] = 1.0;
-    __syncthreads();
-    if (w.do_ruiz && w.m > 0 && w.n > 0 && w.nnz > 0) {
-        for (int it = 0; it < w.ruiz_iters; ++it) {
-            for (int i = tid; i < w.m; i += kBlock) {
-                double rmax = 0.0;
-                for (int k = w.rs[i]; k < w.rs[i + 1]; ++k) rmax = maxRef(rmax, fabs(w.val[k]));
-                if (rmax <= 1e-12) continue;
-                const double f = 1.0 / sqrt(rmax);
-                w.row_scale[i] *= f;
-                for (int k = w.rs[i]; k < w.rs[i + 1]; ++k) w.val[k] *= f;
-            }
-            __syncthreads();
-            for (int j = tid; j < w.n; j += kBlock) {
-                double mx = 0.0;
-                for (int q = w.cs[j]; q < w.cs[j + 1]; ++q) mx = maxRef(mx, fabs(w.val[w.perm[q]]));
-                w.cmax[j] = mx;
-                if (mx > 1e-12) w.col_scale[j] *= 1.0 / sqrt(mx);
-            }
-            __syncthreads();
-            for (long long k = tid; k < w.nnz; k += kBlock) {
-                const double mx = w.cmax[w.ci[k]];
-                if (mx > 1e-12) w.val[k] *= 1.0 / sqrt(mx);
-            }
-            __syncthreads();
-        }
-    }
-    for (int j = tid; j < w.n; j += kBlock) {
-        const double cj = sgn * w.obj_o[j];
-        w.c[j] = cj * w.col_scale[j];
-        const double lj = w.lo_o[j], uj = w.hi_o[j];
-        w.l[j] = isfinite(lj) ? lj / w.col_scale[j] : lj;
-        w.u[j] = isfinite(uj) ? uj / w.col_scale[j] : uj;
-    }
-    for (int i = tid; i < w.m; i += kBlock) {
-        const double a = w.rl_o[i], b = w.ru_o[i];
-        w.rl[i] = isfinite(a) ? a * w.row_scale[i] : a;
-        w.ru[i] = isfinite(b) ? b * w.row_scale[i] : b;
-    }
-    __syncthreads();
-    for (long long k = tid; k < w.nnz; k += kBlock) w.valt[k] = w.val[w.perm[k]];
-    for (int i = tid; i < w.m; i += kBlock) w.sig[i] = 1.0;
-    for (int j = tid; j < w.n; j += kBlock) w.tau[j] = 1.0;
-    __syncthreads();
-    if (w.do_pc && w.m > 0 && w.n > 0) {
-        for (int i = tid; i < w.m; i += kBlock) {
-            double s = 0.0;
-            for (int k = w.rs[i]; k < w.rs[i + 1]; ++k) s += w.val[k] * w.val[k];
-            w.sig[i] = 1.0 / maxRef(sqrt(s), 1e-8);
-        }
-        for (int j = tid; j < w.n; j += kBlock) {
-            double s = 0.0;
-            for (int q = w.cs[j]; q < w.cs[j + 1]; ++q) s += w.valt[q] * w.valt[q];
-            w.tau[j] = 1.0 / maxRef(sqrt(s), 1e-8);
-        }
-    }
-    __syncthreads();
-    double os = 1.0, cs = 1.0;
-    if (w.do_rescale && w.m > 0 && w.n > 0) {
-        double cq = 0.0, bq = 0.0;
-        for (int j = tid; j < w.n; j += kBlock) cq += w.c[j] * w.c[j];
-        for (int i = tid; i < w.m; i += kBlock) {
-            if (isfinite(w.rl[i])) bq += w.rl[i] * w.rl[i];
-            if (isfinite(w.ru[i])) bq += w.ru[i] * w.ru[i];
-        }
-        cq = blockSum1(cq);
-        bq = blockSum1(bq);
-        os = 1.0 / (sqrt(cq) + 1.0);
-        cs = 1.0 / (sqrt(bq) + 1.0);
-        for (int j = tid; j < w.n; j += kBlock) {
-            w.c[j] *= os;
-            if (isfinite(w.l[j])) w.l[j] *= cs;
-            if (isfinite(w.u[j])) w.u[j] *= cs;
-        }
-        for (int i = tid; i < w.m; i += kBlock) {
-            if (isfinite(w.rl[i])) w.rl[i] *= cs;
-            if (isfinite(w.ru[i])) w.ru[i] *= cs;
-        }
-    }
-    // Termination norms of the original problem (pdlp.cpp:53-81).
-    double cn = 0.0, bn = 0.0;
-    if (w.norm_inf) {
-        for (int j = tid; j < w.n; j += kBlock) cn = maxRef(cn, fabs(w.obj_o[j]));
-        for (int i = tid; i < w.m; i += kBlock) {
-            const double lo = w.rl_o[i], hi = w.ru_o[i];
-            if (isfinite(hi)) bn = maxRef(bn, fabs(hi));
-            if (isfinite(lo) && lo != hi) bn = maxRef(bn, fabs(lo));
-        }
-        double a[2] = {cn, bn}, o2[2];
-        const bool mx[2] = {true, true};
-        blockReduce<2>(a, mx, o2);
-        __shared__ double sb[2];
-        if (tid == 0) {
-            sb[0] = o2[0];
-            sb[1] = o2[1];
-        }
-        __syncthreads();
-        cn = sb[0];
-        bn = sb[1];
-    } else {
-        for (int j = tid; j < w.n; j += kBlock) cn += w.obj_o[j] * w.obj_o[j];
-        for (int i = tid; i < w.m; i += kBlock) {
-            const double lo = w.rl_o[i], hi = w.ru_o[i];
-            if (isfinite(hi)) bn += hi * hi;
-            if (isfinite(lo) && lo != hi) bn += lo * lo;
-        }
-        cn = sqrt(blockSum1(cn));
-        bn = sqrt(blockSum1(bn));
-    }
-    __syncthreads();
-    // FP32 copies of the iteration data.
-    for (long long k = tid; k < w.nnz; k += kBlock) {
-        w.valf[k] = static_cast<float>(w.val[k]);
-        w.valtf[k] = static_cast<float>(w.valt[k]);
-    }
-    for (int j = tid; j < w.n; j += kBlock) {
-        w.cf[j] = static_cast<float>(w.c[j]);
-        w.tauf[j] = static_cast<float>(w.tau[j]);
-    }
-    for (int i = tid; i < w.m; i += kBlock) {
-        w.sigf[i] = static_cast<float>(w.sig[i]);
-        w.rlf[i] = static_cast<float>(w.rl[i]);
-        w.ruf[i] = static_cast<float>(w.ru[i]);
-    }
-    if (tid == 0) {
-        w.scal[0] = os;
-        w.scal[1] = cs;
-        w.scal[2] = cn;
-        w.scal[3] = bn;
-    }
-}
-
-// Power iteration of one owner (pdlp.cpp:748-795): px holds the normalized
-// start vector on entry.
-template <int GC, int GR>
-__device__ void ownerPower(const BOwner& w) {
-    const int tid = threadIdx.x;
-    double s2 = 1.0;
-    int it = 0;
-    for (; it < w.sv_max_iter; ++it) {
-        for (int base = 0; base < w.m; base += kBlock / GR) {
-            const int i = base + tid / GR, lane = tid % GR;
-            if (i < w.m) {
-                const double a = bDot<GR>(w.rs, w.ci, w.val, w.px, i, lane);
-                if (lane == 0) w.py[i] = a;
-            }
-        }
-        __syncthreads();
-        double dot = 0.0;
-        for (int base = 0; base < w.n; base += kBlock / GC) {
-            const int j = base + tid / GC, lane = tid % GC;
-            if (j < w.n) {
-                const double a = bDot<GC>(w.cs, w.ri, w.valt, w.py, j, lane);
-                if (lane == 0) {
-                    w.pxn[j] = a;
-                    dot += w.px[j] * a;
-                }
-            }
-        }
-        s2 = blockSum1(dot);
-        double r2 = 0.0, xq = 0.0;
-        for (int j = tid; j < w.n; j += kBlock) {
-            const double d = w.pxn[j] - s2 * w.px[j];
-            r2 += d * d;
-            xq += w.pxn[j] * w.pxn[j];
-        }
-        const double res = sqrt(blockSum1(r2));
-        const double xn = sqrt(blockSum1(xq));
-        if (xn < 1e-15) break;
-        for (int j = tid; j < w.n; j += kBlock) w.px[j] = w.pxn[j] / xn;
-        __syncthreads();
-        if (res < w.sv_tol * fabs(s2)) break;
-    }
-    if (tid == 0) {
-        w.scal[4] = s2;
-        w.scal[5] = static_cast<double>(it < w.sv_max_iter ? it + 1 : w.sv_max_iter);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void kOwnerPower(const BOwner* __restrict__ owners,
-                                                      const int* __restrict__ presets) {
-    const BOwner w = owners[blockIdx.x];
-    if (w.m == 0 || w.n == 0) {
-        if (threadIdx.x == 0) {
-            w.scal[4] = 1.0;
-            w.scal[5] = 0.0;
-        }
-        return;
-    }
-    switch (presets[blockIdx.x]) {
-        case 0: ownerPower<1, 1>(w); break;
-        case 1: ownerPower<2, 2>(w); break;
-        case 2: ownerPower<2, 4>(w); break;
-        case 3: ownerPower<4, 4>(w); break;
-        case 4: ownerPower<4, 8>(w); break;
-        case 5: ownerPower<8, 8>(w); break;
-        case 6: ownerPower<8, 16>(w); break;
-        default: ownerPower<16, 16>(w); break;
-    }
-}
-
 // Per-instance initialisation: scaled column bounds (its own bounds when the
 // instance overrides them), FP32 copies, zero iterates, control state.
 struct BInitArgs {
@@ -942,7 +716,7 @@ size_t ownerBytes(int m, int n, int nnz) {
     return pad(M + 1, 4) + 3 * pad(Z, 4) + pad(N + 1, 4) + pad(Z, 4)  // rs ci ri perm cs
            + 3 * pad(Z, 8)                                          // val_o val valt
            + 6 * pad(N, 8) + 6 * pad(M, 8)                          // obj lo hi c l u cmax, rl ru rl_o ru_o rs sig ..
-           + 3 * pad(N, 8) + 2 * pad(M, 8)                          // col_scale tau + power px pxn, py
+           + 4 * pad(N, 8) + 2 * pad(M, 8)                          // col_scale tau + power px0 px pxn, py
            + 2 * pad(Z, 4) + 4 * pad(N, 4) + 4 * pad(M, 4)          // fp32 copies
            + pad(8, 8) + 4096;
 }
@@ -1026,6 +800,7 @@ void carveOwner(const mpx_lp& lp, const OwnerHost& h, const mpx_pdlp_options& o,
     b.cmax = cv.take<double>(n);
     b.col_scale = cv.take<double>(n);
     b.tau = cv.take<double>(n);
+    b.px0 = cv.take<double>(n);
     b.px = cv.take<double>(n);
     b.pxn = cv.take<double>(n);
     double* rl_o = cv.take<double>(m);
@@ -1058,7 +833,7 @@ void carveOwner(const mpx_lp& lp, const OwnerHost& h, const mpx_pdlp_options& o,
         MPX_HIP(hipMemcpyAsync(rl_o, lp.row_lower, m * 8ull, hipMemcpyHostToDevice, st));
         MPX_HIP(hipMemcpyAsync(ru_o, lp.row_upper, m * 8ull, hipMemcpyHostToDevice, st));
     }
-    MPX_HIP(hipMemcpyAsync(b.px, h.start.data(), n * 8ull, hipMemcpyHostToDevice, st));
+    MPX_HIP(hipMemcpyAsync(b.px0, h.start.data(), n * 8ull, hipMemcpyHostToDevice, st));
     b.rs = rs;
     b.ci = ci;
     b.cs = cs;
@@ -1363,6 +1138,8 @@ public:
     double setupSeconds() const { return setup_seconds_; }
     // Shared-matrix batches may use the lane-per-instance kernels.
     void allowInterleaved(bool on) { allow_interleaved_ = on; }
+    // Copies owner `w`'s scaled problem out (after setup()).
+    void downloadOwner(int w, mpx_scaled_problem* out);
     // Runs `rounds` rounds; returns the number of unfinished instances.
     int step(int rounds);
     // Statistics of the iteration kernels (HIP events around every round's
@@ -1398,7 +1175,9 @@ private:
     std::vector<BOwner> owners_;
     std::vector<BInst> insts_;
     std::vector<int> owner_preset_;
-    std::unique_ptr<DevBuf> d_insts_buf_, d_owners_buf_, d_presets_buf_, d_args_buf_, d_count_buf_;
+    std::unique_ptr<DevBuf> d_insts_buf_, d_args_buf_, d_count_buf_;
+    std::unique_ptr<SetupCohort> cohort_;  // device setup of every owner
+    std::unique_ptr<DevBuf> d_cohort_buf_;
     std::unique_ptr<DevBuf> d_tiles_buf_, d_cnt_buf_;
     Tile *tc32_ = nullptr, *tr32_ = nullptr, *tc64_ = nullptr, *tr64_ = nullptr;
     int* totals_ = nullptr;  // [0..3] tile counts, [4] unfinished instances
@@ -1502,16 +1281,13 @@ void BatchEngine::setup() {
     } else {
         cur_ = {0, 0};
     }
-    d_owners_buf_ = std::make_unique<DevBuf>(sizeof(BOwner) * std::max(1, no));
-    d_presets_buf_ = std::make_unique<DevBuf>(sizeof(int) * std::max(1, no));
+    cohort_ = std::make_unique<SetupCohort>(owners_, owner_preset_);
+    d_cohort_buf_ = std::make_unique<DevBuf>(cohort_->bytes());
+    cohort_->carve(static_cast<char*>(d_cohort_buf_->p));
     d_insts_buf_ = std::make_unique<DevBuf>(sizeof(BInst) * std::max(1, ni));
     d_args_buf_ = std::make_unique<DevBuf>(sizeof(BInitArgs) * std::max(1, ni));
     d_count_buf_ = std::make_unique<DevBuf>(sizeof(int) * 4 + 64);
     stamp_ = reinterpret_cast<unsigned long long*>(static_cast<char*>(d_count_buf_->p) + 32);
-    MPX_HIP(hipMemcpyAsync(d_owners_buf_->p, owners_.data(), sizeof(BOwner) * no,
-                           hipMemcpyHostToDevice, st_));
-    MPX_HIP(hipMemcpyAsync(d_presets_buf_->p, owner_preset_.data(), sizeof(int) * no,
-                           hipMemcpyHostToDevice, st_));
     MPX_HIP(hipMemcpyAsync(d_insts_buf_->p, insts_.data(), sizeof(BInst) * ni,
                            hipMemcpyHostToDevice, st_));
     MPX_HIP(hipMemcpyAsync(d_args_buf_->p, args.data(), sizeof(BInitArgs) * ni,
@@ -1531,11 +1307,7 @@ void BatchEngine::prepare() {
     graphs_.clear();
     rounds_ = 0;
     fp32_gone_ = false;
-    if (no) {
-        kOwnerScale<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p));
-        kOwnerPower<<<no, kBlock, 0, st_>>>(static_cast<const BOwner*>(d_owners_buf_->p),
-                                            static_cast<const int*>(d_presets_buf_->p));
-    }
+    if (no) cohort_->run(st_);
     if (ni)
         kInstInit<<<ni, kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
                                           static_cast<const BInitArgs*>(d_args_buf_->p), bo_,
@@ -1544,6 +1316,35 @@ void BatchEngine::prepare() {
     MPX_HIP(hipGetLastError());
     MPX_HIP(hipStreamSynchronize(st_));
     setup_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void BatchEngine::downloadOwner(int wi, mpx_scaled_problem* out) {
+    const BOwner& w = owners_[wi];
+    const size_t m = static_cast<size_t>(w.m), n = static_cast<size_t>(w.n),
+                 z = static_cast<size_t>(w.nnz);
+    auto get = [&](void* dst, const void* src, size_t bytes) {
+        if (dst && bytes) MPX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st_));
+    };
+    get(out->values, w.val, z * 8);
+    get(out->at_values, w.valt, z * 8);
+    get(out->at_row_starts, w.cs, (n + 1) * 4);
+    get(out->at_col_indices, w.ri, z * 4);
+    get(out->cscaled, w.c, n * 8);
+    get(out->col_lower, w.l, n * 8);
+    get(out->col_upper, w.u, n * 8);
+    get(out->row_lower, w.rl, m * 8);
+    get(out->row_upper, w.ru, m * 8);
+    get(out->row_scale, w.row_scale, m * 8);
+    get(out->col_scale, w.col_scale, n * 8);
+    get(out->sigma_base, w.sig, m * 8);
+    get(out->tau_base, w.tau, n * 8);
+    double scal[8];
+    MPX_HIP(hipMemcpyAsync(scal, w.scal, sizeof(scal), hipMemcpyDeviceToHost, st_));
+    MPX_HIP(hipStreamSynchronize(st_));
+    out->objective_scale = scal[0];
+    out->constraint_scale = scal[1];
+    out->spectral_norm = std::sqrt(std::max(scal[4], 1e-12));
+    out->power_iterations = static_cast<int32_t>(scal[5]);
 }
 
 std::vector<IterPreset> BatchEngine::candidates() const {
@@ -2003,8 +1804,6 @@ struct ServeLp {
     std::vector<double> val, obj, lo, hi, rl, ru;
     mpx_lp lp{};
     BOwner owner{};
-    BOwner* d_owner = nullptr;
-    int* d_preset = nullptr;
     int preset = 0;
 
     explicit ServeLp(const mpx_lp& src) {
@@ -2028,26 +1827,25 @@ struct ServeLp {
         lp.row_upper = ru.data();
     }
     size_t bytes() const {
-        return ownerBytes(lp.num_rows, lp.num_cols, lp.row_starts[lp.num_rows]) +
-               pad(1, sizeof(BOwner)) + pad(1, sizeof(int));
+        return ownerBytes(lp.num_rows, lp.num_cols, lp.row_starts[lp.num_rows]);
     }
     // Carves the matrix arrays and queues their uploads on `st`.
     void upload(const mpx_pdlp_options& o, const OwnerHost& h, Carve& cv, hipStream_t st) {
         carveOwner(lp, h, o, cv, st, owner);
         preset = ownerPreset(o, owner.nnz, owner.m, owner.n);
-        d_owner = cv.take<BOwner>(1);
-        d_preset = cv.take<int>(1);
-        MPX_HIP(hipMemcpyAsync(d_owner, &owner, sizeof(BOwner), hipMemcpyHostToDevice, st));
-        MPX_HIP(hipMemcpyAsync(d_preset, &preset, sizeof(int), hipMemcpyHostToDevice, st));
-    }
-    // Scaling and power iteration (one workgroup, like the batched engine).
-    void prepare(hipStream_t st) {
-        kOwnerScale<<<1, kBlock, 0, st>>>(d_owner);
-        kOwnerPower<<<1, kBlock, 0, st>>>(d_owner, d_preset);
-        MPX_HIP(hipGetLastError());
     }
     // Lane-per-instance state of a base's variants.
     IlSet il;
+};
+
+// Device setup of the fresh LPs admitted in one step, run by the server's
+// setup thread on its own stream.
+struct SetupJob {
+    std::unique_ptr<SetupCohort> cohort;
+    void* mem = nullptr;             // cohort scratch (pool)
+    hipEvent_t uploaded = nullptr;   // the cohort's uploads on the main stream
+    std::atomic<bool> done{false};
+    std::atomic<bool> failed{false};
 };
 
 struct ServeReq {
@@ -2061,7 +1859,7 @@ struct ServeReq {
     BInst inst{};
     BInitArgs args{};
     double *xo = nullptr, *yo = nullptr;
-    hipEvent_t ready = nullptr;
+    std::shared_ptr<SetupJob> job;  // fresh LPs until their setup is done
     int slot = -1;
 };
 
@@ -2086,19 +1884,12 @@ struct PinnedArray {
 
 class Server {
 public:
-    // Side streams for the setup of fresh LPs (with the main stream this is
-    // the hardware queue count); matrices above kBigSetupNnz entries use the
-    // last one.
-    static constexpr int kSetupStreams = 3;
-    static constexpr int kBigSetupNnz = 200000;
 
     Server(const mpx_pdlp_options& o, double budget, int slots)
         : o_(o), S_(slots > 0 ? slots : 1024), h_ctrl_(S_), h_new_(S_), h_args_(S_),
           h_slots_(S_), h_ptrs_(2 * static_cast<size_t>(S_)) {
         MPX_HIP(hipSetDevice(o.device));
         MPX_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-        for (auto& s : setup_st_) MPX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        MPX_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming));
         bo_ = makeBOptions(o);
         mixed_ = o.precision == MPX_PREC_MIXED;
         size_t freeb = 0, total = 0;
@@ -2130,17 +1921,24 @@ public:
         MPX_HIP(hipGetDevice(&dev));
         MPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         grid_ = std::max(kNumXcd, (cus * 8 / kNumXcd) * kNumXcd);
-        growTiles(1 << 14, 1 << 14);
+        // Initial tile capacity per list (8 MB of tiles for pools of 1 GB and
+        // more); it grows on demand.
+        const size_t t0 = pool_->capacity() >= (size_t(1) << 30) ? size_t(1) << 20 : 1024;
+        growTiles(t0, t0);
+        setup_thread_ = std::thread([this] { setupLoop(); });
     }
 
     ~Server() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        job_cv_.notify_all();
+        if (setup_thread_.joinable()) setup_thread_.join();
         (void)hipStreamSynchronize(st_);
-        for (auto& s : setup_st_) (void)hipStreamSynchronize(s);
-        for (auto& r : preparing_)
-            if (r->ready) (void)hipEventDestroy(r->ready);
+        for (auto& j : jobs_live_)
+            if (j->uploaded) (void)hipEventDestroy(j->uploaded);
         for (auto& kv : graphs_) kv.second.reset();
-        for (auto& s : setup_st_) (void)hipStreamDestroy(s);
-        (void)hipEventDestroy(uploaded_);
         (void)hipStreamDestroy(st_);
     }
 
@@ -2156,10 +1954,18 @@ public:
         }
         Carve cv{static_cast<char*>(mem), 0, need};
         b->upload(o_, h, cv, st_);
-        b->prepare(st_);
         b->il.carve(cv, S_, b->lp.num_cols, b->lp.num_rows, mixed_);
         b->il.owner = static_cast<int>(bases_.size());
-        MPX_HIP(hipStreamSynchronize(st_));
+        SetupCohort cohort({b->owner}, {b->preset});
+        void* scratch = pool_->alloc(cohort.bytes());
+        if (!scratch) {
+            pool_->release(mem);
+            setError("mpx_server_add_base: the base LP does not fit the server's memory pool");
+            throw OutOfMemory();
+        }
+        cohort.carve(static_cast<char*>(scratch));
+        cohort.run(st_);
+        pool_->release(scratch);
         bases_.push_back(std::move(b));
         // The round graphs include every base's lane-per-instance kernels.
         for (auto& kv : graphs_) kv.second.reset();
@@ -2185,11 +1991,24 @@ public:
     }
 
     int step() {
+        using clk = std::chrono::steady_clock;
+        auto ms = [](clk::time_point a, clk::time_point b) {
+            return 1000.0 * std::chrono::duration<double>(b - a).count();
+        };
+        const auto t0 = clk::now();
         admit();
+        const auto t1 = clk::now();
         activate();
+        const auto t2 = clk::now();
         iterate();
+        const auto t3 = clk::now();
         retire();
         enforceTimeLimit();
+        const auto t4 = clk::now();
+        admit_ms_ += ms(t0, t1);
+        activate_ms_ += ms(t1, t2);
+        retire_ms_ += ms(t3, t4);
+        max_step_ms_ = std::max(max_step_ms_, ms(t0, t4));
         return static_cast<int>(queued_.size() + preparing_.size()) + running_;
     }
 
@@ -2229,6 +2048,12 @@ public:
         s->pool_peak_bytes = static_cast<double>(pool_->peak());
         s->iterate_ms = iterate_ms_;
         s->instance_iterations = inst_iters_;
+        s->admit_ms = admit_ms_;
+        s->activate_ms = activate_ms_;
+        s->retire_ms = retire_ms_;
+        s->max_step_ms = max_step_ms_;
+        s->setup_query_ms = query_ms_;
+        s->setup_wait_ms = wait_ms_;
     }
 
 private:
@@ -2316,44 +2141,117 @@ private:
             preparing_.push_back(std::move(q));
         }
         if (fresh.empty()) return;
-        MPX_HIP(hipEventRecord(uploaded_, st_));
+        std::vector<BOwner> owners;
+        std::vector<int> presets;
         for (ServeReq* q : fresh) {
-            // Large matrices get the last side stream so that they do not
-            // hold up the small ones behind them.
-            const bool big = q->inst.nnz > kBigSetupNnz;
-            hipStream_t st = big ? setup_st_[kSetupStreams - 1]
-                                 : setup_st_[next_setup_++ % (kSetupStreams - 1)];
-            MPX_HIP(hipStreamWaitEvent(st, uploaded_, 0));
-            q->own->prepare(st);
-            MPX_HIP(hipEventCreateWithFlags(&q->ready, hipEventDisableTiming));
-            MPX_HIP(hipEventRecord(q->ready, st));
+            owners.push_back(q->own->owner);
+            presets.push_back(q->own->preset);
         }
+        auto job = std::make_shared<SetupJob>();
+        job->cohort = std::make_unique<SetupCohort>(std::move(owners), std::move(presets));
+        job->mem = pool_->alloc(job->cohort->bytes());
+        if (!job->mem) {
+            // No room for the setup scratch: put the fresh requests back at
+            // the head of the queue, in order.
+            MPX_HIP(hipStreamSynchronize(st_));
+            for (auto it = preparing_.end(); it != preparing_.begin();) {
+                --it;
+                if (!(*it)->own || (*it)->job) continue;
+                if (std::find(fresh.begin(), fresh.end(), it->get()) == fresh.end()) continue;
+                std::unique_ptr<ServeReq> q = std::move(*it);
+                it = preparing_.erase(it);
+                pool_->release(q->mem);
+                q->mem = nullptr;
+                free_slots_.push_back(q->slot);
+                q->slot = -1;
+                queued_.push_front(std::move(q));
+            }
+            return;
+        }
+        job->cohort->carve(static_cast<char*>(job->mem));
+        MPX_HIP(hipEventCreateWithFlags(&job->uploaded, hipEventDisableTiming));
+        MPX_HIP(hipEventRecord(job->uploaded, st_));
+        for (ServeReq* q : fresh) q->job = job;
+        jobs_live_.push_back(job);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_.push_back(job);
+        }
+        job_cv_.notify_one();
+    }
+
+    // Setup thread: runs each job's cohort on its own stream.
+    void setupLoop() {
+        hipStream_t ws = nullptr;
+        bool ok = hipSetDevice(o_.device) == hipSuccess &&
+                  hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) == hipSuccess;
+        while (true) {
+            std::shared_ptr<SetupJob> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                job_cv_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+                if (jobs_.empty()) break;  // stop_ and nothing left
+                job = jobs_.front();
+                jobs_.pop_front();
+            }
+            bool good = ok;
+            if (good) {
+                try {
+                    MPX_HIP(hipStreamWaitEvent(ws, job->uploaded, 0));
+                    job->cohort->run(ws);
+                } catch (const DeviceError&) {
+                    good = false;
+                }
+            }
+            job->failed.store(!good);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                job->done.store(true);
+            }
+            done_cv_.notify_all();
+        }
+        if (ws) (void)hipStreamDestroy(ws);
     }
 
     // Preparing -> running for every request whose setup has completed:
     // table rows written and instances initialised on the main stream.
     void activate() {
-        if (preparing_.empty()) return;
+        if (preparing_.empty()) {
+            reapJobs();
+            return;
+        }
         if (running_ == 0) {
             // Nothing to iterate meanwhile: wait for the first setup.
             bool any = false;
-            for (auto& q : preparing_) any = any || q->ready == nullptr;
-            if (!any) MPX_HIP(hipEventSynchronize(preparing_.front()->ready));
+            for (auto& q : preparing_) any = any || !q->job || q->job->done.load();
+            if (!any) {
+                const auto tq = std::chrono::steady_clock::now();
+                std::shared_ptr<SetupJob> j = preparing_.front()->job;
+                std::unique_lock<std::mutex> lk(mu_);
+                done_cv_.wait(lk, [&] { return j->done.load(); });
+                wait_ms_ += 1000.0 * std::chrono::duration<double>(
+                                         std::chrono::steady_clock::now() - tq).count();
+            }
         }
         int k = 0;
         size_t need_c = 0, need_r = 0;
         std::vector<std::unique_ptr<ServeReq>> still;
         const auto now = std::chrono::steady_clock::now();
         for (auto& q : preparing_) {
-            if (q->ready) {
-                const hipError_t e = hipEventQuery(q->ready);
-                if (e == hipErrorNotReady) {
+            if (q->job) {
+                if (!q->job->done.load()) {
                     still.push_back(std::move(q));
                     continue;
                 }
-                MPX_HIP(e);
-                (void)hipEventDestroy(q->ready);
-                q->ready = nullptr;
+                const bool failed = q->job->failed.load();
+                q->job.reset();
+                if (failed) {
+                    setError("serving loop: device setup of a fresh LP failed");
+                    pool_->release(q->mem);
+                    free_slots_.push_back(q->slot);
+                    finishNow(std::move(q), MPX_STATUS_ERROR);
+                    continue;
+                }
             }
             // The pageable uploads have been consumed: drop the host copies.
             q->oh = OwnerHost();
@@ -2371,6 +2269,7 @@ private:
             ++running_;
         }
         preparing_ = std::move(still);
+        reapJobs();
         if (k == 0) return;
         tiles_c_ += need_c;
         tiles_r_ += need_r;
@@ -2390,19 +2289,45 @@ private:
         fp32_gone_ = false;
     }
 
+    // Setup jobs that are done and no longer referenced by a request: their
+    // scratch goes back to the pool.
+    void reapJobs() {
+        for (auto it = jobs_live_.begin(); it != jobs_live_.end();) {
+            if ((*it)->done.load() && it->use_count() == 1) {
+                pool_->release((*it)->mem);
+                (void)hipEventDestroy((*it)->uploaded);
+                it = jobs_live_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+
+    // Tile lists of the packed kernels, carved from the pool: no hipMalloc /
+    // hipFree while serving (hipFree synchronises the whole device, i.e. it
+    // would wait for every setup in flight on the side streams).
     void growTiles(size_t c, size_t r) {
         MPX_HIP(hipStreamSynchronize(st_));
         for (auto& kv : graphs_) kv.second.reset();
         graphs_.clear();
-        cap_c_ = std::max(c, cap_c_);
-        cap_r_ = std::max(r, cap_r_);
-        tiles_ = std::make_unique<DevBuf>(2 * (cap_c_ + cap_r_) * sizeof(Tile) + 16 * sizeof(int));
-        Tile* t = static_cast<Tile*>(tiles_->p);
-        tc32_ = t;
-        tr32_ = tc32_ + cap_c_;
-        tc64_ = tr32_ + cap_r_;
-        tr64_ = tc64_ + cap_c_;
-        totals_ = reinterpret_cast<int*>(tr64_ + cap_r_);
+        const size_t nc = std::max(c, cap_c_), nr = std::max(r, cap_r_);
+        const size_t bytes =
+            2 * pad(nc, sizeof(Tile)) + 2 * pad(nr, sizeof(Tile)) + pad(16, sizeof(int));
+        void* mem = pool_->alloc(bytes);
+        if (!mem) {
+            setError("serving loop: memory pool too small for the tile lists");
+            throw DeviceError();
+        }
+        pool_->release(tiles_mem_);
+        tiles_mem_ = mem;
+        cap_c_ = nc;
+        cap_r_ = nr;
+        Carve cv{static_cast<char*>(mem), 0, bytes};
+        tc32_ = cv.take<Tile>(cap_c_);
+        tr32_ = cv.take<Tile>(cap_r_);
+        tc64_ = cv.take<Tile>(cap_c_);
+        tr64_ = cv.take<Tile>(cap_r_);
+        totals_ = cv.take<int>(16);
     }
 
     void enqueueRound() {
@@ -2574,16 +2499,21 @@ private:
     bool mixed_ = false;
     int S_;
     hipStream_t st_ = nullptr;
-    hipStream_t setup_st_[kSetupStreams] = {};
-    int next_setup_ = 0;
     std::unique_ptr<DevicePool> pool_;
-    std::unique_ptr<DevBuf> tab_, ctrl_, cnt_, newi_, newa_, news_, ptrs_, tiles_;
+    std::unique_ptr<DevBuf> tab_, ctrl_, cnt_, newi_, newa_, news_, ptrs_;
+    void* tiles_mem_ = nullptr;
     PinnedArray<BCtrl> h_ctrl_;
     PinnedArray<BInst> h_new_;
     PinnedArray<BInitArgs> h_args_;
     PinnedArray<int> h_slots_;
     PinnedArray<double*> h_ptrs_;
-    hipEvent_t uploaded_ = nullptr;
+    // Setup thread and its job queue.
+    std::thread setup_thread_;
+    std::mutex mu_;
+    std::condition_variable job_cv_, done_cv_;
+    std::deque<std::shared_ptr<SetupJob>> jobs_;
+    std::vector<std::shared_ptr<SetupJob>> jobs_live_;
+    bool stop_ = false;
     Tile *tc32_ = nullptr, *tr32_ = nullptr, *tc64_ = nullptr, *tr64_ = nullptr;
     int* totals_ = nullptr;
     size_t cap_c_ = 0, cap_r_ = 0, tiles_c_ = 0, tiles_r_ = 0;
@@ -2600,7 +2530,8 @@ private:
     std::map<int64_t, ServeDone> done_;
     std::deque<int64_t> newly_done_;
     int64_t next_ticket_ = 0, finished_ = 0, rounds_ = 0, inst_iters_ = 0;
-    double iterate_ms_ = 0.0;
+    double iterate_ms_ = 0.0, admit_ms_ = 0.0, activate_ms_ = 0.0, retire_ms_ = 0.0;
+    double max_step_ms_ = 0.0, query_ms_ = 0.0, wait_ms_ = 0.0;
 
 public:
     using OOM = OutOfMemory;
@@ -2727,6 +2658,24 @@ double pct(std::vector<double> v, double q) {
 }  // namespace
 
 extern "C" {
+
+int mpx_pdlp_scale_problem_batched(const mpx_lp* lp, const mpx_pdlp_options* opts,
+                                   mpx_scaled_problem* out) {
+    if (!validLpB(lp) || !out) {
+        setError("mpx_pdlp_scale_problem_batched: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    if (!batchOptionsOk(opts)) return MPX_ERR_UNSUPPORTED;
+    try {
+        BatchEngine eng(*opts);
+        eng.addInstance(eng.addOwner(*lp), nullptr, nullptr);
+        eng.setup();
+        eng.downloadOwner(0, out);
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}
 
 int mpx_batch_create_shared(const mpx_lp* base, int32_t count, const double* col_lower,
                             const double* col_upper, const mpx_pdlp_options* opts,

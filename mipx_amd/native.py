@@ -20,7 +20,7 @@ EXPORTED = [
     "mpx_server_create", "mpx_server_add_base", "mpx_server_submit", "mpx_server_submit_bounds",
     "mpx_server_step", "mpx_server_result", "mpx_server_poll", "mpx_server_get_stats",
     "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
-    "mpx_pdhg_iterate", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
+    "mpx_pdhg_iterate", "mpx_pdlp_scale_problem_batched", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
     "mpx_batch_results", "mpx_batch_get_stats", "mpx_batch_profile", "mpx_batch_destroy",
 ]
 
@@ -60,6 +60,8 @@ def lib():
                                                                             C.c_void_p]
     L.mpx_pdlp_scale_problem.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions),
                                          C.POINTER(MpxScaledProblem)]
+    L.mpx_pdlp_scale_problem_batched.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions),
+                                                 C.POINTER(MpxScaledProblem)]
     L.mpx_pdhg_iterate.argtypes = ([C.POINTER(MpxLp), C.POINTER(MpxScaledProblem), C.c_int32,
                                     C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32]
                                    + [C.c_void_p] * 6)
@@ -148,12 +150,14 @@ def solve_batch(lps, **opts):
     return [r.as_dict() for r in res], xs, ys
 
 
-def scale_problem(lp, **opts):
+def scale_problem(lp, batched=False, **opts):
+    """Scaled problem from the single-LP setup, or (batched=True) from the
+    batched engines' cohort setup."""
     o = make_options(**opts)
     sp, arrs = alloc_scaled(lp.num_rows, lp.num_cols, lp.nnz)
     clp = lp.as_c()
-    _check(lib().mpx_pdlp_scale_problem(C.byref(clp), C.byref(o), C.byref(sp)),
-           "mpx_pdlp_scale_problem")
+    f = lib().mpx_pdlp_scale_problem_batched if batched else lib().mpx_pdlp_scale_problem
+    _check(f(C.byref(clp), C.byref(o), C.byref(sp)), "mpx_pdlp_scale_problem")
     arrs["objective_scale"] = sp.objective_scale
     arrs["constraint_scale"] = sp.constraint_scale
     arrs["spectral_norm"] = sp.spectral_norm

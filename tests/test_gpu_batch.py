@@ -79,3 +79,30 @@ def test_batch_detects_infeasible_and_unbounded():
     res, xs, ys = native.solve_batch(lps, verbose=0, max_iter=100000)
     assert [r["status"] for r in res] == [1, 2, 0]
     assert abs(res[2]["objective"] + 4.0) <= 1e-4
+
+
+@pytest.mark.parametrize("which", ["c1_small", "c2_small"])
+def test_batched_setup_vs_oracle(which):
+    """The batched engines' cohort setup: element-wise scaling bit-identical
+    to the reference's (oracle), global scales to 1e-14, the power
+    iteration's spectral norm to its tolerance, and repeatable."""
+    lp = (lpgen.c1_large(seed=3, m=3000, n=6000) if which == "c1_small"
+          else lpgen.c2_shared(seed=0xC2, m=300, n=800)).lp
+    got = native.scale_problem(lp, batched=True)
+    again = native.scale_problem(lp, batched=True)
+    want = ob.scale_problem(lp)
+    for k in ("values", "at_values", "at_row_starts", "at_col_indices", "row_scale",
+              "col_scale", "sigma_base", "tau_base"):
+        assert np.array_equal(got[k], want[k]), k
+    for k in ("objective_scale", "constraint_scale"):
+        assert abs(got[k] - want[k]) <= 1e-14 * abs(want[k]), (k, got[k], want[k])
+    for k in ("cscaled", "col_lower", "col_upper", "row_lower", "row_upper"):
+        np.testing.assert_allclose(got[k], want[k], rtol=1e-14, atol=0)
+    print("spectral norm", got["spectral_norm"], want["spectral_norm"], "iters",
+          got["power_iterations"], want["power_iterations"])
+    assert abs(got["spectral_norm"] - want["spectral_norm"]) <= 1e-6 * want["spectral_norm"]
+    for k, v in got.items():
+        if isinstance(v, np.ndarray):
+            assert np.array_equal(v, again[k]), k
+        else:
+            assert v == again[k], k
