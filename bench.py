@@ -57,6 +57,21 @@ def c2_workload(count):
     return g, lo, hi
 
 
+def c3_workload(count):
+    from mipx_amd import lpgen
+    sizes = lpgen.c3_sizes(count)
+    return [lpgen.c3_instance(i, m).lp for i, m in enumerate(sizes)]
+
+
+def packed_primal_bytes(lps, sv):
+    """Algorithmic bytes of one launch of the packed primal kernel with every
+    LP active: per LP x, x0, l, u read and x, x_r written, c and tau read
+    (8n values), y gathered once (m values), its A' in CSR (nnz values +
+    nnz indices + n+1 starts)."""
+    return sum((8 * lp.num_cols + lp.num_rows) * sv + lp.nnz * (sv + 4) + (lp.num_cols + 1) * 4
+               for lp in lps)
+
+
 def primal_kernel_bytes(lp, slots, sv):
     """Algorithmic bytes of one launch of the primal iteration kernel for
     `slots` instances sharing A: per instance x, x0, l, u read, x and x_r
@@ -215,12 +230,27 @@ def run_c4(a, prec):
     }
 
 
+def cpu_baseline_lps(lps, budget_s):
+    """As cpu_baseline, on the LPs of a heterogeneous batch in order."""
+    from oracle import binding as ob
+    kind = "reference" if ob.ref_available() else "port"
+    solve = ob.ref_solve if kind == "reference" else ob.solve
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < budget_s and done < len(lps):
+        solve(lps[done], **TOL)
+        done += 1
+    el = time.perf_counter() - t0
+    return dict(value=done / el, unit="LP solves/s", cores=1, kind=kind,
+                sample=f"the first {done} C3 LPs solved back to back on one host core in {el:.1f} s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--slots", type=int, default=2048)
@@ -248,14 +278,22 @@ def main():
         print(json.dumps(run_c4(a, prec)), flush=True)
         return
     log(f"building workload {a.workload} count={a.count}")
-    g, lo, hi = c2_workload(a.count)
-    lp = g.lp
-    log("uploading (resident batch)")
-    batch = native.ResidentBatch(lp, lo, hi, precision=prec, **TOL)
+    lps = None
+    if a.workload == "c3":
+        lps = c3_workload(a.count)
+        lp = lps[0]
+        log(f"c3: {len(lps)} LPs, sum nnz {sum(x.nnz for x in lps)}")
+        batch = native.ResidentBatch(lps=lps, precision=prec, **TOL)
+    else:
+        g, lo, hi = c2_workload(a.count)
+        lp = g.lp
+        log("uploading (resident batch)")
+        batch = native.ResidentBatch(lp, lo, hi, precision=prec, **TOL)
     if a.profile_only:
         prof = batch.profile(launches=a.launches)
         sv = 4 if prec == 1 else 8
-        bl = primal_kernel_bytes(lp, prof["profiled_slots"], sv)
+        bl = (packed_primal_bytes(lps, sv) if lps is not None
+              else primal_kernel_bytes(lp, prof["profiled_slots"], sv))
         print(json.dumps({"primal_kernel_us": prof["primal_kernel_us"],
                           "dual_kernel_us": prof["dual_kernel_us"],
                           "slots": prof["profiled_slots"], "launches": prof["profiled_launches"],
@@ -286,7 +324,8 @@ def main():
 
     prof = batch.profile(launches=50)
     sv = 4 if prec == 1 else 8
-    bytes_launch = primal_kernel_bytes(lp, prof["profiled_slots"], sv)
+    bytes_launch = (packed_primal_bytes(lps, sv) if lps is not None
+                    else primal_kernel_bytes(lp, prof["profiled_slots"], sv))
     achieved = bytes_launch / (prof["primal_kernel_us"] * 1e-6) / 1e9
     log(f"primal kernel {prof['primal_kernel_us']:.1f} us for {prof['profiled_slots']} instances "
         f"-> {achieved:.0f} GB/s")
@@ -295,15 +334,20 @@ def main():
     # Plain single-instance FP64 GPU PDLP on a sample of the same instances.
     t = time.perf_counter()
     plain_iters = []
-    for i in range(a.plain_sample):
-        r, _, _ = native.solve(lp.with_col_bounds(lo[i], hi[i]), **TOL)
+    # C3 sizes are log-uniform: sample across the batch, not its head.
+    sample = (np.linspace(0, len(lps) - 1, a.plain_sample).astype(int).tolist() if lps is not None
+              else list(range(a.plain_sample)))
+    for i in sample:
+        one = lps[i] if lps is not None else lp.with_col_bounds(lo[i], hi[i])
+        r, _, _ = native.solve(one, **TOL)
         plain_iters.append(r["iterations"])
     plain = a.plain_sample / (time.perf_counter() - t)
     log(f"plain fp64 single-instance: {plain:.2f} solves/s")
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(lp, lo, hi, a.cpu_seconds)
+        cpu = (cpu_baseline_lps(lps, a.cpu_seconds) if lps is not None
+               else cpu_baseline(lp, lo, hi, a.cpu_seconds))
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -318,15 +362,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if prec == 1 else "f64",
-        "data": "synthetic (seeded generator: mipx_amd/csrc/lpgen.cpp, base seed 0xC2)",
-        "config": {"workload": "c2: 4096 B&B node relaxations sharing A, m=3000 n=8000 "
-                               "nnz=48000, tol 1e-6 rel KKT",
+        "data": "synthetic (seeded generator: mipx_amd/csrc/lpgen.cpp, base seed "
+                + ("0xC3)" if lps is not None else "0xC2)"),
+        "config": {"workload": ("c3: 1024 independent LPs, m log-uniform on [200, 5e4], n=2.5m, "
+                                "4-8 nnz per column, tol 1e-6 rel KKT" if lps is not None else
+                                "c2: 4096 B&B node relaxations sharing A, m=3000 n=8000 "
+                                "nnz=48000, tol 1e-6 rel KKT"),
                    "instances": a.count, "precision": a.precision,
                    "kernels": "lane-per-instance" if st["interleaved"] else "packed"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": read_traffic(a.workload, a.precision),
-                     "kernel": "kILPrimal (primal half step, lane-per-instance)",
+                     "kernel": ("kPackedPrimal (primal half step, packed tiles)" if lps is not None
+                                else "kILPrimal (primal half step, lane-per-instance)"),
                      "kernel_us": round(prof["primal_kernel_us"], 2),
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "instances_in_launch": prof["profiled_slots"]},

@@ -147,88 +147,153 @@ __device__ inline void forTiles(int cnt, F&& f) {
     }
 }
 
+// Per-round record of an active packed instance in the precision it runs
+// this round: every pointer and scalar a tile needs, written once per round
+// by kTileFill, so that a tile costs one indirection instead of a chain of
+// dependent loads.  step and pw are rounded to T exactly as the iteration
+// uses them.
+template <typename T>
+struct PRec {
+    const int *cs, *ri, *rs, *ci;
+    const T *valt, *val, *c, *tau, *l, *u, *x0, *sig, *rl, *ru, *y0;
+    T *x, *xr, *xh, *y, *yh;
+    T step, pw;
+    int32_t inner, lim, n, m;
+};
+
+// Columns (rows) per lane group in a packed tile: their loads are issued
+// together; each column still sums its entries per lane in storage order, so
+// results do not depend on this constant.
+constexpr int kPackCW = 4;
+
 template <int G, typename T>
-__device__ inline void primalTile(const BInst& in, int first, int kiter) {
-    const BCtrl& c = *in.ctrl;
-    const int lim = c.round_iters;
-    if (kiter > lim) return;
-    const BVecs<T>& v = pickVecs<T>(in);
-    const int j = first + static_cast<int>(threadIdx.x) / G;
-    const int lane = static_cast<int>(threadIdx.x) % G;
-    if (j >= in.n) return;
-    const T aty = bDot<G>(in.cs, in.ri, v.valt, v.y, j, lane);
-    if (lane != 0) return;
-    const int kk = c.inner + kiter;
+__device__ inline void primalTile(const PRec<T>& r, int first, int kiter) {
+    if (kiter > r.lim) return;
+    constexpr int per = kBlock / G;
+    const int lane = static_cast<int>(threadIdx.x) % G, grp = static_cast<int>(threadIdx.x) / G;
+    int j[kPackCW], b[kPackCW], cnt[kPackCW];
+    T acc[kPackCW];
+    int most = 0;
+#pragma unroll
+    for (int q = 0; q < kPackCW; ++q) {
+        j[q] = first + grp + q * per;
+        const bool ok = j[q] < r.n;
+        const int bb = ok ? r.cs[j[q]] : 0, ee = ok ? r.cs[j[q] + 1] : 0;
+        b[q] = bb + lane;
+        cnt[q] = ee > b[q] ? (ee - b[q] + G - 1) / G : 0;
+        most = cnt[q] > most ? cnt[q] : most;
+        acc[q] = T(0);
+    }
+    for (int s = 0; s < most; ++s) {
+#pragma unroll
+        for (int q = 0; q < kPackCW; ++q)
+            if (s < cnt[q]) {
+                const int k = b[q] + s * G;
+                acc[q] += r.valt[k] * r.y[r.ri[k]];
+            }
+    }
+    const int kk = r.inner + kiter;
     const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
-    const T step = static_cast<T>(c.step), pw = static_cast<T>(c.pw);
-    const T tau = step * v.tau[j] / pw;
-    const T xj = v.x[j];
-    const T tt = xj - tau * (v.c[j] + aty);
-    const T h = clampRef(tt, v.l[j], v.u[j]);
-    const T r = T(2) * h - xj;
-    if (kiter == lim) v.xh[j] = h;
-    v.xr[j] = r;
-    v.x[j] = lam * r + (T(1) - lam) * v.x0[j];
+#pragma unroll
+    for (int q = 0; q < kPackCW; ++q) {
+        const T aty = groupSum<G>(acc[q]);
+        if (lane != 0 || j[q] >= r.n) continue;
+        const int jj = j[q];
+        const T tau = r.step * r.tau[jj] / r.pw;
+        const T xj = r.x[jj];
+        const T tt = xj - tau * (r.c[jj] + aty);
+        const T h = clampRef(tt, r.l[jj], r.u[jj]);
+        const T rr = T(2) * h - xj;
+        if (kiter == r.lim) r.xh[jj] = h;
+        r.xr[jj] = rr;
+        r.x[jj] = lam * rr + (T(1) - lam) * r.x0[jj];
+    }
 }
 
 template <int G, typename T>
-__device__ inline void dualTile(const BInst& in, int first, int kiter) {
-    const BCtrl& c = *in.ctrl;
-    const int lim = c.round_iters;
-    if (kiter > lim) return;
-    const BVecs<T>& v = pickVecs<T>(in);
-    const int i = first + static_cast<int>(threadIdx.x) / G;
-    const int lane = static_cast<int>(threadIdx.x) % G;
-    if (i >= in.m) return;
-    const T axr = bDot<G>(in.rs, in.ci, v.val, v.xr, i, lane);
-    if (lane != 0) return;
-    const int kk = c.inner + kiter;
+__device__ inline void dualTile(const PRec<T>& r, int first, int kiter) {
+    if (kiter > r.lim) return;
+    constexpr int per = kBlock / G;
+    const int lane = static_cast<int>(threadIdx.x) % G, grp = static_cast<int>(threadIdx.x) / G;
+    int i[kPackCW], b[kPackCW], cnt[kPackCW];
+    T acc[kPackCW];
+    int most = 0;
+#pragma unroll
+    for (int q = 0; q < kPackCW; ++q) {
+        i[q] = first + grp + q * per;
+        const bool ok = i[q] < r.m;
+        const int bb = ok ? r.rs[i[q]] : 0, ee = ok ? r.rs[i[q] + 1] : 0;
+        b[q] = bb + lane;
+        cnt[q] = ee > b[q] ? (ee - b[q] + G - 1) / G : 0;
+        most = cnt[q] > most ? cnt[q] : most;
+        acc[q] = T(0);
+    }
+    for (int s = 0; s < most; ++s) {
+#pragma unroll
+        for (int q = 0; q < kPackCW; ++q)
+            if (s < cnt[q]) {
+                const int k = b[q] + s * G;
+                acc[q] += r.val[k] * r.xr[r.ci[k]];
+            }
+    }
+    const int kk = r.inner + kiter;
     const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
-    const T step = static_cast<T>(c.step), pw = static_cast<T>(c.pw);
-    const T sig = step * v.sig[i] * pw;
-    const T yi = v.y[i];
-    const T vv = yi + sig * axr;
-    const T h = dualProx(vv, sig, v.rl[i], v.ru[i]);
-    const T r = T(2) * h - yi;
-    if (kiter == lim) v.yh[i] = h;
-    v.y[i] = lam * r + (T(1) - lam) * v.y0[i];
+#pragma unroll
+    for (int q = 0; q < kPackCW; ++q) {
+        const T axr = groupSum<G>(acc[q]);
+        if (lane != 0 || i[q] >= r.m) continue;
+        const int ii = i[q];
+        const T sig = r.step * r.sig[ii] * r.pw;
+        const T yi = r.y[ii];
+        const T vv = yi + sig * axr;
+        const T h = dualProx(vv, sig, r.rl[ii], r.ru[ii]);
+        const T rr = T(2) * h - yi;
+        if (kiter == r.lim) r.yh[ii] = h;
+        r.y[ii] = lam * rr + (T(1) - lam) * r.y0[ii];
+    }
 }
 
 // The lane-group size of a tile is its instance's own preset, so an
 // instance's arithmetic does not depend on the rest of the batch.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kPackedPrimal(const BInst* __restrict__ insts,
+                                                         const PRec<T>* __restrict__ recs,
                                                          const Tile* __restrict__ tiles,
                                                          const int* __restrict__ total, int kiter) {
     forTiles(*total, [&](int t) {
         const Tile tl = tiles[t];
-        const BInst& in = insts[tl.inst];
-        switch (presetGC(in.preset)) {
-            case 1: primalTile<1, T>(in, tl.first, kiter); break;
-            case 2: primalTile<2, T>(in, tl.first, kiter); break;
-            case 4: primalTile<4, T>(in, tl.first, kiter); break;
-            case 8: primalTile<8, T>(in, tl.first, kiter); break;
-            default: primalTile<16, T>(in, tl.first, kiter); break;
+        const PRec<T>& r = recs[tl.inst];
+        switch (presetGC(insts[tl.inst].preset)) {
+            case 1: primalTile<1, T>(r, tl.first, kiter); break;
+            case 2: primalTile<2, T>(r, tl.first, kiter); break;
+            case 4: primalTile<4, T>(r, tl.first, kiter); break;
+            case 8: primalTile<8, T>(r, tl.first, kiter); break;
+            default: primalTile<16, T>(r, tl.first, kiter); break;
         }
     });
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kPackedDual(const BInst* __restrict__ insts,
+                                                       const PRec<T>* __restrict__ recs,
                                                        const Tile* __restrict__ tiles,
                                                        const int* __restrict__ total, int kiter) {
     forTiles(*total, [&](int t) {
         const Tile tl = tiles[t];
-        const BInst& in = insts[tl.inst];
-        switch (presetGR(in.preset)) {
-            case 1: dualTile<1, T>(in, tl.first, kiter); break;
-            case 2: dualTile<2, T>(in, tl.first, kiter); break;
-            case 4: dualTile<4, T>(in, tl.first, kiter); break;
-            case 8: dualTile<8, T>(in, tl.first, kiter); break;
-            default: dualTile<16, T>(in, tl.first, kiter); break;
+        const PRec<T>& r = recs[tl.inst];
+        switch (presetGR(insts[tl.inst].preset)) {
+            case 1: dualTile<1, T>(r, tl.first, kiter); break;
+            case 2: dualTile<2, T>(r, tl.first, kiter); break;
+            case 4: dualTile<4, T>(r, tl.first, kiter); break;
+            case 8: dualTile<8, T>(r, tl.first, kiter); break;
+            default: dualTile<16, T>(r, tl.first, kiter); break;
         }
     });
 }
+
+// Columns / rows covered by one packed tile.
+__host__ __device__ constexpr int tileCols(int preset) { return kBlock / presetGC(preset) * kPackCW; }
+__host__ __device__ constexpr int tileRows(int preset) { return kBlock / presetGR(preset) * kPackCW; }
 
 // Per-instance tile counts for the four lists (FP32 columns, FP32 rows,
 // FP64 columns, FP64 rows) of the coming round.
@@ -236,7 +301,7 @@ __global__ void kTileCounts(const BInst* __restrict__ insts, int K, int* __restr
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= K) return;
     const BInst& in = insts[k];
-    const int cpt = kBlock / presetGC(in.preset), rpt = kBlock / presetGR(in.preset);
+    const int cpt = tileCols(in.preset), rpt = tileRows(in.preset);
     const BCtrl& c = *in.ctrl;
     const bool act = !in.il && c.phase != kPhaseDone && c.round_iters > 0;
     const bool p32 = act && c.phase == kPhaseFp32, p64 = act && c.phase != kPhaseFp32;
@@ -289,17 +354,52 @@ __global__ __launch_bounds__(1024) void kScan4(int* __restrict__ cnt, int K,
     if (threadIdx.x == 0) totals[4] = part[0];
 }
 
+template <typename T>
+__device__ inline void fillRec(const BInst& in, const BCtrl& c, PRec<T>& r) {
+    const BVecs<T>& v = pickVecs<T>(in);
+    r.cs = in.cs;
+    r.ri = in.ri;
+    r.rs = in.rs;
+    r.ci = in.ci;
+    r.valt = v.valt;
+    r.val = v.val;
+    r.c = v.c;
+    r.tau = v.tau;
+    r.l = v.l;
+    r.u = v.u;
+    r.x0 = v.x0;
+    r.sig = v.sig;
+    r.rl = v.rl;
+    r.ru = v.ru;
+    r.y0 = v.y0;
+    r.x = v.x;
+    r.xr = v.xr;
+    r.xh = v.xh;
+    r.y = v.y;
+    r.yh = v.yh;
+    r.step = static_cast<T>(c.step);
+    r.pw = static_cast<T>(c.pw);
+    r.inner = c.inner;
+    r.lim = c.round_iters;
+    r.n = in.n;
+    r.m = in.m;
+}
+
+// Tile lists and per-round records of the active packed instances.
 __global__ void kTileFill(const BInst* __restrict__ insts, int K, const int* __restrict__ off,
                           Tile* __restrict__ c32, Tile* __restrict__ r32, Tile* __restrict__ c64,
-                          Tile* __restrict__ r64) {
+                          Tile* __restrict__ r64, PRec<float>* __restrict__ rec32,
+                          PRec<double>* __restrict__ rec64) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= K) return;
     const BInst& in = insts[k];
-    const int cpt = kBlock / presetGC(in.preset), rpt = kBlock / presetGR(in.preset);
+    const int cpt = tileCols(in.preset), rpt = tileRows(in.preset);
     const BCtrl& c = *in.ctrl;
     const bool act = !in.il && c.phase != kPhaseDone && c.round_iters > 0;
     if (!act) return;
     const bool p32 = c.phase == kPhaseFp32;
+    if (p32) fillRec<float>(in, c, rec32[k]);
+    else fillRec<double>(in, c, rec64[k]);
     Tile* cl = p32 ? c32 : c64;
     Tile* rl = p32 ? r32 : r64;
     const int oc = off[(p32 ? 0 : 2) * K + k], orr = off[(p32 ? 1 : 3) * K + k];
@@ -1178,7 +1278,9 @@ private:
     std::unique_ptr<DevBuf> d_insts_buf_, d_args_buf_, d_count_buf_;
     std::unique_ptr<SetupCohort> cohort_;  // device setup of every owner
     std::unique_ptr<DevBuf> d_cohort_buf_;
-    std::unique_ptr<DevBuf> d_tiles_buf_, d_cnt_buf_;
+    std::unique_ptr<DevBuf> d_tiles_buf_, d_cnt_buf_, d_recs_buf_;
+    PRec<float>* rec32_ = nullptr;
+    PRec<double>* rec64_ = nullptr;
     Tile *tc32_ = nullptr, *tr32_ = nullptr, *tc64_ = nullptr, *tr64_ = nullptr;
     int* totals_ = nullptr;  // [0..3] tile counts, [4] unfinished instances
     size_t max_col_tiles_ = 0, max_row_tiles_ = 0;
@@ -1256,6 +1358,11 @@ void BatchEngine::setup() {
         tr64_ = tc64_ + max_col_tiles_;
         totals_ = reinterpret_cast<int*>(tr64_ + max_row_tiles_);
         d_cnt_buf_ = std::make_unique<DevBuf>(sizeof(int) * 4 * static_cast<size_t>(std::max(1, ni)));
+        const size_t nr = static_cast<size_t>(std::max(1, ni));
+        d_recs_buf_ = std::make_unique<DevBuf>(pad(nr, sizeof(PRec<float>)) + pad(nr, sizeof(PRec<double>)));
+        rec32_ = static_cast<PRec<float>*>(d_recs_buf_->p);
+        rec64_ = reinterpret_cast<PRec<double>*>(static_cast<char*>(d_recs_buf_->p) +
+                                                 pad(nr, sizeof(PRec<float>)));
     }
     // Lane-per-instance mode when every instance shares one matrix.
     interleaved_ = allow_interleaved_ && no == 1 && ni >= 1;
@@ -1394,16 +1501,16 @@ void BatchEngine::enqueueRound(const IterPreset& p, bool timed) {
     const int gk = ceilDiv(K, kBlock);
     kTileCounts<<<gk, kBlock, 0, st_>>>(di, K, cnt);
     kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
-    kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
+    kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_, rec32_, rec64_);
     if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
     const bool f32 = mixed_ && !fp32_gone_;
     for (int k = 1; k <= bo_.N; ++k) {
         if (f32) {
-            kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
-            kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, tr32_, totals_ + 1, k);
+            kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tc32_, totals_ + 0, k);
+            kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tr32_, totals_ + 1, k);
         }
-        kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, tc64_, totals_ + 2, k);
-        kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, tr64_, totals_ + 3, k);
+        kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tc64_, totals_ + 2, k);
+        kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
     }
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
     if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
@@ -1480,14 +1587,14 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
         const int gk = ceilDiv(K, kBlock);
         kTileCounts<<<gk, kBlock, 0, st_>>>(di, K, cnt);
         kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
-        kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
+        kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_, rec32_, rec64_);
         for (int k = 1; k <= launches; ++k) {
             MPX_HIP(hipEventRecord(e[0], st_));
-            if (f32) kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
-            else kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, tc64_, totals_ + 2, k);
+            if (f32) kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tc32_, totals_ + 0, k);
+            else kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tc64_, totals_ + 2, k);
             MPX_HIP(hipEventRecord(e[1], st_));
-            if (f32) kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, tr32_, totals_ + 1, k);
-            else kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, tr64_, totals_ + 3, k);
+            if (f32) kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tr32_, totals_ + 1, k);
+            else kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
             MPX_HIP(hipEventRecord(e[2], st_));
             MPX_HIP(hipEventSynchronize(e[2]));
             float a = 0.0f, b = 0.0f;
@@ -1900,6 +2007,10 @@ public:
         tab_ = std::make_unique<DevBuf>(sizeof(BInst) * S);
         ctrl_ = std::make_unique<DevBuf>(sizeof(BCtrl) * S);
         cnt_ = std::make_unique<DevBuf>(sizeof(int) * 4 * S);
+        recs_ = std::make_unique<DevBuf>(pad(S, sizeof(PRec<float>)) + pad(S, sizeof(PRec<double>)));
+        rec32_ = static_cast<PRec<float>*>(recs_->p);
+        rec64_ = reinterpret_cast<PRec<double>*>(static_cast<char*>(recs_->p) +
+                                                 pad(S, sizeof(PRec<float>)));
         newi_ = std::make_unique<DevBuf>(sizeof(BInst) * S);
         newa_ = std::make_unique<DevBuf>(sizeof(BInitArgs) * S);
         news_ = std::make_unique<DevBuf>(sizeof(int) * S);
@@ -2337,16 +2448,16 @@ private:
         const int gk = ceilDiv(K, kBlock);
         kTileCounts<<<gk, kBlock, 0, st_>>>(di, K, cnt);
         kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
-        kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_);
+        kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_, rec32_, rec64_);
         const bool f32 = mixed_ && !fp32_gone_;
         for (auto& b : bases_) ilGather(st_, grid_, di, K, b->owner, b->il, f32);
         for (int k = 1; k <= bo_.N; ++k) {
             if (f32) {
-                kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, tc32_, totals_ + 0, k);
-                kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, tr32_, totals_ + 1, k);
+                kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tc32_, totals_ + 0, k);
+                kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tr32_, totals_ + 1, k);
             }
-            kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, tc64_, totals_ + 2, k);
-            kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, tr64_, totals_ + 3, k);
+            kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tc64_, totals_ + 2, k);
+            kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
             for (auto& b : bases_) ilIterate(st_, grid_, di, b->owner, b->il, f32, 1, k);
         }
         for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, f32);
@@ -2500,7 +2611,9 @@ private:
     int S_;
     hipStream_t st_ = nullptr;
     std::unique_ptr<DevicePool> pool_;
-    std::unique_ptr<DevBuf> tab_, ctrl_, cnt_, newi_, newa_, news_, ptrs_;
+    std::unique_ptr<DevBuf> tab_, ctrl_, cnt_, newi_, newa_, news_, ptrs_, recs_;
+    PRec<float>* rec32_ = nullptr;
+    PRec<double>* rec64_ = nullptr;
     void* tiles_mem_ = nullptr;
     PinnedArray<BCtrl> h_ctrl_;
     PinnedArray<BInst> h_new_;
