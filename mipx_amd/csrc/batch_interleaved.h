@@ -226,6 +226,43 @@ __device__ inline T laneDot(const int* __restrict__ idx, const T* __restrict__ v
     return acc;
 }
 
+// Gathered dot products acc[c] = sum_q val[q] * V[idx[q] * 64] over the
+// entries [qb[c], qe[c]) of CW columns (rows) at once: entries go in chunks
+// of four per column, so up to 4 * CW gathers are in flight and a short tail
+// costs one round trip, not one per entry.  Each column's additions stay in
+// storage order (an entry past the end is skipped, not added as zero).
+// (Measured slower than laneDot for one column per wave; used for CW > 1.)
+template <typename T, int CW>
+__device__ inline void laneDotN(const int* __restrict__ idx, const T* __restrict__ val,
+                                const int (&qb)[CW], const int (&qe)[CW],
+                                const T* __restrict__ V, T (&acc)[CW]) {
+    int most = 0;
+#pragma unroll
+    for (int c = 0; c < CW; ++c) most = max(most, qe[c] - qb[c]);
+    for (int t = 0; t < most; t += 4) {
+        int ix[CW][4];
+        T a[CW][4], v[CW][4];
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = qb[c] + t + u;
+                const bool ok = q < qe[c];
+                ix[c][u] = ok ? idx[q] : 0;
+                a[c][u] = ok ? val[q] : T(0);
+            }
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[c][u] = V[static_cast<size_t>(ix[c][u]) * kLanes];
+#pragma unroll
+        for (int c = 0; c < CW; ++c)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (qb[c] + t + u < qe[c]) acc[c] += a[c][u] * v[c][u];
+    }
+}
+
 // CW = columns (rows) per wave in the iteration kernels: their epilogue
 // operands and gathers are issued together so that several memory round trips
 // are in flight per wave; each column still sums its entries in storage
@@ -257,7 +294,6 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         const size_t base = (static_cast<size_t>(g) * n + j0) * kLanes + lane;
         T xv[kCW], x0v[kCW], lv[kCW], uv[kCW], acc[kCW];
         int qb[kCW], qe[kCW];
-        int maxlen = 0;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             const bool in = j0 + cc < n;
@@ -269,19 +305,9 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
             qb[cc] = in ? cs[j0 + cc] : 0;
             qe[cc] = in ? cs[j0 + cc + 1] : 0;
             acc[cc] = T(0);
-            maxlen = max(maxlen, qe[cc] - qb[cc]);
         }
-        if constexpr (kCW == 1) {
-            acc[0] = laneDot(ri, valt, qb[0], qe[0], Y);
-        } else {
-            for (int t = 0; t < maxlen; ++t) {
-#pragma unroll
-                for (int cc = 0; cc < kCW; ++cc) {
-                    const int q = qb[cc] + t;
-                    if (q < qe[cc]) acc[cc] += valt[q] * Y[static_cast<size_t>(ri[q]) * kLanes];
-                }
-            }
-        }
+        if constexpr (kCW == 1) acc[0] = laneDot(ri, valt, qb[0], qe[0], Y);
+        else laneDotN<T, kCW>(ri, valt, qb, qe, Y, acc);
         const int kk = ln.inner + kiter;
         const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
         const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
@@ -324,7 +350,6 @@ __global__ __launch_bounds__(kBlock) void kILDual(
         const size_t base = (static_cast<size_t>(g) * m + i0) * kLanes + lane;
         T yv[kCW], y0v[kCW], acc[kCW];
         int qb[kCW], qe[kCW];
-        int maxlen = 0;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             const bool in = i0 + cc < m;
@@ -334,19 +359,9 @@ __global__ __launch_bounds__(kBlock) void kILDual(
             qb[cc] = in ? rs[i0 + cc] : 0;
             qe[cc] = in ? rs[i0 + cc + 1] : 0;
             acc[cc] = T(0);
-            maxlen = max(maxlen, qe[cc] - qb[cc]);
         }
-        if constexpr (kCW == 1) {
-            acc[0] = laneDot(ci, val, qb[0], qe[0], X);
-        } else {
-            for (int t = 0; t < maxlen; ++t) {
-#pragma unroll
-                for (int cc = 0; cc < kCW; ++cc) {
-                    const int q = qb[cc] + t;
-                    if (q < qe[cc]) acc[cc] += val[q] * X[static_cast<size_t>(ci[q]) * kLanes];
-                }
-            }
-        }
+        if constexpr (kCW == 1) acc[0] = laneDot(ci, val, qb[0], qe[0], X);
+        else laneDotN<T, kCW>(ci, val, qb, qe, X, acc);
         const int kk = ln.inner + kiter;
         const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
         const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
