@@ -188,6 +188,16 @@ def run_c4(a, prec):
     elapsed = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
+    # Plain sequential single-LP FP64 GPU PDLP on a sample of the same stream.
+    sample = np.linspace(0, len(sched) - 1, a.plain_sample).astype(int).tolist()
+    tp = time.perf_counter()
+    for i in sample:
+        _, variant, nnz = sched[i]
+        one = (g.lp.with_col_bounds(*g.branch_bounds(lpgen.c4_variant_seed(i))) if variant
+               else lpgen.c4_fresh(i, nnz).lp)
+        native.solve(one, max_iter=a.max_iter, **TOL)
+    plain = len(sample) / (time.perf_counter() - tp)
+    log(f"plain fp64 sequential: {plain:.2f} requests/s")
     all_lat = lat["variant"] + lat["fresh"]
     pc = lambda v, p: round(float(np.percentile(v, p)), 2) if v else None
     done = sum(status)
@@ -210,6 +220,8 @@ def run_c4(a, prec):
                    "offered_requests": len(sched), "precision": a.precision, "slots": a.slots,
                    "hbm_budget_gb": a.budget_gb},
         "offered_rate": round(len(sched) / a.duration, 2),
+        "plain_fp64_gpu_requests_per_s": round(plain, 3),
+        "speedup_vs_plain": round(done / elapsed / plain, 2),
         "finished": done,
         "unfinished": len(sched) - done,
         "latency_ms": {"p50": pc(all_lat, 50), "p99": pc(all_lat, 99), "max": pc(all_lat, 100),
@@ -245,16 +257,65 @@ def cpu_baseline_lps(lps, budget_s):
                 sample=f"the first {done} C3 LPs solved back to back on one host core in {el:.1f} s")
 
 
+def run_c1(a):
+    """One large LP (m=1e6, n=2e6, power-law columns): the single-LP path in
+    FP64 and in mixed precision (FP32 iterations, FP64 checks, refinement to
+    1e-8), each for a fixed number of PDHG iterations; reports iterations/s
+    and the algorithmic HBM rate of an iteration."""
+    from mipx_amd import lpgen, native
+    g = lpgen.c1_large()
+    lp = g.lp
+    m, n, nnz = lp.num_rows, lp.num_cols, lp.nnz
+    log(f"c1: m={m} n={n} nnz={nnz}")
+    runs = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        opts = dict(TOL, max_iter=a.c1_iters, precision=prec)
+        if prec == 1:
+            opts["refine_tol"] = 1e-8
+        native.solve(lp, **dict(opts, max_iter=200))  # warm-up: code load, graph build
+        t = time.perf_counter()
+        r, _, _ = native.solve(lp, **opts)
+        el = time.perf_counter() - t
+        it = r["iterations"]
+        sv = 4 if (prec == 1 and r["iterations_fp32"] == it) else 8
+        b_iter = 2 * nnz * (sv + 4) + (m + n + 2) * 4 + 9 * n * sv + 7 * m * sv
+        per = (el - r["setup_seconds"]) / max(1, it)
+        runs[name] = {"iterations": it, "iterations_fp32": r["iterations_fp32"],
+                      "status": r["status"], "rel_primal": r["rel_primal"],
+                      "rel_dual": r["rel_dual"], "rel_gap": r["rel_gap"],
+                      "setup_s": round(r["setup_seconds"], 3), "wall_s": round(el, 3),
+                      "us_per_iteration": round(1e6 * per, 2),
+                      "iterations_per_s": round(1.0 / per, 1),
+                      "algorithmic_GBs": round(b_iter / per / 1e9, 1),
+                      "bytes_per_iteration": b_iter}
+        log(f"{name}: {runs[name]}")
+    best = runs["mixed"]
+    return {
+        "metric": "PDHG iterations/s on one large LP (single-LP path); HBM GB/s",
+        "value": best["iterations_per_s"], "unit": "iterations/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(best["wall_s"] * 1000, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (seeded generator: mipx_amd/csrc/lpgen.cpp, seed 0xC1)",
+        "config": {"workload": f"c1: m={m} n={n} nnz={nnz}, power-law columns 1..200 mean 6, "
+                               f"{a.c1_iters} PDHG iterations per run"},
+        "roofline": {"bound": "hbm", "achieved": best["algorithmic_GBs"], "peak": PEAK_GBS,
+                     "unit": "GB/s", "frac": round(best["algorithmic_GBs"] / PEAK_GBS, 4),
+                     "traffic": None, "kernel": "one PDHG iteration (primal + dual fused SpMV steps)"},
+        "runs": runs,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--c1-iters", type=int, default=4000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--slots", type=int, default=2048)
-    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--drain-seconds", type=float, default=120.0)
     ap.add_argument("--max-iter", type=int, default=400000)
     ap.add_argument("--count", type=int, default=4096)
@@ -276,6 +337,9 @@ def main():
     prec = 1 if a.precision == "mixed" else 0
     if a.workload == "c4":
         print(json.dumps(run_c4(a, prec)), flush=True)
+        return
+    if a.workload == "c1":
+        print(json.dumps(run_c1(a)), flush=True)
         return
     log(f"building workload {a.workload} count={a.count}")
     lps = None

@@ -282,13 +282,19 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
     const T* __restrict__ IU, int kiter) {
     const int nslots = *count;
     const int per_group = (n + 4 * kCW - 1) / (4 * kCW);
+    // A lane's scalars are the same for every column of its group: loaded
+    // when the group changes (work items come group by group).
+    int curg = -1;
+    ILane ln{1.0, 1.0, 0, 0};
     forGroupItems(nslots, per_group, [&](int g, int jb) {
         const int j0 = (jb * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW;
         const int lane = static_cast<int>(threadIdx.x) % kLanes;
         if (j0 >= n) return;
         const int s = g * kLanes + lane;
-        const bool live = s < nslots;
-        const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+        if (g != curg) {
+            curg = g;
+            ln = s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+        }
         if (kiter > ln.lim) return;  // lane idle this iteration
         const T* Y = IY + static_cast<size_t>(g) * m * kLanes + lane;
         const size_t base = (static_cast<size_t>(g) * n + j0) * kLanes + lane;
@@ -338,13 +344,17 @@ __global__ __launch_bounds__(kBlock) void kILDual(
     T* __restrict__ IY, const T* __restrict__ IY0, int kiter) {
     const int nslots = *count;
     const int per_group = (m + 4 * kCW - 1) / (4 * kCW);
+    int curg = -1;
+    ILane ln{1.0, 1.0, 0, 0};
     forGroupItems(nslots, per_group, [&](int g, int ib) {
         const int i0 = (ib * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW;
         const int lane = static_cast<int>(threadIdx.x) % kLanes;
         if (i0 >= m) return;
         const int s = g * kLanes + lane;
-        const bool live = s < nslots;
-        const ILane ln = live ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+        if (g != curg) {
+            curg = g;
+            ln = s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+        }
         if (kiter > ln.lim) return;
         const T* X = IXR + static_cast<size_t>(g) * n * kLanes + lane;
         const size_t base = (static_cast<size_t>(g) * m + i0) * kLanes + lane;

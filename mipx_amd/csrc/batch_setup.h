@@ -546,8 +546,11 @@ public:
         a_.maxT = maxT_;
     }
 
-    // Uploads the cohort tables and runs the whole setup on `st`.
-    void run(hipStream_t st) {
+    // Uploads the cohort tables and runs the whole setup on `st`; the power
+    // iteration is captured in graphs unless `graphs` is false (a caller on
+    // a second host thread launches directly, so that no two threads capture
+    // at once).
+    void run(hipStream_t st, bool graphs = true) {
         const int k = count();
         if (k == 0) return;
         auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -584,7 +587,7 @@ public:
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         bool graphed =
-            hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            graphs && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (graphed) {
             powerGroup(st, k);
             graphed = hipStreamEndCapture(st, &g) == hipSuccess && g != nullptr &&

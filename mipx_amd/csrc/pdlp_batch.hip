@@ -865,10 +865,9 @@ int ownerPreset(const mpx_pdlp_options& o, int nnz, int m, int n) {
     return best;
 }
 
-// Carves the device arrays of one distinct matrix from `cv`, queues the
-// uploads of `lp` (and of `h`) on `st` and fills `b`.
-void carveOwner(const mpx_lp& lp, const OwnerHost& h, const mpx_pdlp_options& o, Carve& cv,
-                hipStream_t st, BOwner& b) {
+// Carves the device arrays of one distinct matrix from `cv` and fills `b`
+// (uploadOwner() then fills the arrays).
+void carveOwner(const mpx_lp& lp, const mpx_pdlp_options& o, Carve& cv, BOwner& b) {
     const int m = lp.num_rows, n = lp.num_cols, nnz = lp.row_starts[m];
     std::memset(&b, 0, sizeof(b));
     b.m = m;
@@ -918,6 +917,34 @@ void carveOwner(const mpx_lp& lp, const OwnerHost& h, const mpx_pdlp_options& o,
     b.rlf = cv.take<float>(m);
     b.ruf = cv.take<float>(m);
     b.scal = cv.take<double>(8);
+    b.rs = rs;
+    b.ci = ci;
+    b.cs = cs;
+    b.ri = ri;
+    b.perm = perm;
+    b.val_o = val_o;
+    b.obj_o = obj_o;
+    b.lo_o = lo_o;
+    b.hi_o = hi_o;
+    b.rl_o = rl_o;
+    b.ru_o = ru_o;
+}
+
+// Queues the uploads of `lp` and `h` into the arrays of `b` on `st`; the
+// host data must stay alive until the stream has consumed them.
+void uploadOwner(const mpx_lp& lp, const OwnerHost& h, const BOwner& b, hipStream_t st) {
+    const int m = lp.num_rows, n = lp.num_cols, nnz = lp.row_starts[m];
+    int* rs = const_cast<int*>(b.rs);
+    int* ci = const_cast<int*>(b.ci);
+    int* ri = const_cast<int*>(b.ri);
+    int* perm = const_cast<int*>(b.perm);
+    int* cs = const_cast<int*>(b.cs);
+    double* val_o = const_cast<double*>(b.val_o);
+    double* obj_o = const_cast<double*>(b.obj_o);
+    double* lo_o = const_cast<double*>(b.lo_o);
+    double* hi_o = const_cast<double*>(b.hi_o);
+    double* rl_o = const_cast<double*>(b.rl_o);
+    double* ru_o = const_cast<double*>(b.ru_o);
     MPX_HIP(hipMemcpyAsync(rs, lp.row_starts, (m + 1) * 4ull, hipMemcpyHostToDevice, st));
     if (nnz) {
         MPX_HIP(hipMemcpyAsync(ci, lp.col_indices, nnz * 4ull, hipMemcpyHostToDevice, st));
@@ -934,17 +961,6 @@ void carveOwner(const mpx_lp& lp, const OwnerHost& h, const mpx_pdlp_options& o,
         MPX_HIP(hipMemcpyAsync(ru_o, lp.row_upper, m * 8ull, hipMemcpyHostToDevice, st));
     }
     MPX_HIP(hipMemcpyAsync(b.px0, h.start.data(), n * 8ull, hipMemcpyHostToDevice, st));
-    b.rs = rs;
-    b.ci = ci;
-    b.cs = cs;
-    b.ri = ri;
-    b.perm = perm;
-    b.val_o = val_o;
-    b.obj_o = obj_o;
-    b.lo_o = lo_o;
-    b.hi_o = hi_o;
-    b.rl_o = rl_o;
-    b.ru_o = ru_o;
 }
 
 // Carves the device arrays of one instance of matrix `w` (its own column
@@ -1329,7 +1345,8 @@ void BatchEngine::setup() {
     for (int w = 0; w < no; ++w) {
         const mpx_lp& lp = *owners_host_[w];
         hosts[w] = ownerHostData(lp);
-        carveOwner(lp, hosts[w], o_, cv, st_, owners_[w]);
+        carveOwner(lp, o_, cv, owners_[w]);
+        uploadOwner(lp, hosts[w], owners_[w], st_);
         owner_preset_[w] = ownerPreset(o_, owners_[w].nnz, owners_[w].m, owners_[w].n);
     }
     insts_.resize(static_cast<size_t>(ni));
@@ -1936,9 +1953,9 @@ struct ServeLp {
     size_t bytes() const {
         return ownerBytes(lp.num_rows, lp.num_cols, lp.row_starts[lp.num_rows]);
     }
-    // Carves the matrix arrays and queues their uploads on `st`.
-    void upload(const mpx_pdlp_options& o, const OwnerHost& h, Carve& cv, hipStream_t st) {
-        carveOwner(lp, h, o, cv, st, owner);
+    // Carves the matrix arrays (uploadOwner() fills them).
+    void carve(const mpx_pdlp_options& o, Carve& cv) {
+        carveOwner(lp, o, cv, owner);
         preset = ownerPreset(o, owner.nnz, owner.m, owner.n);
     }
     // Lane-per-instance state of a base's variants.
@@ -1949,10 +1966,13 @@ struct ServeLp {
 // setup thread on its own stream.
 struct SetupJob {
     std::unique_ptr<SetupCohort> cohort;
-    void* mem = nullptr;             // cohort scratch (pool)
-    hipEvent_t uploaded = nullptr;   // the cohort's uploads on the main stream
+    void* mem = nullptr;  // cohort scratch (pool)
+    // What to upload first: each fresh LP's host data into its carved arrays
+    // (the requests own the host data and outlive the job).
+    std::vector<std::pair<const ServeLp*, const OwnerHost*>> uploads;
     std::atomic<bool> done{false};
     std::atomic<bool> failed{false};
+    std::string error;  // set by the setup thread before `done`
 };
 
 struct ServeReq {
@@ -2047,8 +2067,6 @@ public:
         job_cv_.notify_all();
         if (setup_thread_.joinable()) setup_thread_.join();
         (void)hipStreamSynchronize(st_);
-        for (auto& j : jobs_live_)
-            if (j->uploaded) (void)hipEventDestroy(j->uploaded);
         for (auto& kv : graphs_) kv.second.reset();
         (void)hipStreamDestroy(st_);
     }
@@ -2064,7 +2082,8 @@ public:
             throw OutOfMemory();
         }
         Carve cv{static_cast<char*>(mem), 0, need};
-        b->upload(o_, h, cv, st_);
+        b->carve(o_, cv);
+        uploadOwner(b->lp, h, b->owner, st_);
         b->il.carve(cv, S_, b->lp.num_cols, b->lp.num_rows, mixed_);
         b->il.owner = static_cast<int>(bases_.size());
         SetupCohort cohort({b->owner}, {b->preset});
@@ -2237,7 +2256,8 @@ private:
             Carve cv{static_cast<char*>(mem), 0, need};
             BCtrl* ctrl = static_cast<BCtrl*>(ctrl_->p) + q->slot;
             if (q->own) {
-                q->own->upload(o_, q->oh, cv, st_);
+                // Uploads and setup are the setup thread's (its own stream).
+                q->own->carve(o_, cv);
                 carveInstance(q->own->owner, q->own->lp, -1, q->own->preset, nullptr, nullptr,
                               mixed_, cv, st_, ctrl, q->inst, q->args);
                 fresh.push_back(q.get());
@@ -2280,9 +2300,10 @@ private:
             return;
         }
         job->cohort->carve(static_cast<char*>(job->mem));
-        MPX_HIP(hipEventCreateWithFlags(&job->uploaded, hipEventDisableTiming));
-        MPX_HIP(hipEventRecord(job->uploaded, st_));
-        for (ServeReq* q : fresh) q->job = job;
+        for (ServeReq* q : fresh) {
+            q->job = job;
+            job->uploads.emplace_back(q->own.get(), &q->oh);
+        }
         jobs_live_.push_back(job);
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -2308,11 +2329,15 @@ private:
             bool good = ok;
             if (good) {
                 try {
-                    MPX_HIP(hipStreamWaitEvent(ws, job->uploaded, 0));
-                    job->cohort->run(ws);
+                    for (const auto& u : job->uploads)
+                        uploadOwner(u.first->lp, *u.second, u.first->owner, ws);
+                    job->cohort->run(ws, false);
                 } catch (const DeviceError&) {
                     good = false;
+                    job->error = lastError();
                 }
+            } else {
+                job->error = "setup thread: no device stream";
             }
             job->failed.store(!good);
             {
@@ -2355,9 +2380,10 @@ private:
                     continue;
                 }
                 const bool failed = q->job->failed.load();
+                const std::string why = failed ? q->job->error : std::string();
                 q->job.reset();
                 if (failed) {
-                    setError("serving loop: device setup of a fresh LP failed");
+                    setError("serving loop: device setup of a fresh LP failed: " + why);
                     pool_->release(q->mem);
                     free_slots_.push_back(q->slot);
                     finishNow(std::move(q), MPX_STATUS_ERROR);
@@ -2406,7 +2432,6 @@ private:
         for (auto it = jobs_live_.begin(); it != jobs_live_.end();) {
             if ((*it)->done.load() && it->use_count() == 1) {
                 pool_->release((*it)->mem);
-                (void)hipEventDestroy((*it)->uploaded);
                 it = jobs_live_.erase(it);
             } else {
                 ++it;

@@ -59,19 +59,24 @@ __global__ __launch_bounds__(kBlock) void kSpmv(int rows, const int* __restrict_
     if (lane == 0) y[row] = s;
 }
 
-// ---- primal half step (CSC SpMV + projection + reflection + Halpern) -------
-template <int G, typename T, bool STORE_HAT>
-__global__ __launch_bounds__(kBlock) void kPrimalStep(
-    int n, const int* __restrict__ cstarts, const int* __restrict__ ridx,
-    const T* __restrict__ valt, const T* __restrict__ y, T* __restrict__ x,
-    const T* __restrict__ x0, T* __restrict__ xr, T* __restrict__ xh,
-    const T* __restrict__ c, const T* __restrict__ tau_base, const T* __restrict__ lo,
-    const T* __restrict__ hi, const IterScalars* __restrict__ sc, int k) {
-    const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    const int j = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
-    if (j >= n) return;
-    const T aty = groupDot<G>(cstarts, ridx, valt, y, j, lane);
-    if (lane != 0) return;
+// Group dot product over the entries [b, e) (G lanes, butterfly finish).
+template <int G, typename T, typename TX>
+__device__ inline T groupDotBE(int b, int e, const int* __restrict__ idx, const T* __restrict__ val,
+                               const TX* __restrict__ x, int lane) {
+    T acc = T(0);
+    for (int k = b + lane; k < e; k += G) acc += val[k] * static_cast<T>(x[idx[k]]);
+    return groupSum<G>(acc);
+}
+
+// Columns (rows) with more than `skip_above` entries are left to the
+// one-wave-per-column kernels below, so a power-law tail of long columns
+// does not hold a whole wave of short ones.
+template <typename T, bool STORE_HAT>
+__device__ inline void primalEpilogue(int j, T aty, T* __restrict__ x, const T* __restrict__ x0,
+                                      T* __restrict__ xr, T* __restrict__ xh,
+                                      const T* __restrict__ c, const T* __restrict__ tau_base,
+                                      const T* __restrict__ lo, const T* __restrict__ hi,
+                                      const IterScalars* __restrict__ sc, int k) {
     const int kk = sc->inner + k;
     const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
     const T step = static_cast<T>(sc->step), pw = static_cast<T>(sc->primal_weight);
@@ -83,6 +88,41 @@ __global__ __launch_bounds__(kBlock) void kPrimalStep(
     if (STORE_HAT) xh[j] = h;
     xr[j] = r;
     x[j] = lam * r + (T(1) - lam) * x0[j];
+}
+
+// ---- primal half step (CSC SpMV + projection + reflection + Halpern) -------
+template <int G, typename T, bool STORE_HAT>
+__global__ __launch_bounds__(kBlock) void kPrimalStep(
+    int n, const int* __restrict__ cstarts, const int* __restrict__ ridx,
+    const T* __restrict__ valt, const T* __restrict__ y, T* __restrict__ x,
+    const T* __restrict__ x0, T* __restrict__ xr, T* __restrict__ xh,
+    const T* __restrict__ c, const T* __restrict__ tau_base, const T* __restrict__ lo,
+    const T* __restrict__ hi, const IterScalars* __restrict__ sc, int k, int skip_above) {
+    const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int j = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
+    if (j >= n) return;
+    const int b = cstarts[j], e = cstarts[j + 1];
+    if (e - b > skip_above) return;
+    const T aty = groupDotBE<G>(b, e, ridx, valt, y, lane);
+    if (lane != 0) return;
+    primalEpilogue<T, STORE_HAT>(j, aty, x, x0, xr, xh, c, tau_base, lo, hi, sc, k);
+}
+
+// One wave per listed (long) column.
+template <typename T, bool STORE_HAT>
+__global__ __launch_bounds__(kBlock) void kPrimalStepLong(
+    int count, const int* __restrict__ list, const int* __restrict__ cstarts,
+    const int* __restrict__ ridx, const T* __restrict__ valt, const T* __restrict__ y,
+    T* __restrict__ x, const T* __restrict__ x0, T* __restrict__ xr, T* __restrict__ xh,
+    const T* __restrict__ c, const T* __restrict__ tau_base, const T* __restrict__ lo,
+    const T* __restrict__ hi, const IterScalars* __restrict__ sc, int k) {
+    const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int w = static_cast<int>(gid / kWave), lane = static_cast<int>(gid % kWave);
+    if (w >= count) return;
+    const int j = list[w];
+    const T aty = groupDotBE<kWave>(cstarts[j], cstarts[j + 1], ridx, valt, y, lane);
+    if (lane != 0) return;
+    primalEpilogue<T, STORE_HAT>(j, aty, x, x0, xr, xh, c, tau_base, lo, hi, sc, k);
 }
 
 // Row-type prox of the dual step (src/lp/pdlp.cpp:1043-1067): equality,
@@ -97,19 +137,11 @@ __device__ inline T dualProx(T v, T sig, T rl, T ru) {
     return T(0);
 }
 
-// ---- dual half step (CSR SpMV of x_r + prox + reflection + Halpern) --------
-template <int G, typename T, bool STORE_HAT>
-__global__ __launch_bounds__(kBlock) void kDualStep(
-    int m, const int* __restrict__ rstarts, const int* __restrict__ cidx,
-    const T* __restrict__ val, const T* __restrict__ xr, T* __restrict__ y,
-    const T* __restrict__ y0, T* __restrict__ yh, const T* __restrict__ sigma_base,
-    const T* __restrict__ rl, const T* __restrict__ ru, const IterScalars* __restrict__ sc,
-    int k) {
-    const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    const int i = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
-    if (i >= m) return;
-    const T axr = groupDot<G>(rstarts, cidx, val, xr, i, lane);
-    if (lane != 0) return;
+template <typename T, bool STORE_HAT>
+__device__ inline void dualEpilogue(int i, T axr, T* __restrict__ y, const T* __restrict__ y0,
+                                    T* __restrict__ yh, const T* __restrict__ sigma_base,
+                                    const T* __restrict__ rl, const T* __restrict__ ru,
+                                    const IterScalars* __restrict__ sc, int k) {
     const int kk = sc->inner + k;
     const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
     const T step = static_cast<T>(sc->step), pw = static_cast<T>(sc->primal_weight);
@@ -120,6 +152,40 @@ __global__ __launch_bounds__(kBlock) void kDualStep(
     const T r = T(2) * h - yi;
     if (STORE_HAT) yh[i] = h;
     y[i] = lam * r + (T(1) - lam) * y0[i];
+}
+
+// ---- dual half step (CSR SpMV of x_r + prox + reflection + Halpern) --------
+template <int G, typename T, bool STORE_HAT>
+__global__ __launch_bounds__(kBlock) void kDualStep(
+    int m, const int* __restrict__ rstarts, const int* __restrict__ cidx,
+    const T* __restrict__ val, const T* __restrict__ xr, T* __restrict__ y,
+    const T* __restrict__ y0, T* __restrict__ yh, const T* __restrict__ sigma_base,
+    const T* __restrict__ rl, const T* __restrict__ ru, const IterScalars* __restrict__ sc,
+    int k, int skip_above) {
+    const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int i = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
+    if (i >= m) return;
+    const int b = rstarts[i], e = rstarts[i + 1];
+    if (e - b > skip_above) return;
+    const T axr = groupDotBE<G>(b, e, cidx, val, xr, lane);
+    if (lane != 0) return;
+    dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, sc, k);
+}
+
+template <typename T, bool STORE_HAT>
+__global__ __launch_bounds__(kBlock) void kDualStepLong(
+    int count, const int* __restrict__ list, const int* __restrict__ rstarts,
+    const int* __restrict__ cidx, const T* __restrict__ val, const T* __restrict__ xr,
+    T* __restrict__ y, const T* __restrict__ y0, T* __restrict__ yh,
+    const T* __restrict__ sigma_base, const T* __restrict__ rl, const T* __restrict__ ru,
+    const IterScalars* __restrict__ sc, int k) {
+    const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int w = static_cast<int>(gid / kWave), lane = static_cast<int>(gid % kWave);
+    if (w >= count) return;
+    const int i = list[w];
+    const T axr = groupDotBE<kWave>(rstarts[i], rstarts[i + 1], cidx, val, xr, lane);
+    if (lane != 0) return;
+    dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, sc, k);
 }
 
 // ---- evaluation (src/lp/pdlp.cpp:1076-1202) --------------------------------

@@ -293,6 +293,10 @@ private:
     double sgn_;
     bool mixed_;
     int gr_, gc_;
+    // Long columns / rows handled one wave each (counts 0 when not used).
+    int *long_c_ = nullptr, *long_r_ = nullptr;
+    int n_long_c_ = 0, n_long_r_ = 0;
+    int skip_c_ = std::numeric_limits<int>::max(), skip_r_ = std::numeric_limits<int>::max();
     Stream stream_;
     Arena arena_;
     // structure and original data
@@ -349,6 +353,7 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     bytes += padBytes(static_cast<size_t>(std::max(nbAt, 2 * nbred_)), 8);
     bytes += 3 * padBytes(std::max(n, m), 8);                 // power iteration vectors
     bytes += padBytes(1, sizeof(IterScalars)) + padBytes(1, sizeof(PowerState));
+    bytes += padBytes(n, 4) + padBytes(m, 4);                 // long column / row lists
     if (mixed_) {
         bytes += 2 * padBytes(nnz, 4);
         bytes += 8 * padBytes(n, 4) + 6 * padBytes(m, 4);
@@ -418,6 +423,36 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     // Host transpose structure.
     std::vector<int> cs, ri, perm;
     buildTransposeStructure(m_, n_, lp_.row_starts, lp_.col_indices, cs, ri, perm);
+    // Long columns / rows (more than four entries per lane of their group)
+    // go to the one-wave kernels when they carry at least 5% of the entries.
+    // Only when the group size is chosen here: a group forced by the caller
+    // (spmv_group = 1 is the reference's summation order) is kept for all.
+    auto longList = [&](const int* starts, int count, int g, std::vector<int>& list) {
+        if (o_.spmv_group > 0) return std::numeric_limits<int>::max();
+        const int thresh = 4 * g;
+        long long inLong = 0;
+        for (int t = 0; t < count; ++t)
+            if (starts[t + 1] - starts[t] > thresh) {
+                list.push_back(t);
+                inLong += starts[t + 1] - starts[t];
+            }
+        if (g >= kWave || list.empty() || 20 * inLong < static_cast<long long>(nnz_)) {
+            list.clear();
+            return std::numeric_limits<int>::max();
+        }
+        return thresh;
+    };
+    std::vector<int> longC, longR;
+    skip_c_ = longList(cs.data(), n_, gc_, longC);
+    skip_r_ = longList(lp_.row_starts, m_, gr_, longR);
+    n_long_c_ = static_cast<int>(longC.size());
+    n_long_r_ = static_cast<int>(longR.size());
+    long_c_ = arena_.take<int>(std::max<size_t>(1, longC.size()));
+    long_r_ = arena_.take<int>(std::max<size_t>(1, longR.size()));
+    if (n_long_c_)
+        MPX_HIP(hipMemcpyAsync(long_c_, longC.data(), longC.size() * 4, hipMemcpyHostToDevice, st));
+    if (n_long_r_)
+        MPX_HIP(hipMemcpyAsync(long_r_, longR.data(), longR.size() * 4, hipMemcpyHostToDevice, st));
     MPX_HIP(hipMemcpyAsync(rs_, lp_.row_starts, (m + 1) * 4, hipMemcpyHostToDevice, st));
     if (nnz) {
         MPX_HIP(hipMemcpyAsync(ci_, lp_.col_indices, nnz * 4, hipMemcpyHostToDevice, st));
@@ -622,22 +657,45 @@ void SingleSolver::launchIteration(int k, bool storeHat) {
         if (storeHat)
             kPrimalStep<G, T, true><<<gridC, kBlock, 0, st>>>(n_, cs_, ri_, v.valt, v.y, v.x, v.x0,
                                                              v.xr, v.xh, v.c, v.tau, v.l, v.u,
-                                                             sc_dev_, k);
+                                                             sc_dev_, k, skip_c_);
         else
             kPrimalStep<G, T, false><<<gridC, kBlock, 0, st>>>(n_, cs_, ri_, v.valt, v.y, v.x,
                                                               v.x0, v.xr, v.xh, v.c, v.tau, v.l,
-                                                              v.u, sc_dev_, k);
+                                                              v.u, sc_dev_, k, skip_c_);
     });
+    if (n_long_c_) {
+        const int g = ceilDiv(static_cast<long long>(n_long_c_) * kWave, kBlock);
+        if (storeHat)
+            kPrimalStepLong<T, true><<<g, kBlock, 0, st>>>(n_long_c_, long_c_, cs_, ri_, v.valt,
+                                                           v.y, v.x, v.x0, v.xr, v.xh, v.c, v.tau,
+                                                           v.l, v.u, sc_dev_, k);
+        else
+            kPrimalStepLong<T, false><<<g, kBlock, 0, st>>>(n_long_c_, long_c_, cs_, ri_, v.valt,
+                                                            v.y, v.x, v.x0, v.xr, v.xh, v.c,
+                                                            v.tau, v.l, v.u, sc_dev_, k);
+    }
     withGroup(gr_, [&](auto GI) {
         constexpr int G = decltype(GI)::value;
         if (storeHat)
             kDualStep<G, T, true><<<gridR, kBlock, 0, st>>>(m_, rs_, ci_, v.val, v.xr, v.y, v.y0,
-                                                           v.yh, v.sig, v.rl, v.ru, sc_dev_, k);
+                                                           v.yh, v.sig, v.rl, v.ru, sc_dev_, k,
+                                                           skip_r_);
         else
             kDualStep<G, T, false><<<gridR, kBlock, 0, st>>>(m_, rs_, ci_, v.val, v.xr, v.y,
                                                             v.y0, v.yh, v.sig, v.rl, v.ru,
-                                                            sc_dev_, k);
+                                                            sc_dev_, k, skip_r_);
     });
+    if (n_long_r_) {
+        const int g = ceilDiv(static_cast<long long>(n_long_r_) * kWave, kBlock);
+        if (storeHat)
+            kDualStepLong<T, true><<<g, kBlock, 0, st>>>(n_long_r_, long_r_, rs_, ci_, v.val, v.xr,
+                                                         v.y, v.y0, v.yh, v.sig, v.rl, v.ru,
+                                                         sc_dev_, k);
+        else
+            kDualStepLong<T, false><<<g, kBlock, 0, st>>>(n_long_r_, long_r_, rs_, ci_, v.val,
+                                                          v.xr, v.y, v.y0, v.yh, v.sig, v.rl,
+                                                          v.ru, sc_dev_, k);
+    }
 }
 
 template <typename T>

@@ -39,7 +39,7 @@ def _drain(srv, tickets, submit_at=None):
 def _same(a, b, tag):
     ra, xa, ya = a
     rb, xb, yb = b
-    assert ra["status"] == rb["status"], (tag, ra, rb)
+    assert ra["status"] == rb["status"], (tag, ra, rb, native.last_error())
     assert ra["iterations"] == rb["iterations"], (tag, ra["iterations"], rb["iterations"])
     assert ra["objective"] == rb["objective"], (tag, ra["objective"], rb["objective"])
     assert np.array_equal(xa, xb) and np.array_equal(ya, yb), tag
