@@ -287,7 +287,10 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
     int curg = -1;
     ILane ln{1.0, 1.0, 0, 0};
     forGroupItems(nslots, per_group, [&](int g, int jb) {
-        const int j0 = (jb * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW;
+        // Wave-uniform column: in a scalar register, so the column's matrix
+        // entries, c and tau come through the scalar cache.
+        const int j0 = __builtin_amdgcn_readfirstlane(
+            (jb * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW);
         const int lane = static_cast<int>(threadIdx.x) % kLanes;
         if (j0 >= n) return;
         const int s = g * kLanes + lane;
@@ -347,7 +350,8 @@ __global__ __launch_bounds__(kBlock) void kILDual(
     int curg = -1;
     ILane ln{1.0, 1.0, 0, 0};
     forGroupItems(nslots, per_group, [&](int g, int ib) {
-        const int i0 = (ib * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW;
+        const int i0 = __builtin_amdgcn_readfirstlane(
+            (ib * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW);
         const int lane = static_cast<int>(threadIdx.x) % kLanes;
         if (i0 >= m) return;
         const int s = g * kLanes + lane;

@@ -117,7 +117,8 @@ __global__ __launch_bounds__(kBlock) void kPrimalStepLong(
     const T* __restrict__ c, const T* __restrict__ tau_base, const T* __restrict__ lo,
     const T* __restrict__ hi, const IterScalars* __restrict__ sc, int k) {
     const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    const int w = static_cast<int>(gid / kWave), lane = static_cast<int>(gid % kWave);
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(gid / kWave));
+    const int lane = static_cast<int>(gid % kWave);
     if (w >= count) return;
     const int j = list[w];
     const T aty = groupDotBE<kWave>(cstarts[j], cstarts[j + 1], ridx, valt, y, lane);
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(kBlock) void kDualStepLong(
     const T* __restrict__ sigma_base, const T* __restrict__ rl, const T* __restrict__ ru,
     const IterScalars* __restrict__ sc, int k) {
     const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    const int w = static_cast<int>(gid / kWave), lane = static_cast<int>(gid % kWave);
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(gid / kWave));
+    const int lane = static_cast<int>(gid % kWave);
     if (w >= count) return;
     const int i = list[w];
     const T axr = groupDotBE<kWave>(rstarts[i], rstarts[i + 1], cidx, val, xr, lane);

@@ -1291,10 +1291,25 @@ int mpx_device_count(int32_t* count) {
     return MPX_OK;
 }
 
-__global__ void kCopyBench(long long n4, const int4* __restrict__ src, int4* __restrict__ dst) {
-    for (long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; t < n4;
-         t += static_cast<long long>(gridDim.x) * kBlock)
-        dst[t] = src[t];
+// Streaming copy: four 16-byte loads in flight per thread, non-temporal
+// (the data is touched once); the tail is copied one element at a time.
+using V4i = int __attribute__((ext_vector_type(4)));
+__global__ void kCopyBench(long long n4, const int4* __restrict__ src4, int4* __restrict__ dst4) {
+    const V4i* src = reinterpret_cast<const V4i*>(src4);
+    V4i* dst = reinterpret_cast<V4i*>(dst4);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    for (; t + 3 * stride < n4; t += 4 * stride) {
+        const V4i a = __builtin_nontemporal_load(src + t);
+        const V4i b = __builtin_nontemporal_load(src + t + stride);
+        const V4i c = __builtin_nontemporal_load(src + t + 2 * stride);
+        const V4i d = __builtin_nontemporal_load(src + t + 3 * stride);
+        __builtin_nontemporal_store(a, dst + t);
+        __builtin_nontemporal_store(b, dst + t + stride);
+        __builtin_nontemporal_store(c, dst + t + 2 * stride);
+        __builtin_nontemporal_store(d, dst + t + 3 * stride);
+    }
+    for (; t < n4; t += stride) dst[t] = src[t];
 }
 
 int mpx_copy_bench(int64_t bytes, double* ms) {
