@@ -93,4 +93,12 @@ __device__ inline void blockReduce(double (&v)[NF], const bool (&isMax)[NF], dou
     __syncthreads();
 }
 
+// A pointer loaded from memory (a descriptor field) is generic to the
+// compiler, which then emits flat loads; every such pointer here refers to
+// device global memory, and this cast says so (global loads and stores).
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T* asGlobal(T* p) {
+    return (__attribute__((address_space(1))) T*)(p);
+}
+
 }  // namespace mpx

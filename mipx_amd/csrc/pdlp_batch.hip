@@ -167,8 +167,18 @@ struct PRec {
 constexpr int kPackCW = 4;
 
 template <int G, typename T>
-__device__ inline void primalTile(const PRec<T>& r, int first, int kiter) {
-    if (kiter > r.lim) return;
+__device__ inline void primalTile(const PRec<T>& rec, int first, int kiter) {
+    if (kiter > rec.lim) return;
+    struct {
+        decltype(asGlobal(rec.cs)) cs;
+        decltype(asGlobal(rec.ri)) ri;
+        decltype(asGlobal(rec.valt)) valt, c, tau, l, u, x0;
+        decltype(asGlobal(rec.x)) y, x, xr, xh;
+        T step, pw;
+        int32_t inner, lim, n;
+    } r{asGlobal(rec.cs), asGlobal(rec.ri), asGlobal(rec.valt), asGlobal(rec.c), asGlobal(rec.tau),
+        asGlobal(rec.l), asGlobal(rec.u), asGlobal(rec.x0), asGlobal(rec.y), asGlobal(rec.x),
+        asGlobal(rec.xr), asGlobal(rec.xh), rec.step, rec.pw, rec.inner, rec.lim, rec.n};
     constexpr int per = kBlock / G;
     const int lane = static_cast<int>(threadIdx.x) % G, grp = static_cast<int>(threadIdx.x) / G;
     int j[kPackCW], b[kPackCW], cnt[kPackCW];
@@ -211,8 +221,18 @@ __device__ inline void primalTile(const PRec<T>& r, int first, int kiter) {
 }
 
 template <int G, typename T>
-__device__ inline void dualTile(const PRec<T>& r, int first, int kiter) {
-    if (kiter > r.lim) return;
+__device__ inline void dualTile(const PRec<T>& rec, int first, int kiter) {
+    if (kiter > rec.lim) return;
+    struct {
+        decltype(asGlobal(rec.rs)) rs;
+        decltype(asGlobal(rec.ci)) ci;
+        decltype(asGlobal(rec.val)) val, sig, rl, ru, y0;
+        decltype(asGlobal(rec.xr)) xr, y, yh;
+        T step, pw;
+        int32_t inner, lim, m;
+    } r{asGlobal(rec.rs), asGlobal(rec.ci), asGlobal(rec.val), asGlobal(rec.sig), asGlobal(rec.rl),
+        asGlobal(rec.ru), asGlobal(rec.y0), asGlobal(rec.xr), asGlobal(rec.y), asGlobal(rec.yh),
+        rec.step, rec.pw, rec.inner, rec.lim, rec.m};
     constexpr int per = kBlock / G;
     const int lane = static_cast<int>(threadIdx.x) % G, grp = static_cast<int>(threadIdx.x) / G;
     int i[kPackCW], b[kPackCW], cnt[kPackCW];
