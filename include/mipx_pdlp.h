@@ -242,6 +242,9 @@ typedef struct mpx_batch_stats {
     double dual_kernel_us;
     int32_t profiled_slots;    /* instances active in the profiled launches */
     int32_t profiled_launches;
+    int32_t columns_per_wave;  /* lane-per-instance kernels: autotuned columns
+                                  per wave (primal), 0 before the first run */
+    int32_t rows_per_wave;     /* ... rows per wave (dual) */
 } mpx_batch_stats;
 
 int mpx_batch_create_shared(const mpx_lp* base, int32_t count, const double* col_lower,

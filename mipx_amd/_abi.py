@@ -215,6 +215,8 @@ class MpxBatchStats(C.Structure):
         ("dual_kernel_us", C.c_double),
         ("profiled_slots", C.c_int32),
         ("profiled_launches", C.c_int32),
+        ("columns_per_wave", C.c_int32),
+        ("rows_per_wave", C.c_int32),
     ]
 
     def as_dict(self):

@@ -1151,27 +1151,43 @@ void ilGather(hipStream_t st, int grid, const BInst* di, int K, const BOwner& w,
                                                  nullptr, nullptr);
 }
 
-// Iteration k (primal then dual; FP32 instances first).
+// Iteration k (primal then dual; FP32 instances first) with `cwp` columns
+// per wave in the primal kernels and `cwd` rows per wave in the dual ones.
 void ilIterate(hipStream_t st, int grid, const BInst* di, const BOwner& w, IlSet& s, bool f32,
-               int cw, int k) {
+               int cwp, int cwd, int k) {
     const int n = w.n, m = w.m;
-    withCW(cw, [&](auto CWI) {
-        constexpr int CW = decltype(CWI)::value;
-        if (f32) {
-            kILPrimal<float, CW><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, s.lanes32, n, m,
-                                                          w.cs, w.ri, w.valtf, w.cf, w.tauf, s.f.y,
-                                                          s.f.x, s.f.x0, s.f.xr, s.f.l, s.f.u, k);
-            kILDual<float, CW><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, s.lanes32, n, m,
-                                                        w.rs, w.ci, w.valf, w.sigf, w.rlf, w.ruf,
-                                                        s.f.xr, s.f.y, s.f.y0, k);
-        }
-        kILPrimal<double, CW><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, s.lanes64, n,
-                                                       m, w.cs, w.ri, w.valt, w.c, w.tau, s.d.y,
-                                                       s.d.x, s.d.x0, s.d.xr, s.d.l, s.d.u, k);
-        kILDual<double, CW><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, s.lanes64, n, m,
-                                                     w.rs, w.ci, w.val, w.sig, w.rl, w.ru, s.d.xr,
-                                                     s.d.y, s.d.y0, k);
-    });
+    auto primal = [&](bool fp32) {
+        withCW(cwp, [&](auto CWI) {
+            constexpr int CW = decltype(CWI)::value;
+            if (fp32)
+                kILPrimal<float, CW><<<grid, kBlock, 0, st>>>(
+                    di, s.slots32, s.counts, s.lanes32, n, m, w.cs, w.ri, w.valtf, w.cf, w.tauf,
+                    s.f.y, s.f.x, s.f.x0, s.f.xr, s.f.l, s.f.u, k);
+            else
+                kILPrimal<double, CW><<<grid, kBlock, 0, st>>>(
+                    di, s.slots64, s.counts + 1, s.lanes64, n, m, w.cs, w.ri, w.valt, w.c, w.tau,
+                    s.d.y, s.d.x, s.d.x0, s.d.xr, s.d.l, s.d.u, k);
+        });
+    };
+    auto dual = [&](bool fp32) {
+        withCW(cwd, [&](auto CWI) {
+            constexpr int CW = decltype(CWI)::value;
+            if (fp32)
+                kILDual<float, CW><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, s.lanes32, n,
+                                                            m, w.rs, w.ci, w.valf, w.sigf, w.rlf,
+                                                            w.ruf, s.f.xr, s.f.y, s.f.y0, k);
+            else
+                kILDual<double, CW><<<grid, kBlock, 0, st>>>(
+                    di, s.slots64, s.counts + 1, s.lanes64, n, m, w.rs, w.ci, w.val, w.sig, w.rl,
+                    w.ru, s.d.xr, s.d.y, s.d.y0, k);
+        });
+    };
+    if (f32) {
+        primal(true);
+        dual(true);
+    }
+    primal(false);
+    dual(false);
 }
 
 // Interleaved -> per-instance x and y after the round's last iteration.
@@ -1283,6 +1299,8 @@ public:
     double iterMs() const { return iter_ms_; }
     int roundsTimed() const { return rounds_timed_; }
     IterPreset preset() const { return cur_; }
+    int tunedColumnsPerWave() const { return interleaved_ ? tuned_.gc : 0; }
+    int tunedRowsPerWave() const { return interleaved_ ? tuned_.gr : 0; }
 
 private:
     struct G {
@@ -1334,8 +1352,10 @@ private:
     unsigned long long* stamp_ = nullptr;  // device wall clock at the end of prepare()
     BCtrl* ctrl_all_ = nullptr;
     bool allow_interleaved_ = false;
-    int cw_ = 1;  // columns per wave of the lane-per-instance kernels
-    int tuned_cw_ = 0;
+    // Lane-per-instance kernels: columns per wave (primal) and rows per wave
+    // (dual) in use, and the autotuned pair ({0, 0} before tuning).
+    int cwp_ = 1, cwd_ = 1;
+    IterPreset tuned_{0, 0};
     bool fp32_gone_ = false;  // no instance left in its FP32 phase
     // Lane-per-instance mode (every instance shares owner 0).
     bool interleaved_ = false;
@@ -1416,10 +1436,11 @@ void BatchEngine::setup() {
     grid_ = std::max(kNumXcd, (cus * 8 / kNumXcd) * kNumXcd);
     // Online autotuning of the lane-per-instance kernels: columns per wave.
     if (interleaved_) {
-        if (tuned_cw_ > 0) {
-            cur_ = {tuned_cw_, tuned_cw_};
+        if (tuned_.gc > 0) {
+            cur_ = tuned_;
         } else {
-            tune_left_ = {{4, 4}, {2, 2}, {1, 1}};
+            // (columns per wave, rows per wave); tried from the back.
+            tune_left_ = {{4, 4}, {2, 2}, {2, 1}, {4, 1}, {1, 1}};
             cur_ = {1, 1};
         }
     } else {
@@ -1563,7 +1584,7 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     const bool f32 = mixed_ && !fp32_gone_;
     ilGather(st_, grid_, di, K, w, il_, f32);
     if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
-    for (int k = 1; k <= bo_.N; ++k) ilIterate(st_, grid_, di, w, il_, f32, cw_, k);
+    for (int k = 1; k <= bo_.N; ++k) ilIterate(st_, grid_, di, w, il_, f32, cwp_, cwd_, k);
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
     ilScatter(st_, grid_, di, w, il_, f32);
     if (f32) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
@@ -1585,7 +1606,7 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
         ilGather(st_, grid_, di, K, w, il_, f32);
         for (int k = 1; k <= launches; ++k) {
             MPX_HIP(hipEventRecord(e[0], st_));
-            withCW(cw_, [&](auto CWI) {
+            withCW(cwp_, [&](auto CWI) {
                 constexpr int CW = decltype(CWI)::value;
                 if (f32)
                     kILPrimal<float, CW><<<grid_, kBlock, 0, st_>>>(
@@ -1597,7 +1618,7 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
                         w.c, w.tau, il_.d.y, il_.d.x, il_.d.x0, il_.d.xr, il_.d.l, il_.d.u, k);
             });
             MPX_HIP(hipEventRecord(e[1], st_));
-            withCW(cw_, [&](auto CWI) {
+            withCW(cwd_, [&](auto CWI) {
                 constexpr int CW = decltype(CWI)::value;
                 if (f32)
                     kILDual<float, CW><<<grid_, kBlock, 0, st_>>>(
@@ -1667,7 +1688,10 @@ std::vector<double> BatchEngine::finishTimesMs() {
 }
 
 void BatchEngine::launchRound(const IterPreset& p, bool timed) {
-    if (interleaved_) cw_ = p.gc > 0 ? p.gc : 1;
+    if (interleaved_) {
+        cwp_ = p.gc > 0 ? p.gc : 1;
+        cwd_ = p.gr > 0 ? p.gr : 1;
+    }
     if (timed || graph_failed_) {
         if (interleaved_) enqueueInterleavedRound(timed);
         else enqueueRound(p, timed);
@@ -1718,7 +1742,7 @@ int BatchEngine::step(int rounds) {
                 std::sort(tune_res_.begin(), tune_res_.end(),
                           [](const auto& a, const auto& b) { return a.first < b.first; });
                 cur_ = tune_res_.front().second;
-                if (interleaved_) tuned_cw_ = cur_.gc;
+                if (interleaved_) tuned_ = cur_;
             }
             continue;
         }
@@ -2503,7 +2527,9 @@ private:
             }
             kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tc64_, totals_ + 2, k);
             kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
-            for (auto& b : bases_) ilIterate(st_, grid_, di, b->owner, b->il, f32, 1, k);
+            // Four columns per wave in the primal, one row per wave in the
+            // dual: the pair the batched engine's autotuner settles on.
+            for (auto& b : bases_) ilIterate(st_, grid_, di, b->owner, b->il, f32, 4, 1, k);
         }
         for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, f32);
         if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
@@ -2906,6 +2932,8 @@ int mpx_batch_run(mpx_batch* b) {
         b->stats.latency_p50_ms = pct(ft, 0.50);
         b->stats.latency_p99_ms = pct(ft, 0.99);
         b->stats.latency_max_ms = pct(ft, 1.0);
+        b->stats.columns_per_wave = b->eng->tunedColumnsPerWave();
+        b->stats.rows_per_wave = b->eng->tunedRowsPerWave();
     } catch (const DeviceError&) {
         return MPX_ERR_DEVICE;
     }
@@ -2946,6 +2974,8 @@ int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats) {
         b->stats.dual_kernel_us = d;
         b->stats.profiled_slots = slots;
         b->stats.profiled_launches = std::min(launches, b->opts.termination_eval_frequency);
+        b->stats.columns_per_wave = b->eng->tunedColumnsPerWave();
+        b->stats.rows_per_wave = b->eng->tunedRowsPerWave();
         b->stats.rounds = 1;  // state is mid-run: the next run re-prepares
     } catch (const DeviceError&) {
         return MPX_ERR_DEVICE;
