@@ -343,9 +343,9 @@ def main():
         return
     log(f"building workload {a.workload} count={a.count}")
     lps = None
-    # --profile-only: four rounds are enough for the columns-per-wave
+    # --profile-only: six rounds are enough for the columns-per-wave
     # autotuning that precedes the profiled launches.
-    cap = dict(max_iter=4 * 200) if a.profile_only else {}
+    cap = dict(max_iter=6 * 200) if a.profile_only else {}
     if a.workload == "c3":
         lps = c3_workload(a.count)
         lp = lps[0]
@@ -359,8 +359,9 @@ def main():
     if a.profile_only:
         # A capped solve first: it autotunes the lane-per-instance kernels'
         # columns per wave, which the profiled launches then use (as in the
-        # bench).
-        batch.run()
+        # bench).  Packed batches have nothing to tune.
+        if lps is None:
+            batch.run()
         prof = batch.profile(launches=a.launches)
         sv = 4 if prec == 1 else 8
         bl = (packed_primal_bytes(lps, sv) if lps is not None

@@ -2966,7 +2966,7 @@ int mpx_batch_get_stats(mpx_batch* b, mpx_batch_stats* stats) {
 int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats) {
     if (!b || launches <= 0) return MPX_ERR_INVALID;
     try {
-        b->eng->prepare();
+        if (b->eng->rounds() > 0) b->eng->prepare();  // fresh after create otherwise
         double p = 0.0, d = 0.0;
         int slots = 0;
         b->eng->profileRound(launches, &p, &d, &slots);
