@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--handover", type=float, default=0.0,
+                    help="mixed precision: FP32 -> FP64 handover KKT ratio (0: library default)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -346,6 +348,8 @@ def main():
     # --profile-only: six rounds are enough for the columns-per-wave
     # autotuning that precedes the profiled launches.
     cap = dict(max_iter=6 * 200) if a.profile_only else {}
+    if a.handover > 0:
+        cap["mixed_handover"] = a.handover
     if a.workload == "c3":
         lps = c3_workload(a.count)
         lp = lps[0]
@@ -461,6 +465,7 @@ def main():
                        "max": round(st["latency_max_ms"], 2)},
         "status_counts": status,
         "mean_iterations": float(iters.mean()),
+        "mean_fp32_iterations": float(np.mean([r["iterations_fp32"] for r in res])),
         "plain_mean_iterations": float(np.mean(plain_iters)),
         "max_rel_kkt": kkt,
         "setup_ms": round(st["setup_ms"], 2),

@@ -15,9 +15,9 @@
 // over its own tiles.  Tiles never straddle owners, so an owner's results do
 // not depend on which other owners share the cohort.
 //
-// The power iteration runs in captured groups of kPowerGroup steps; every
-// kernel skips owners that have stopped, so extra steps change nothing.  The
-// host checks the number of unfinished owners after each group.
+// The power iteration runs in groups of kPowerGroup steps; every kernel
+// skips owners that have stopped, so extra steps change nothing.  The host
+// checks the number of unfinished owners after each group.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -547,9 +547,9 @@ public:
     }
 
     // Uploads the cohort tables and runs the whole setup on `st`; the power
-    // iteration is captured in graphs unless `graphs` is false (a caller on
-    // a second host thread launches directly, so that no two threads capture
-    // at once).
+    // iteration is captured in graphs only if `graphs` is true (the engines
+    // launch directly: sixteen launches per host check cost about what a
+    // graph launch does, and no two host threads ever capture at once).
     void run(hipStream_t st, bool graphs = true) {
         const int k = count();
         if (k == 0) return;

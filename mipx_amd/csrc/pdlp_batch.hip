@@ -1472,7 +1472,7 @@ void BatchEngine::prepare() {
     graphs_.clear();
     rounds_ = 0;
     fp32_gone_ = false;
-    if (no) cohort_->run(st_);
+    if (no) cohort_->run(st_, false);
     if (ni)
         kInstInit<<<ni, kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
                                           static_cast<const BInitArgs*>(d_args_buf_->p), bo_,
@@ -2138,7 +2138,7 @@ public:
             throw OutOfMemory();
         }
         cohort.carve(static_cast<char*>(scratch));
-        cohort.run(st_);
+        cohort.run(st_, false);
         pool_->release(scratch);
         bases_.push_back(std::move(b));
         // The round graphs include every base's lane-per-instance kernels.
