@@ -153,7 +153,7 @@ typedef struct mpx_pdlp_result {
     double rel_gap;          /* last evaluated relative gap */
     double primal_obj;       /* last evaluated primal objective (original units) */
     double dual_obj;         /* last evaluated dual objective (original units) */
-    double step;             /* 0.998 / ||A_scaled||_2 */
+    double step;             /* final step: 0.998 / ||A_scaled||_2, moved by adaptive_step */
     double primal_weight;    /* final primal weight */
     int32_t restarts;
     int32_t iterations_fp32; /* MPX_PREC_MIXED: iterations run in FP32 */
@@ -185,8 +185,13 @@ int mpx_copy_bench(int64_t bytes, double* ms);
  * the recovered primal and dual (on Infeasible: the Farkas ray in y_out; on
  * Unbounded: the primal ray in x_out; pdlp.cpp:2318-2340); either may be NULL.
  * The trivial n == 0 and m == 0 cases are handled in closed form
- * (pdlp.cpp:812-860).  Returns MPX_OK unless the arguments are invalid; a
- * device failure is reported as MPX_STATUS_ERROR in `res`.
+ * (pdlp.cpp:812-860).  The non-default options behave as in the GPU host
+ * loop: adaptive_step adapts at each evaluation from x^ and A x^
+ * (pdlp.cpp:2271-2307), anderson_acceleration extrapolates at main-loop
+ * restarts (293-437, 2418-2466; window at most 64), matrix_zero_tol drops
+ * |a| < tol before scaling (499-518; certificates use the LP as given).
+ * Returns MPX_OK unless the arguments are invalid; a device failure is
+ * reported as MPX_STATUS_ERROR in `res`.
  */
 int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts,
                    const double* warm_x, const double* warm_y,
@@ -198,9 +203,12 @@ int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts,
  * relaxations (the reference's caller: mip_solver.cpp node LPs via
  * setColBounds, pdlp.cpp:2890-2898).  col_lower/col_upper are [count][num_cols]
  * row-major (NULL = use base->col_lower / col_upper for every instance).
- * All instances run in one persistent kernel with per-instance restart,
- * primal-weight and termination logic on the device.  x_out is
- * [count][num_cols], y_out [count][num_rows] (either may be NULL); results[count].
+ * All instances iterate in the same launches (lane-per-instance kernels that
+ * read A once for 64 instances) with per-instance restart, primal-weight and
+ * termination logic on the device.  x_out is [count][num_cols], y_out
+ * [count][num_rows] (either may be NULL); results[count].  The batched and
+ * serving entry points return MPX_ERR_UNSUPPORTED for adaptive_step,
+ * anderson_acceleration and matrix_zero_tol (single-LP options only).
  */
 int mpx_pdlp_solve_batch_shared(const mpx_lp* base, int32_t count,
                                 const double* col_lower, const double* col_upper,

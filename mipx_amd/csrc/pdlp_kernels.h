@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void kEvalRow(
                                                     double* __restrict__ out) {
     __shared__ double red[kBlock];
     for (int f = 0; f < nf; ++f) {
-        const bool mx = (maxMask >> f) & 1u;
+        const bool mx = f < 32 && ((maxMask >> f) & 1u);
         double a = 0.0;
         for (int b = threadIdx.x; b < nblocks; b += kBlock) {
             const double p = partials[f * nblocks + b];

@@ -35,6 +35,8 @@
 
 #include "../../include/mipx_pdlp.h"
 #include "device_common.h"
+#include "lp_filter.h"
+#include "pdlp_accel.h"
 #include "pdlp_kernels.h"
 #include "pdlp_setup.h"
 
@@ -227,6 +229,10 @@ bool rayHolds(const mpx_lp& lp, const std::vector<double>& x, double tol) {
 }
 
 // ---- the solver ------------------------------------------------------------
+// Largest Anderson window accepted (the Gram pass launches one grid row per
+// pair, w (w + 1) / 2 of them).
+constexpr int kMaxAndersonWindow = 64;
+
 struct EvalOut {
     double fpe, rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj, dx2, dy2;
 };
@@ -242,7 +248,10 @@ struct IterVecs {
 
 class SingleSolver {
 public:
-    SingleSolver(const mpx_lp& lp, const mpx_pdlp_options& o) : lp_(lp), o_(o) {
+    // `orig` (default: `lp`) is the problem as given, for the certificate
+    // checks, when `lp` is its copy without near-zero entries.
+    SingleSolver(const mpx_lp& lp, const mpx_pdlp_options& o, const mpx_lp* orig = nullptr)
+        : lp_(lp), orig_(orig ? *orig : lp), o_(o) {
         m_ = lp.num_rows;
         n_ = lp.num_cols;
         nnz_ = lp.row_starts[m_];
@@ -250,6 +259,8 @@ public:
         mixed_ = o.precision == MPX_PREC_MIXED;
         gr_ = pickGroup(nnz_, m_, o.spmv_group);
         gc_ = pickGroup(nnz_, n_, o.spmv_group);
+        adaptive_ = o.adaptive_step != 0;
+        aa_w_ = (o.anderson_acceleration && o.anderson_window_size >= 2) ? o.anderson_window_size : 0;
     }
 
     // Device setup: arena, uploads, scaling, power iteration.
@@ -286,8 +297,13 @@ private:
     void powerIteration();
     void convertIterates64to32();
     void convertIterates32to64();
+    template <typename T>
+    void adaptStep(double lb, double ub);
+    template <typename T>
+    bool andersonRestart();
 
     const mpx_lp& lp_;
+    const mpx_lp& orig_;
     const mpx_pdlp_options& o_;
     int m_, n_, nnz_;
     double sgn_;
@@ -319,6 +335,16 @@ private:
     double pw_ = 1.0;
     int inner_ = 0;
     double cnorm_ = 0.0, bnorm_ = 0.0;
+    // Adaptive step of the GPU host loop (pdlp.cpp:2271-2307): previous x^
+    // and A x^, reduction partials and the two sums.
+    bool adaptive_ = false, have_prev_ = false;
+    double *prev_ax_ = nullptr, *prev_x_ = nullptr, *acc_part_ = nullptr, *acc_out_ = nullptr;
+    int nbn_ = 1;
+    // Anderson acceleration (pdlp.cpp:293-437): window of aa_w_ restart
+    // candidates and residuals in HBM, aa_count_ stored so far.
+    int aa_w_ = 0, aa_count_ = 0, nbaa_ = 1;
+    double *aa_z_ = nullptr, *aa_r_ = nullptr, *aa_alpha_ = nullptr;
+    std::unique_ptr<PinnedHost> acc_pinned_;
 };
 
 template <>
@@ -354,6 +380,15 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     bytes += 3 * padBytes(std::max(n, m), 8);                 // power iteration vectors
     bytes += padBytes(1, sizeof(IterScalars)) + padBytes(1, sizeof(PowerState));
     bytes += padBytes(n, 4) + padBytes(m, 4);                 // long column / row lists
+    nbn_ = std::max(1, ceilDiv(n_, kBlock));
+    const size_t dim = n + m;
+    nbaa_ = std::max(1, std::min(1024, ceilDiv(static_cast<long long>(dim), kBlock)));
+    const size_t pairs = static_cast<size_t>(aa_w_) * static_cast<size_t>(aa_w_ + 1) / 2;
+    const size_t accParts = std::max({static_cast<size_t>(nbr_), static_cast<size_t>(nbn_),
+                                      static_cast<size_t>(nbaa_) * std::max<size_t>(pairs, 1)});
+    if (adaptive_) bytes += padBytes(m, 8) + padBytes(n, 8);
+    if (aa_w_) bytes += 2 * padBytes(static_cast<size_t>(aa_w_) * dim, 8) + padBytes(aa_w_, 8);
+    if (adaptive_ || aa_w_) bytes += padBytes(accParts, 8) + padBytes(std::max<size_t>(pairs, 2), 8);
     if (mixed_) {
         bytes += 2 * padBytes(nnz, 4);
         bytes += 8 * padBytes(n, 4) + 6 * padBytes(m, 4);
@@ -397,6 +432,20 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     pcol_ = arena_.take<double>(static_cast<size_t>(nbc_) * kNumColFields);
     prow_ = arena_.take<double>(static_cast<size_t>(nbr_) * kNumRowFields);
     sc_dev_ = arena_.take<IterScalars>(1);
+    if (adaptive_) {
+        prev_ax_ = arena_.take<double>(m);
+        prev_x_ = arena_.take<double>(n);
+    }
+    if (aa_w_) {
+        aa_z_ = arena_.take<double>(static_cast<size_t>(aa_w_) * dim);
+        aa_r_ = arena_.take<double>(static_cast<size_t>(aa_w_) * dim);
+        aa_alpha_ = arena_.take<double>(aa_w_);
+    }
+    if (adaptive_ || aa_w_) {
+        acc_part_ = arena_.take<double>(accParts);
+        acc_out_ = arena_.take<double>(std::max<size_t>(pairs, 2));
+        acc_pinned_ = std::make_unique<PinnedHost>(sizeof(double) * (std::max<size_t>(pairs, 2) + aa_w_));
+    }
     if (mixed_) {
         v32_.val = arena_.take<float>(nnz);
         v32_.valt = arena_.take<float>(nnz);
@@ -548,6 +597,10 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
         MPX_HIP(hipMemsetAsync(v64_.y, 0, m * 8, st));
         MPX_HIP(hipMemsetAsync(v64_.y0, 0, m * 8, st));
         MPX_HIP(hipMemsetAsync(v64_.yh, 0, m * 8, st));
+    }
+    if (adaptive_) {
+        MPX_HIP(hipMemsetAsync(prev_ax_, 0, m * 8, st));
+        MPX_HIP(hipMemsetAsync(prev_x_, 0, n * 8, st));
     }
     if (warm_x && warm_y) {
         double* wx = cmax_;   // n scratch
@@ -899,6 +952,118 @@ void convertProblem(hipStream_t st, IterVecs<T>& d, const IterVecs<double>& s, i
     cv(d.ru, s.ru, m);
 }
 
+// Adaptive step of the GPU host loop (pdlp.cpp:2271-2307), after the
+// termination test of an evaluation: from this x^ and A x^ and the previous
+// evaluation's, local = 0.998 / sqrt(sum sigma (A dx)^2 / sum dx^2 / tau);
+// the step shrinks (adaptive_step_down) above it and grows
+// (adaptive_step_up) otherwise, within [lb, ub].  The first evaluation after
+// a restart only records its point.
+template <typename T>
+void SingleSolver::adaptStep(double lb, double ub) {
+    IterVecs<T>& v = vecs<T>();
+    hipStream_t st = stream_.s;
+    withGroup(gr_, [&](auto GI) {
+        constexpr int G = decltype(GI)::value;
+        kAdaptRows<G, T><<<nbr_, kBlock, 0, st>>>(m_, rs_, ci_, v64_.val, v.xh, v64_.sig, prev_ax_,
+                                                  acc_part_);
+    });
+    kFinalize<<<1, kBlock, 0, st>>>(acc_part_, nbr_, 1, 0u, acc_out_);
+    kAdaptCols<T><<<nbn_, kBlock, 0, st>>>(n_, v.xh, v64_.tau, prev_x_, acc_part_);
+    kFinalize<<<1, kBlock, 0, st>>>(acc_part_, nbn_, 1, 0u, acc_out_ + 1);
+    MPX_HIP(hipGetLastError());
+    if (!have_prev_) {
+        have_prev_ = true;
+        return;
+    }
+    double* h = static_cast<double*>(acc_pinned_->p);
+    MPX_HIP(hipMemcpyAsync(h, acc_out_, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipStreamSynchronize(st));
+    const double adiff = h[0], xdiff = h[1];
+    if (xdiff > 1e-30) {
+        const double local = 0.998 / std::sqrt(maxRef(adiff / xdiff, 1e-30));
+        const double ns = step_ > local ? step_ * o_.adaptive_step_down : step_ * o_.adaptive_step_up;
+        step_ = clampRef(ns, lb, ub);
+    }
+}
+
+// Anderson restart (pdlp.cpp:2418-2466 with the accelerator of 293-437):
+// stores [x^; y^] and its residual against [x0; y0], then extrapolates over
+// the newest d = min(count, window) points: Gram matrix of the residuals on
+// the device, (G + mu I) a = 1 by Cholesky on the host (mu = reg * trace /
+// d), a normalised to sum 1, accepted only if |sum a_i r_i| < |r_newest|.
+// Returns true when x0 = x and y0 = y now hold the extrapolated point
+// (x projected onto the scaled bounds); false means a plain restart is due.
+template <typename T>
+bool SingleSolver::andersonRestart() {
+    IterVecs<T>& v = vecs<T>();
+    hipStream_t st = stream_.s;
+    const long long dim = static_cast<long long>(n_) + m_;
+    const int slot = aa_count_ % aa_w_;
+    kAaPush<T><<<ceilDiv(dim, kBlock), kBlock, 0, st>>>(n_, m_, v.xh, v.yh, v.x0, v.y0,
+                                                        aa_z_ + slot * dim, aa_r_ + slot * dim);
+    ++aa_count_;
+    const int d = std::min(aa_count_, aa_w_);
+    if (d < 2) {
+        MPX_HIP(hipGetLastError());
+        return false;
+    }
+    const int first = (aa_count_ - d) % aa_w_;
+    const int pairs = d * (d + 1) / 2;
+    kAaGram<<<dim3(nbaa_, pairs), kBlock, 0, st>>>(dim, d, first, aa_w_, aa_r_, acc_part_);
+    kFinalize<<<1, kBlock, 0, st>>>(acc_part_, nbaa_, pairs, 0u, acc_out_);
+    MPX_HIP(hipGetLastError());
+    double* h = static_cast<double*>(acc_pinned_->p);
+    MPX_HIP(hipMemcpyAsync(h, acc_out_, pairs * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipStreamSynchronize(st));
+    std::vector<double> g(static_cast<size_t>(d * d));
+    for (int i = 0, p = 0; i < d; ++i)
+        for (int j = i; j < d; ++j, ++p) g[i * d + j] = g[j * d + i] = h[p];
+    const double newest = g[(d - 1) * d + (d - 1)];
+    double tr = 0.0;
+    for (int i = 0; i < d; ++i) tr += g[i * d + i];
+    const double mu = o_.anderson_regularization * tr / static_cast<double>(d);
+    for (int i = 0; i < d; ++i) g[i * d + i] += mu;
+    std::vector<double> l(static_cast<size_t>(d * d), 0.0), t(static_cast<size_t>(d)),
+        a(static_cast<size_t>(d));
+    for (int i = 0; i < d; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = g[i * d + j];
+            for (int q = 0; q < j; ++q) s -= l[i * d + q] * l[j * d + q];
+            if (i == j) {
+                if (s <= 0.0) return false;
+                l[i * d + i] = std::sqrt(s);
+            } else {
+                l[i * d + j] = s / l[j * d + j];
+            }
+        }
+    for (int i = 0; i < d; ++i) {
+        double s = 1.0;
+        for (int q = 0; q < i; ++q) s -= l[i * d + q] * t[q];
+        t[i] = s / l[i * d + i];
+    }
+    for (int i = d - 1; i >= 0; --i) {
+        double s = t[i];
+        for (int q = i + 1; q < d; ++q) s -= l[q * d + i] * a[q];
+        a[i] = s / l[i * d + i];
+    }
+    double sa = 0.0;
+    for (int i = 0; i < d; ++i) sa += a[i];
+    if (std::abs(sa) < 1e-15) return false;
+    double* ha = h + pairs;
+    for (int i = 0; i < d; ++i) ha[i] = a[i] / sa;
+    MPX_HIP(hipMemcpyAsync(aa_alpha_, ha, d * sizeof(double), hipMemcpyHostToDevice, st));
+    kAaResid<<<nbaa_, kBlock, 0, st>>>(dim, d, first, aa_w_, aa_r_, aa_alpha_, acc_part_);
+    kFinalize<<<1, kBlock, 0, st>>>(acc_part_, nbaa_, 1, 0u, acc_out_);
+    MPX_HIP(hipGetLastError());
+    MPX_HIP(hipMemcpyAsync(h, acc_out_, sizeof(double), hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipStreamSynchronize(st));
+    if (!(h[0] < newest)) return false;
+    kAaApply<T><<<ceilDiv(dim, kBlock), kBlock, 0, st>>>(n_, m_, d, first, aa_w_, aa_z_, aa_alpha_,
+                                                         v64_.l, v64_.u, v.x, v.x0, v.y, v.y0);
+    MPX_HIP(hipGetLastError());
+    return true;
+}
+
 void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double* x_out,
                          double* y_out, mpx_pdlp_result* res) {
     const mpx_pdlp_options& o = o_;
@@ -911,7 +1076,9 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
     auto stopRequested = [&]() { return o.stop_fn != nullptr && o.stop_fn(o.stop_user) != 0; };
     const double work_per_iter =
         4.0 * static_cast<double>(nnz_) + 2.0 * static_cast<double>(m_ + n_);
-    res->step = step_;
+    const double step_lb = step_ * o.adaptive_step_min, step_ub = step_ * o.adaptive_step_max;
+    have_prev_ = false;
+    aa_count_ = 0;
 
     int total = 0, restarts = 0, fp32_iters = 0;
     double fpe0 = -1.0, last_fpe = std::numeric_limits<double>::infinity();
@@ -945,10 +1112,17 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
         const double td = o.optimality_norm == 1 ? ev.rel_d_inf : ev.rel_d;
         return std::max(std::max(tp / o.primal_tol, td / o.dual_tol), ev.gap / o.optimality_tol);
     };
-    auto restartNow = [&](bool polish) {
-        if (fp32_phase) restartCopy<float>();
-        else restartCopy<double>();
-        (void)polish;
+    // A restart of the main loop may extrapolate (Anderson acceleration);
+    // the handover and the polish phase restart plainly.
+    auto restartNow = [&](bool accelerate) {
+        bool extrapolated = false;
+        if (accelerate && aa_w_)
+            extrapolated = fp32_phase ? andersonRestart<float>() : andersonRestart<double>();
+        if (!extrapolated) {
+            if (fp32_phase) restartCopy<float>();
+            else restartCopy<double>();
+        }
+        have_prev_ = false;
         inner_ = 0;
         fpe0 = -1.0;
         last_fpe = std::numeric_limits<double>::infinity();
@@ -1027,6 +1201,10 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
             converged = true;
             break;
         }
+        if (adaptive_) {
+            if (fp32_phase) adaptStep<float>(step_lb, step_ub);
+            else adaptStep<double>(step_lb, step_ub);
+        }
         // Certificates every 10 N (pdlp.cpp:2309-2341), on the host.
         if (total >= 10 * N && total % (10 * N) == 0) {
             std::vector<double> xh, yh;
@@ -1038,7 +1216,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
             std::vector<double> yo(static_cast<size_t>(m_)), xo(static_cast<size_t>(n_));
             const double inv_os = 1.0 / os_;
             for (int i = 0; i < m_; ++i) yo[i] = sgn_ * yh[i] * rsc[i] * inv_os;
-            if (farkasHolds(lp_, yo, 1e-6)) {
+            if (farkasHolds(orig_, yo, 1e-6)) {
                 status = MPX_STATUS_INFEASIBLE;
                 if (y_out) std::memcpy(y_out, yo.data(), yo.size() * 8);
                 if (x_out) std::memset(x_out, 0, static_cast<size_t>(n_) * 8);
@@ -1047,7 +1225,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
             }
             const double inv_cs = 1.0 / cs_host_;
             for (int j = 0; j < n_; ++j) xo[j] = xh[j] * csc[j] * inv_cs;
-            if (rayHolds(lp_, xo, 1e-6)) {
+            if (rayHolds(orig_, xo, 1e-6)) {
                 status = MPX_STATUS_UNBOUNDED;
                 if (x_out) std::memcpy(x_out, xo.data(), xo.size() * 8);
                 if (y_out) std::memset(y_out, 0, static_cast<size_t>(m_) * 8);
@@ -1076,7 +1254,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
                     o.do_pock_chambolle_scaling)
                     precond_pw = pw_;
             }
-            restartNow(false);
+            restartNow(true);
         }
     }
 
@@ -1084,7 +1262,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
         const double rp = o.optimality_norm == 1 ? e.rel_p_inf : e.rel_p;
         const double rd = o.optimality_norm == 1 ? e.rel_d_inf : e.rel_d;
         if (rp > o.polish_primal_tol || rd > o.polish_dual_tol) {
-            restartNow(true);
+            restartNow(false);
             int pcount = 0;
             while (pcount < o.polish_max_iter) {
                 if (timeUp() || stopRequested()) break;
@@ -1114,7 +1292,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
                 if (restart) {
                     ++restarts;
                     if (o.update_primal_weight) updatePw(e, true);
-                    restartNow(true);
+                    restartNow(false);
                 }
             }
         }
@@ -1139,6 +1317,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
     res->primal_obj = e.pobj;
     res->dual_obj = e.dobj;
     res->primal_weight = pw_;
+    res->step = step_;
     res->restarts = restarts;
     res->iterations_fp32 = fp32_iters;
 }
@@ -1359,11 +1538,9 @@ int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
         setError("mpx_pdlp_solve: termination_eval_frequency must be positive");
         return MPX_ERR_INVALID;
     }
-    if (opts->adaptive_step || (opts->anderson_acceleration && opts->anderson_window_size >= 2) ||
-        opts->matrix_zero_tol > 0.0) {
-        setError("mpx_pdlp_solve: adaptive_step, anderson_acceleration and matrix_zero_tol are "
-                 "not supported by this library");
-        return MPX_ERR_UNSUPPORTED;
+    if (opts->anderson_acceleration && opts->anderson_window_size > kMaxAndersonWindow) {
+        setError("mpx_pdlp_solve: anderson_window_size above 64");
+        return MPX_ERR_INVALID;
     }
     std::memset(res, 0, sizeof(*res));
     const int m = lp->num_rows, n = lp->num_cols;
@@ -1411,7 +1588,9 @@ int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
     }
     try {
         MPX_HIP(hipSetDevice(opts->device));
-        SingleSolver s(*lp, *opts);
+        std::unique_ptr<FilteredLp> filtered;
+        if (opts->matrix_zero_tol > 0.0) filtered = std::make_unique<FilteredLp>(*lp, opts->matrix_zero_tol);
+        SingleSolver s(filtered ? filtered->lp : *lp, *opts, lp);
         s.setup(warm_x, warm_y);
         res->setup_seconds =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1431,6 +1610,10 @@ int mpx_pdlp_scale_problem(const mpx_lp* lp, const mpx_pdlp_options* opts,
     if (!validLp(lp) || !opts || !out || lp->num_rows == 0 || lp->num_cols == 0) {
         setError("mpx_pdlp_scale_problem: invalid arguments");
         return MPX_ERR_INVALID;
+    }
+    if (opts->matrix_zero_tol > 0.0) {
+        setError("mpx_pdlp_scale_problem: matrix_zero_tol changes the structure; not supported here");
+        return MPX_ERR_UNSUPPORTED;
     }
     try {
         MPX_HIP(hipSetDevice(opts->device));

@@ -30,6 +30,7 @@ def oracle_lib():
     lib.orc_pdlp_solve.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions), C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(MpxPdlpResult)]
+    lib.orc_pdlp_solve_gpu.argtypes = lib.orc_pdlp_solve.argtypes
     lib.orc_scale_problem.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxPdlpOptions),
                                       C.POINTER(MpxScaledProblem)]
     lib.orc_pdhg_iterate.argtypes = [C.POINTER(MpxLp), C.POINTER(MpxScaledProblem), C.c_int32,
@@ -77,6 +78,11 @@ def _solve_with(fn, lp, warm=None, **opts):
 def solve(lp, warm=None, **opts):
     """CPU restatement of PdlpSolver::solve (CPU path)."""
     return _solve_with(oracle_lib().orc_pdlp_solve, lp, warm, **opts)
+
+
+def solve_gpu(lp, warm=None, **opts):
+    """The same solve with the GPU host loop's adaptive step (pdlp.cpp:2271-2307)."""
+    return _solve_with(oracle_lib().orc_pdlp_solve_gpu, lp, warm, **opts)
 
 
 def ref_solve(lp, warm=None, **opts):

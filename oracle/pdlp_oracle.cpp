@@ -19,8 +19,11 @@
 //   solve (trivial cases, loop, restart/PI,
 //          polish, recovery, work units)         src/lp/pdlp.cpp:801-1720
 //   getReducedCosts                              src/lp/pdlp.cpp:2808-2819
-// Anderson acceleration (pdlp.cpp:293-437, off by default) is not restated:
-// orc_pdlp_solve returns MPX_ERR_UNSUPPORTED when it is requested.
+//   AndersonAccelerator + its restarts           src/lp/pdlp.cpp:293-437, 1339-1378
+//   adaptive step, CPU and GPU loops             src/lp/pdlp.cpp:1004-1040, 2271-2307
+// orc_pdlp_solve follows the CPU loop; orc_pdlp_solve_gpu differs from it only
+// in the adaptive step, which the GPU loop takes from x^ and A x^ at each
+// evaluation instead of from x_r and A x_r at each iteration.
 //
 // Parity pins for this file live in tests/test_oracle.py: the reference's
 // closed-form LPs (tests/test_pdlp.cpp), the SpMV known answers
@@ -548,6 +551,102 @@ void updatePrimalWeight(Real& pw, PiState& pi, Real pdist, Real ddist, Real rel_
     }
 }
 
+// Type-I Anderson acceleration over restart points, pdlp.cpp:293-437 (the
+// accelerator) as applied at main-loop restarts, pdlp.cpp:1339-1378 (CPU) and
+// 2418-2466 (GPU, same arithmetic).  The window holds the last `w` restart
+// candidates z = [x^; y^] and residuals r = z - [x0; y0]; extrapolation
+// solves (R'R + mu I) a = 1 by Cholesky (mu = reg * trace / d), normalises
+// sum(a) = 1 and accepts sum(a_i z_i) only if its residual sum(a_i r_i) is
+// shorter than the newest one.  Every sum runs in index order.
+struct AaWindow {
+    int w = 0, count = 0;
+    size_t dim = 0;
+    Real reg = 0.0;
+    std::vector<std::vector<Real>> z, r;
+
+    void init(int window, size_t d, Real regularization) {
+        w = window;
+        dim = d;
+        reg = regularization;
+        count = 0;
+        z.assign(static_cast<size_t>(w), std::vector<Real>(dim, 0.0));
+        r.assign(static_cast<size_t>(w), std::vector<Real>(dim, 0.0));
+    }
+    void push(const Iterate& st) {
+        const size_t n = st.xh.size(), m = st.yh.size();
+        std::vector<Real>& zs = z[static_cast<size_t>(count % w)];
+        std::vector<Real>& rs = r[static_cast<size_t>(count % w)];
+        for (size_t j = 0; j < n; ++j) {
+            zs[j] = st.xh[j];
+            rs[j] = st.xh[j] - st.x0[j];
+        }
+        for (size_t i = 0; i < m; ++i) {
+            zs[n + i] = st.yh[i];
+            rs[n + i] = st.yh[i] - st.y0[i];
+        }
+        ++count;
+    }
+    bool extrapolate(std::vector<Real>& out) const {
+        const int d = std::min(count, w);
+        if (d < 2) return false;
+        auto slot = [&](int i) { return static_cast<size_t>((count - d + i) % w); };
+        std::vector<Real> g(static_cast<size_t>(d * d), 0.0);
+        for (int i = 0; i < d; ++i)
+            for (int j = i; j < d; ++j) {
+                const std::vector<Real>&a = r[slot(i)], &b = r[slot(j)];
+                Real acc = 0.0;
+                for (size_t k = 0; k < dim; ++k) acc += a[k] * b[k];
+                g[i * d + j] = g[j * d + i] = acc;
+            }
+        Real tr = 0.0;
+        for (int i = 0; i < d; ++i) tr += g[i * d + i];
+        const Real mu = reg * tr / static_cast<Real>(d);
+        for (int i = 0; i < d; ++i) g[i * d + i] += mu;
+        std::vector<Real> l(static_cast<size_t>(d * d), 0.0);
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j <= i; ++j) {
+                Real v = g[i * d + j];
+                for (int q = 0; q < j; ++q) v -= l[i * d + q] * l[j * d + q];
+                if (i == j) {
+                    if (v <= 0.0) return false;
+                    l[i * d + i] = std::sqrt(v);
+                } else {
+                    l[i * d + j] = v / l[j * d + j];
+                }
+            }
+        std::vector<Real> t(static_cast<size_t>(d)), a(static_cast<size_t>(d));
+        for (int i = 0; i < d; ++i) {
+            Real v = 1.0;
+            for (int q = 0; q < i; ++q) v -= l[i * d + q] * t[q];
+            t[i] = v / l[i * d + i];
+        }
+        for (int i = d - 1; i >= 0; --i) {
+            Real v = t[i];
+            for (int q = i + 1; q < d; ++q) v -= l[q * d + i] * a[q];
+            a[i] = v / l[i * d + i];
+        }
+        Real sa = 0.0;
+        for (int i = 0; i < d; ++i) sa += a[i];
+        if (std::abs(sa) < 1e-15) return false;
+        for (int i = 0; i < d; ++i) a[i] /= sa;
+        out.assign(dim, 0.0);
+        for (int i = 0; i < d; ++i) {
+            const std::vector<Real>& zi = z[slot(i)];
+            for (size_t k = 0; k < dim; ++k) out[k] += a[i] * zi[k];
+        }
+        Real ext = 0.0;
+        for (size_t k = 0; k < dim; ++k) {
+            Real v = 0.0;
+            for (int i = 0; i < d; ++i) v += a[i] * r[slot(i)][k];
+            ext += v * v;
+        }
+        const std::vector<Real>& rl = r[slot(d - 1)];
+        Real last = 0.0;
+        for (size_t k = 0; k < dim; ++k) last += rl[k] * rl[k];
+        return ext < last;
+    }
+};
+
 bool timeUp(std::chrono::steady_clock::time_point t0, Real limit) {
     if (limit < 0.0) return false;
     return std::chrono::duration<Real>(std::chrono::steady_clock::now() - t0).count() >= limit;
@@ -667,12 +766,15 @@ int orc_pdhg_iterate(const mpx_lp* lp, const mpx_scaled_problem* sp, int32_t ite
     return MPX_OK;
 }
 
-// The whole CPU PDLP solve, pdlp.cpp:801-1720.
-int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double* warm_x,
-                   const double* warm_y, double* x_out, double* y_out, mpx_pdlp_result* res) {
+// The whole CPU PDLP solve, pdlp.cpp:801-1720.  With gpu_path the adaptive
+// step follows the GPU host loop instead (pdlp.cpp:2271-2307: x^ and A x^ at
+// evaluation boundaries rather than x_r and A x_r every iteration); nothing
+// else differs between the two paths' results.
+static int solveImpl(const mpx_lp* lp, const mpx_pdlp_options* opts, const double* warm_x,
+                     const double* warm_y, double* x_out, double* y_out, mpx_pdlp_result* res,
+                     bool gpu_path) {
     if (!lp || !opts) return MPX_ERR_INVALID;
     const mpx_pdlp_options& o = *opts;
-    if (o.anderson_acceleration && o.anderson_window_size >= 2) return MPX_ERR_UNSUPPORTED;
     const auto t0 = std::chrono::steady_clock::now();
     const Problem p = copyProblem(*lp);
     Scaled s = buildScaled(p, o);
@@ -734,12 +836,13 @@ int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
     st.axr = st.y;
     std::vector<Real> ax(static_cast<size_t>(m)), dyv(static_cast<size_t>(m)),
         atdy(static_cast<size_t>(n)), atyh(static_cast<size_t>(n));
-    std::vector<Real> prev_axr, prev_xr;
-    if (adaptive) {
-        prev_axr.assign(static_cast<size_t>(m), 0.0);
-        prev_xr.assign(static_cast<size_t>(n), 0.0);
-    }
     bool have_prev = false;
+    AaWindow aa;
+    std::vector<Real> aa_out;
+    const bool use_aa = o.anderson_acceleration && o.anderson_window_size >= 2;
+    if (use_aa)
+        aa.init(o.anderson_window_size, static_cast<size_t>(n) + static_cast<size_t>(m),
+                o.anderson_regularization);
 
     // pdlp.cpp:915-930: warm start mapped into the scaled space.
     if (warm_x && warm_y) {
@@ -758,18 +861,24 @@ int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
     Eval e;
     auto stopRequested = [&]() { return o.stop_fn != nullptr && o.stop_fn(o.stop_user) != 0; };
 
-    auto oneIteration = [&](Real lambda) {
-        primalHalf(s, st, step, pw, lambda);
-        // pdlp.cpp:1004-1040: Malitsky-Tam style step adaptation.
-        if (adaptive && have_prev) {
+    // Malitsky-Tam style step adaptation from two consecutive points (z, A z):
+    // pdlp.cpp:1004-1040 (CPU: z = x_r every iteration) and 2271-2307 (GPU:
+    // z = x^ at every evaluation).
+    std::vector<Real> prev_az, prev_z;
+    if (adaptive) {
+        prev_az.assign(static_cast<size_t>(m), 0.0);
+        prev_z.assign(static_cast<size_t>(n), 0.0);
+    }
+    auto adaptStep = [&](const std::vector<Real>& az, const std::vector<Real>& z) {
+        if (have_prev) {
             Real adiff = 0.0;
             for (Index i = 0; i < m; ++i) {
-                const Real d = st.axr[i] - prev_axr[i];
+                const Real d = az[i] - prev_az[i];
                 adiff += s.sig[i] * d * d;
             }
             Real xdiff = 0.0;
             for (Index j = 0; j < n; ++j) {
-                const Real d = st.xr[j] - prev_xr[j];
+                const Real d = z[j] - prev_z[j];
                 xdiff += d * d / s.tau[j];
             }
             if (xdiff > 1e-30) {
@@ -779,11 +888,14 @@ int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
                 step = std::clamp(ns, step_lb, step_ub);
             }
         }
-        if (adaptive) {
-            prev_axr = st.axr;
-            prev_xr = st.xr;
-            have_prev = true;
-        }
+        prev_az = az;
+        prev_z = z;
+        have_prev = true;
+    };
+
+    auto oneIteration = [&](Real lambda) {
+        primalHalf(s, st, step, pw, lambda);
+        if (adaptive && !gpu_path) adaptStep(st.axr, st.xr);
         dualHalf(s, st, step, pw, lambda);
     };
 
@@ -809,6 +921,7 @@ int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
             converged = true;
             break;
         }
+        if (adaptive && gpu_path) adaptStep(ax, st.xh);
         // pdlp.cpp:1220-1262: certificates every 10 N iterations.  The
         // reference adds elapsed seconds * 1e-6 to the work units here.
         if (total >= 10 * N && total % (10 * N) == 0) {
@@ -851,10 +964,21 @@ int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
                     }
                 }
             }
-            st.x0 = st.xh;
-            st.x = st.xh;
-            st.y0 = st.yh;
-            st.y = st.yh;
+            bool extrapolated = false;
+            if (use_aa) {
+                aa.push(st);
+                extrapolated = aa.extrapolate(aa_out);
+            }
+            if (extrapolated) {
+                for (Index j = 0; j < n; ++j)
+                    st.x0[j] = st.x[j] = std::clamp(aa_out[static_cast<size_t>(j)], s.cl[j], s.cu[j]);
+                for (Index i = 0; i < m; ++i) st.y0[i] = st.y[i] = aa_out[static_cast<size_t>(n + i)];
+            } else {
+                st.x0 = st.xh;
+                st.x = st.xh;
+                st.y0 = st.yh;
+                st.y = st.yh;
+            }
             inner = 0;
             fpe0 = -1.0;
             last_fpe = kInf;
@@ -945,6 +1069,17 @@ int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
         res->restarts = restarts;
     }
     return MPX_OK;
+}
+
+int orc_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double* warm_x,
+                   const double* warm_y, double* x_out, double* y_out, mpx_pdlp_result* res) {
+    return solveImpl(lp, opts, warm_x, warm_y, x_out, y_out, res, false);
+}
+
+// The same solve with the GPU host loop's adaptive step (pdlp.cpp:2271-2307).
+int orc_pdlp_solve_gpu(const mpx_lp* lp, const mpx_pdlp_options* opts, const double* warm_x,
+                       const double* warm_y, double* x_out, double* y_out, mpx_pdlp_result* res) {
+    return solveImpl(lp, opts, warm_x, warm_y, x_out, y_out, res, true);
 }
 
 // Reduced costs c + A'y in original space, pdlp.cpp:2808-2819.

@@ -43,6 +43,16 @@ def test_invalid_arguments_rejected_without_gpu():
     o = _abi.make_options()
     assert lib.mpx_pdlp_solve(None, C.byref(o), None, None, None, None, C.byref(res)) == 1
     assert "invalid" in native.last_error()
+    # Option checks that return before any device work.
+    from mipx_amd import lpgen
+    lp = lpgen.c0_plain(seed=3, m=20, n=50).lp
+    clp = lp.as_c()
+    o.anderson_acceleration, o.anderson_window_size = 1, 65
+    assert lib.mpx_pdlp_solve(C.byref(clp), C.byref(o), None, None, None, None, C.byref(res)) == 1
+    assert "anderson_window_size" in native.last_error()
+    o = _abi.make_options(matrix_zero_tol=1e-9)
+    sp, _ = _abi.alloc_scaled(lp.num_rows, lp.num_cols, lp.nnz)
+    assert lib.mpx_pdlp_scale_problem(C.byref(clp), C.byref(o), C.byref(sp)) == 4
 
 
 def test_server_rejects_invalid_arguments_without_gpu():

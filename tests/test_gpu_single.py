@@ -186,3 +186,92 @@ def test_stop_callback_reports_iter_limit():
     cb = STOP_FN(stop)
     r, _, _ = native.solve(lp, stop_fn=cb, **TIGHT)
     assert r["status"] == 3 and r["iterations"] == 400
+
+
+# ---- non-default options: Anderson acceleration, adaptive step, matrix filter ----
+
+OPTIONAL = [
+    dict(anderson_acceleration=1),
+    dict(anderson_acceleration=1, anderson_window_size=3, anderson_regularization=1e-8),
+    dict(adaptive_step=1, adaptive_step_up=1.05, adaptive_step_down=0.5, max_iter=30000),
+    dict(adaptive_step=1, anderson_acceleration=1),
+]
+
+
+@pytest.mark.parametrize("group", [1, 0])
+@pytest.mark.parametrize("opts", OPTIONAL, ids=lambda o: "-".join(sorted(o)))
+def test_optional_features_vs_oracle(opts, group):
+    """Anderson acceleration at restarts and the GPU loop's adaptive step:
+    same status and objective (1e-6) as the oracle's GPU-loop solve, KKT
+    certified, iteration count tracking the oracle's."""
+    lp = lpgen.c0_plain(seed=101, m=300, n=750).lp
+    o = dict(TIGHT)
+    o.update(opts)
+    r, x, y = native.solve(lp, spmv_group=group, **o)
+    ro, _, _ = ob.solve_gpu(lp, **o)
+    print(f"gpu iters={r['iterations']} step={r['step']!r} obj={r['objective']!r}  "
+          f"oracle iters={ro['iterations']} step={ro['step']!r} obj={ro['objective']!r}")
+    assert r["status"] == ro["status"] == 0
+    assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
+    k = ob.kkt_check(lp, x, y)
+    assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
+    assert abs(r["iterations"] - ro["iterations"]) <= 0.25 * ro["iterations"]
+
+
+@pytest.mark.parametrize("N,up,down", [(200, 1.05, 0.5), (20, 3.0, 0.5), (50, 4.0, 0.25)])
+def test_adaptive_step_trajectory_vs_oracle(N, up, down):
+    """With one restart (after the first evaluation) the step after each of
+    the next five evaluations equals the oracle's GPU-loop step: the same
+    grow / shrink decisions (group 1: the reference's SpMV order)."""
+    lp = lpgen.c0_plain(seed=3, m=120, n=300).lp
+    o = dict(TIGHT, adaptive_step=1, adaptive_step_up=up, adaptive_step_down=down,
+             termination_eval_frequency=N, restart_artificial_fraction=1.0,
+             restart_sufficient_decay=0.0, restart_necessary_decay=0.0)
+    for e in range(2, 7):
+        r, _, _ = native.solve(lp, spmv_group=1, **dict(o, max_iter=e * N))
+        ro, _, _ = ob.solve_gpu(lp, **dict(o, max_iter=e * N))
+        assert r["restarts"] == ro["restarts"] == 1
+        # The power iteration's last bits differ; the grow / shrink factors
+        # applied to it must be the same.
+        assert r["step"] == pytest.approx(ro["step"], rel=1e-12), (e, r["step"], ro["step"])
+
+
+def test_adaptive_and_anderson_mixed_precision():
+    """FP32 iterations with both accelerations, FP64 refinement: KKT
+    certified and within 5e-6 of the oracle's objective."""
+    lp = lpgen.c0_plain(seed=0xC0, m=2000, n=5000).lp
+    o = dict(TIGHT, adaptive_step=1, anderson_acceleration=1)
+    r, x, y = native.solve(lp, precision=1, **o)
+    ro, _, _ = ob.solve_gpu(lp, **o)
+    print(r)
+    assert r["status"] == ro["status"] == 0 and r["iterations_fp32"] > 0
+    assert abs(r["objective"] - ro["objective"]) <= 5e-6 * (1 + abs(ro["objective"]))
+    k = ob.kkt_check(lp, x, y)
+    assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
+
+
+def test_matrix_zero_tol_on_gpu():
+    """pdlp.cpp:499-518 on the device path: the filtered solve equals the solve
+    of the LP with the small entries removed (bit for bit), matches the
+    oracle's objective, and the work units count the kept entries."""
+    import copy
+    lp = lpgen.c0_plain(seed=5, m=150, n=400).lp
+    lp = copy.deepcopy(lp)
+    lp.values = lp.values.copy()
+    lp.values[::7] *= 1e-7
+    tol = 1e-5
+    keep = np.abs(lp.values) >= tol
+    rows = np.repeat(np.arange(lp.num_rows), np.diff(lp.row_starts))
+    dropped = copy.deepcopy(lp)
+    dropped.values = lp.values[keep].copy()
+    dropped.col_indices = lp.col_indices[keep].copy()
+    dropped.row_starts = np.concatenate(
+        [[0], np.cumsum(np.bincount(rows[keep], minlength=lp.num_rows))]).astype(np.int32)
+    r1, x1, y1 = native.solve(lp, matrix_zero_tol=tol, **TIGHT)
+    r2, x2, y2 = native.solve(dropped, **TIGHT)
+    ro, _, _ = ob.solve(lp, matrix_zero_tol=tol, **TIGHT)
+    assert r1["status"] == r2["status"] == ro["status"] == 0
+    assert r1["iterations"] == r2["iterations"] and r1["objective"] == r2["objective"]
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+    assert abs(r1["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
+    assert r1["work_units"] == r1["iterations"] * (4.0 * int(keep.sum()) + 2.0 * (150 + 400))

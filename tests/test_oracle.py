@@ -176,3 +176,137 @@ def test_oracle_objective_envelope_vs_exact_optimum():
         opt = highs_objective(lp)
         errs.append(abs(r["objective"] - opt) / (1 + abs(opt)))
     assert max(errs) <= 3e-6
+
+
+# ---- non-default options: Anderson acceleration, adaptive step, matrix filter ----
+
+OPTIONAL = [
+    dict(anderson_acceleration=1),
+    dict(anderson_acceleration=1, anderson_window_size=3, anderson_regularization=1e-8),
+    dict(adaptive_step=1, adaptive_step_up=1.05, adaptive_step_down=0.5, max_iter=12000),
+    dict(adaptive_step=1, anderson_acceleration=1),
+]
+
+
+def _with_small_entries(lp, every=7, factor=1e-7):
+    """The same LP with every `every`-th matrix entry shrunk by `factor`."""
+    import copy
+    out = copy.deepcopy(lp)
+    out.values = out.values.copy()
+    out.values[::every] *= factor
+    return out
+
+
+def _dropped(lp, tol):
+    """The LP with entries |a| < tol removed, as a new CSR (no filtering option)."""
+    import copy
+    keep = np.abs(lp.values) >= tol
+    rows = np.repeat(np.arange(lp.num_rows), np.diff(lp.row_starts))
+    out = copy.deepcopy(lp)
+    out.values = lp.values[keep].copy()
+    out.col_indices = lp.col_indices[keep].copy()
+    out.row_starts = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=lp.num_rows))]).astype(np.int32)
+    return out
+
+
+@pytest.mark.parametrize("opts", OPTIONAL, ids=lambda o: "-".join(sorted(o)))
+def test_oracle_optional_features_match_reference_build(opts):
+    """Anderson acceleration (pdlp.cpp:293-437, 1339-1378) and the CPU loop's
+    adaptive step (1004-1040): bit-identical to the reference built from its
+    sources, including runs whose step grows and shrinks."""
+    if not ob.ref_available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    for seed, (m, n) in [(3, (120, 300)), (0xC0, (600, 1500))]:
+        lp = lpgen.c0_plain(seed=seed, m=m, n=n).lp
+        o = dict(TIGHT)
+        o.update(opts)
+        r1, x1, y1 = ob.solve(lp, **o)
+        r2, x2, y2 = ob.ref_solve(lp, **o)
+        assert (r1["status"], r1["iterations"]) == (r2["status"], r2["iterations"])
+        assert r1["objective"] == r2["objective"]
+        assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+
+
+def test_oracle_matrix_zero_tol():
+    """pdlp.cpp:499-518: entries with |a| < matrix_zero_tol are dropped before
+    scaling; the result equals solving the LP with those entries removed, and
+    the reference build's."""
+    lp = _with_small_entries(lpgen.c0_plain(seed=5, m=150, n=400).lp)
+    tol = 1e-5
+    r1, x1, y1 = ob.solve(lp, matrix_zero_tol=tol, **TIGHT)
+    r2, x2, y2 = ob.solve(_dropped(lp, tol), **TIGHT)
+    assert r1["status"] == 0
+    assert r1["iterations"] == r2["iterations"] and r1["objective"] == r2["objective"]
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+    r0, _, _ = ob.solve(lp, **TIGHT)  # the filter changes the run
+    assert r0["iterations"] != r1["iterations"] or r0["objective"] != r1["objective"]
+    if ob.ref_available():
+        r3, x3, y3 = ob.ref_solve(lp, matrix_zero_tol=tol, **TIGHT)
+        assert r3["iterations"] == r1["iterations"] and r3["objective"] == r1["objective"]
+        assert np.array_equal(x1, x3) and np.array_equal(y1, y3)
+
+
+@pytest.mark.parametrize("N,up,down", [(200, 1.05, 0.5), (20, 3.0, 0.5), (50, 4.0, 0.25)])
+def test_oracle_gpu_adaptive_step_rule(N, up, down):
+    """The GPU loop's adaptive step (pdlp.cpp:2271-2307), rebuilt outside the
+    oracle: the test drives the Halpern iteration itself (orc_pdhg_iterate),
+    takes x^ and A x^ (scipy) at each evaluation and applies the rule; with
+    restarts limited to the first evaluation the oracle's step after each
+    evaluation must be that value.  The chosen factors make the step both grow
+    and shrink."""
+    import scipy.sparse as sps
+    lp = lpgen.c0_plain(seed=3, m=120, n=300).lp
+    o = dict(TIGHT, adaptive_step=1, adaptive_step_up=up, adaptive_step_down=down,
+             termination_eval_frequency=N, restart_artificial_fraction=1.0,
+             restart_sufficient_decay=0.0, restart_necessary_decay=0.0)
+    sc = ob.scale_problem(lp, **o)
+    a = sps.csr_matrix((sc["values"], lp.col_indices, lp.row_starts),
+                       shape=(lp.num_rows, lp.num_cols))
+    sig, tau = sc["sigma_base"], sc["tau_base"]
+    step0 = 0.998 / sc["spectral_norm"]
+    r1, _, _ = ob.solve_gpu(lp, **dict(o, max_iter=N))
+    assert r1["restarts"] == 1
+    m, n = lp.num_rows, lp.num_cols
+    st = {k: np.zeros(n) for k in ("cx", "ix", "px")}
+    st.update({k: np.zeros(m) for k in ("cy", "iy", "py")})
+    ob.pdhg_iterate(lp, sc, N, 0, step0, 1.0, st)
+    for dst, src in (("cx", "px"), ("ix", "px"), ("cy", "py"), ("iy", "py")):
+        st[dst][:] = st[src]
+    step, prev, inner = step0, None, 0
+    seen = set()
+    for e in range(2, 7):
+        ob.pdhg_iterate(lp, sc, N, inner, step, r1["primal_weight"], st)
+        inner += N
+        z = st["px"].copy()
+        az = a @ z
+        if prev is not None:
+            ad = np.sum(sig * (az - prev[1]) ** 2)
+            xd = np.sum((z - prev[0]) ** 2 / tau)
+            local = 0.998 / np.sqrt(max(ad / xd, 1e-30))
+            assert abs(step - local) > 1e-6 * local  # no near-tie
+            seen.add(step > local)
+            step = min(max(step * (down if step > local else up), step0 * 1e-6), step0 * 1e3)
+        prev = (z, az)
+        r, _, _ = ob.solve_gpu(lp, **dict(o, max_iter=e * N))
+        assert r["restarts"] == 1
+        assert r["step"] == pytest.approx(step, rel=1e-14), e
+    if up > 2:
+        assert seen == {True, False}
+
+
+@pytest.mark.parametrize("opts", [dict(adaptive_step=1), dict(anderson_acceleration=1)],
+                         ids=["adaptive", "anderson"])
+def test_oracle_closed_form_with_optional_features(opts):
+    """The reference's feature-toggle tests (tests/test_pdlp.cpp:739-843,
+    1207-1340): the closed-form LPs keep their optima with the feature on, in
+    both loops."""
+    for c, lp in closed_form_cases():
+        if c["status"] != "Optimal" or "objective" not in c:
+            continue
+        o = dict(verbose=0)
+        o.update(c.get("options", {}))
+        o.update(opts)
+        for solve in (ob.solve, ob.solve_gpu):
+            r, x, y = solve(lp, **o)
+            assert r["status"] == 0, c["name"]
+            assert abs(r["objective"] - c["objective"]) <= max(c["tol"], 1e-4), c["name"]
