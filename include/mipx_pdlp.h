@@ -217,6 +217,21 @@ int mpx_pdlp_solve_batch_shared(const mpx_lp* base, int32_t count,
                                 mpx_pdlp_result* results);
 
 /*
+ * mpx_pdlp_solve_batch_shared with a warm start per instance: warm_x is
+ * [count][num_cols], warm_y [count][num_rows] in original space (both NULL:
+ * cold), mapped into the scaled space as setWarmStart does
+ * (pdlp.cpp:915-930).  Instance i then solves like mpx_pdlp_solve of the base
+ * LP with its column bounds and warm start (pdlp.cpp:2890-2898 setColBounds
+ * followed by setWarmStart, without rebuilding the scaling).
+ */
+int mpx_pdlp_solve_batch_shared_warm(const mpx_lp* base, int32_t count,
+                                     const double* col_lower, const double* col_upper,
+                                     const double* warm_x, const double* warm_y,
+                                     const mpx_pdlp_options* opts,
+                                     double* x_out, double* y_out,
+                                     mpx_pdlp_result* results);
+
+/*
  * Batched solve of `count` independent LPs of uneven sizes in one persistent
  * launch.  x_out[i] / y_out[i] are per-LP output arrays (or NULL).
  */
@@ -327,6 +342,15 @@ int mpx_server_submit(mpx_server* srv, const mpx_lp* lp, int64_t* ticket);
 /* A variant of base `base_id` with its own column bounds (copied). */
 int mpx_server_submit_bounds(mpx_server* srv, int32_t base_id, const double* col_lower,
                              const double* col_upper, int64_t* ticket);
+/* The same with an original-space warm start warm_x[num_cols] /
+ * warm_y[num_rows] (copied), mapped into the scaled space as the reference's
+ * setWarmStart (pdlp.cpp:915-930): a child node started from its parent's
+ * solution.  The result equals mpx_pdlp_solve_batch_shared_warm of the
+ * request alone.  No counterpart in the reference, which solves node LPs one
+ * at a time. */
+int mpx_server_submit_bounds_warm(mpx_server* srv, int32_t base_id, const double* col_lower,
+                                  const double* col_upper, const double* warm_x,
+                                  const double* warm_y, int64_t* ticket);
 /* Runs one scheduling round: activate prepared requests, admit queued ones,
  * iterate one round, retire finished ones.  Returns the number of requests
  * still queued, preparing or running in *active. */

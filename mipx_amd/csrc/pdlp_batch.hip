@@ -689,6 +689,10 @@ struct BInitArgs {
     const double* col_scale;
     const double* scal;  // owner scal
     int n, m, own_bounds, mixed;
+    // Warm start in original space, staged in the instance's FP64 x_r (n)
+    // and y^ (m) arrays by the upload; consumed (and those arrays cleared)
+    // by kInstInit.
+    int warm;
     double offset, sgn;
 };
 
@@ -698,7 +702,7 @@ __global__ __launch_bounds__(kBlock) void kInstInit(const BInst* __restrict__ in
     const BInst in = insts[blockIdx.x];
     const BInitArgs a = args[blockIdx.x];
     const int tid = threadIdx.x;
-    const double cs = a.scal[1];
+    const double cs = a.scal[1], os = a.scal[0];
     for (int j = tid; j < a.n; j += kBlock) {
         if (a.own_bounds) {
             const double lj = a.lo_o[j], uj = a.hi_o[j];
@@ -711,12 +715,20 @@ __global__ __launch_bounds__(kBlock) void kInstInit(const BInst* __restrict__ in
         }
         a.lf[j] = static_cast<float>(a.l[j]);
         a.uf[j] = static_cast<float>(a.u[j]);
-        in.d.x[j] = in.d.x0[j] = in.d.xh[j] = in.d.xr[j] = 0.0;
-        in.f.x[j] = in.f.x0[j] = in.f.xh[j] = in.f.xr[j] = 0.0f;
+        // pdlp.cpp:915-930 / 1812-1827: x = x0 = warm_x / col_scale * cs.
+        const double xs = a.warm ? in.d.xr[j] / a.col_scale[j] * cs : 0.0;
+        in.d.x[j] = in.d.x0[j] = xs;
+        in.d.xh[j] = in.d.xr[j] = 0.0;
+        in.f.x[j] = in.f.x0[j] = static_cast<float>(xs);
+        in.f.xh[j] = in.f.xr[j] = 0.0f;
     }
     for (int i = tid; i < a.m; i += kBlock) {
-        in.d.y[i] = in.d.y0[i] = in.d.yh[i] = 0.0;
-        in.f.y[i] = in.f.y0[i] = in.f.yh[i] = 0.0f;
+        // y = y0 = warm_y * os / (sign * row_scale).
+        const double ys = a.warm ? in.d.yh[i] * os / (a.sgn * in.row_scale[i]) : 0.0;
+        in.d.y[i] = in.d.y0[i] = ys;
+        in.d.yh[i] = 0.0;
+        in.f.y[i] = in.f.y0[i] = static_cast<float>(ys);
+        in.f.yh[i] = 0.0f;
     }
     if (tid == 0) {
         BCtrl& c = *in.ctrl;
@@ -988,7 +1000,8 @@ void uploadOwner(const mpx_lp& lp, const OwnerHost& h, const BOwner& b, hipStrea
 // instance descriptor and its initialisation arguments.
 void carveInstance(const BOwner& w, const mpx_lp& lp, int owner_id, int preset, const double* lo,
                    const double* hi, bool mixed, Carve& cv, hipStream_t st, BCtrl* ctrl,
-                   BInst& in, BInitArgs& a) {
+                   BInst& in, BInitArgs& a, const double* warm_x = nullptr,
+                   const double* warm_y = nullptr) {
     const int m = w.m, n = w.n;
     std::memset(&in, 0, sizeof(in));
     std::memset(&a, 0, sizeof(a));
@@ -1081,6 +1094,11 @@ void carveInstance(const BOwner& w, const mpx_lp& lp, int owner_id, int preset, 
     in.f.rl = w.rlf;
     in.f.ru = w.ruf;
     in.ctrl = ctrl;
+    if (warm_x && warm_y) {
+        MPX_HIP(hipMemcpyAsync(in.d.xr, warm_x, n * 8ull, hipMemcpyHostToDevice, st));
+        MPX_HIP(hipMemcpyAsync(in.d.yh, warm_y, m * 8ull, hipMemcpyHostToDevice, st));
+        a.warm = 1;
+    }
 }
 
 // ---- lane-per-instance rounds of one shared matrix ----------------------------
@@ -1264,10 +1282,15 @@ public:
         return static_cast<int>(owners_host_.size()) - 1;
     }
     // Registers an instance of owner `ow` (col bounds override or null).
-    int addInstance(int ow, const double* lo, const double* hi) {
+    // lo/hi: the instance's column bounds (NULL: the matrix's own); wx/wy:
+    // an original-space warm start (NULL: cold), read at setup().
+    int addInstance(int ow, const double* lo, const double* hi, const double* wx = nullptr,
+                    const double* wy = nullptr) {
         inst_owner_.push_back(ow);
         inst_lo_.push_back(lo);
         inst_hi_.push_back(hi);
+        inst_wx_.push_back(wx && wy ? wx : nullptr);
+        inst_wy_.push_back(wx && wy ? wy : nullptr);
         return static_cast<int>(inst_owner_.size()) - 1;
     }
 
@@ -1324,7 +1347,7 @@ private:
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<const mpx_lp*> owners_host_;
     std::vector<int> inst_owner_;
-    std::vector<const double*> inst_lo_, inst_hi_;
+    std::vector<const double*> inst_lo_, inst_hi_, inst_wx_, inst_wy_;
     std::unique_ptr<DevBuf> arena_;
     std::vector<BOwner> owners_;
     std::vector<BInst> insts_;
@@ -1395,7 +1418,8 @@ void BatchEngine::setup() {
     for (int i = 0; i < ni; ++i) {
         const int ow = inst_owner_[i];
         carveInstance(owners_[ow], *owners_host_[ow], ow, owner_preset_[ow], inst_lo_[i],
-                      inst_hi_[i], mixed_, cv, st_, ctrl_all_ + i, insts_[i], args[i]);
+                      inst_hi_[i], mixed_, cv, st_, ctrl_all_ + i, insts_[i], args[i], inst_wx_[i],
+                      inst_wy_[i]);
     }
     // Tile lists for the packed kernels, sized for the smallest tiles (16
     // columns or rows per tile at 16 lanes each).
@@ -2025,6 +2049,7 @@ struct ServeReq {
     std::unique_ptr<ServeLp> own;   // a fresh LP
     OwnerHost oh;                   // its transpose and power start vector
     std::vector<double> vlo, vhi;   // a variant's column bounds
+    std::vector<double> wx, wy;     // a variant's warm start (empty: cold)
     std::chrono::steady_clock::time_point t_submit, t_active;
     void* mem = nullptr;
     BInst inst{};
@@ -2155,12 +2180,17 @@ public:
         r->oh = ownerHostData(r->own->lp);
         return enqueue(std::move(r));
     }
-    int64_t submitBounds(int base, const double* lo, const double* hi) {
+    int64_t submitBounds(int base, const double* lo, const double* hi,
+                         const double* wx = nullptr, const double* wy = nullptr) {
         auto r = std::make_unique<ServeReq>();
-        const int n = bases_[base]->lp.num_cols;
+        const int n = bases_[base]->lp.num_cols, m = bases_[base]->lp.num_rows;
         r->base = base;
         r->vlo.assign(lo, lo + n);
         r->vhi.assign(hi, hi + n);
+        if (wx && wy) {
+            r->wx.assign(wx, wx + n);
+            r->wy.assign(wy, wy + m);
+        }
         return enqueue(std::move(r));
     }
 
@@ -2308,7 +2338,9 @@ private:
             } else {
                 const ServeLp& b = *bases_[q->base];
                 carveInstance(b.owner, b.lp, q->base, b.preset, q->vlo.data(), q->vhi.data(),
-                              mixed_, cv, st_, ctrl, q->inst, q->args);
+                              mixed_, cv, st_, ctrl, q->inst, q->args,
+                              q->wx.empty() ? nullptr : q->wx.data(),
+                              q->wy.empty() ? nullptr : q->wy.data());
                 q->inst.il = 1;
             }
             q->xo = cv.take<double>(static_cast<size_t>(q->inst.n));
@@ -2730,8 +2762,17 @@ extern "C" {
 int mpx_pdlp_solve_batch_shared(const mpx_lp* base, int32_t count, const double* col_lower,
                                 const double* col_upper, const mpx_pdlp_options* opts,
                                 double* x_out, double* y_out, mpx_pdlp_result* results) {
+    return mpx_pdlp_solve_batch_shared_warm(base, count, col_lower, col_upper, nullptr, nullptr,
+                                            opts, x_out, y_out, results);
+}
+
+int mpx_pdlp_solve_batch_shared_warm(const mpx_lp* base, int32_t count, const double* col_lower,
+                                     const double* col_upper, const double* warm_x,
+                                     const double* warm_y, const mpx_pdlp_options* opts,
+                                     double* x_out, double* y_out, mpx_pdlp_result* results) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (!validLpB(base) || count < 0 || !results || (col_lower == nullptr) != (col_upper == nullptr)) {
+    if (!validLpB(base) || count < 0 || !results || (col_lower == nullptr) != (col_upper == nullptr) ||
+        (warm_x == nullptr) != (warm_y == nullptr)) {
         setError("mpx_pdlp_solve_batch_shared: invalid arguments");
         return MPX_ERR_INVALID;
     }
@@ -2745,7 +2786,9 @@ int mpx_pdlp_solve_batch_shared(const mpx_lp* base, int32_t count, const double*
         const int ow = eng.addOwner(*base);
         for (int i = 0; i < count; ++i)
             eng.addInstance(ow, col_lower ? col_lower + static_cast<size_t>(i) * n : nullptr,
-                            col_upper ? col_upper + static_cast<size_t>(i) * n : nullptr);
+                            col_upper ? col_upper + static_cast<size_t>(i) * n : nullptr,
+                            warm_x ? warm_x + static_cast<size_t>(i) * n : nullptr,
+                            warm_y ? warm_y + static_cast<size_t>(i) * m : nullptr);
         eng.setup();
         eng.run(t0);
         std::vector<double*> xs(static_cast<size_t>(count)), ys(static_cast<size_t>(count));
@@ -3043,6 +3086,19 @@ int mpx_server_submit_bounds(mpx_server* srv, int32_t base_id, const double* col
     }
     if (srv->broken) return MPX_ERR_DEVICE;
     *ticket = srv->s->submitBounds(base_id, col_lower, col_upper);
+    return MPX_OK;
+}
+
+int mpx_server_submit_bounds_warm(mpx_server* srv, int32_t base_id, const double* col_lower,
+                                  const double* col_upper, const double* warm_x,
+                                  const double* warm_y, int64_t* ticket) {
+    if (!srv || !ticket || !col_lower || !col_upper || !warm_x || !warm_y || base_id < 0 ||
+        base_id >= srv->s->numBases()) {
+        setError("mpx_server_submit_bounds_warm: invalid arguments");
+        return MPX_ERR_INVALID;
+    }
+    if (srv->broken) return MPX_ERR_DEVICE;
+    *ticket = srv->s->submitBounds(base_id, col_lower, col_upper, warm_x, warm_y);
     return MPX_OK;
 }
 

@@ -16,8 +16,9 @@ _lib = None
 EXPORTED = [
     "mpx_pdlp_default_options", "mpx_last_error", "mpx_version", "mpx_device_count",
     "mpx_device_synchronize", "mpx_copy_bench",
-    "mpx_pdlp_solve", "mpx_pdlp_solve_batch_shared", "mpx_pdlp_solve_batch",
-    "mpx_server_create", "mpx_server_add_base", "mpx_server_submit", "mpx_server_submit_bounds",
+    "mpx_pdlp_solve", "mpx_pdlp_solve_batch_shared", "mpx_pdlp_solve_batch_shared_warm",
+    "mpx_pdlp_solve_batch", "mpx_server_create", "mpx_server_add_base", "mpx_server_submit",
+    "mpx_server_submit_bounds", "mpx_server_submit_bounds_warm",
     "mpx_server_step", "mpx_server_result", "mpx_server_poll", "mpx_server_get_stats",
     "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
     "mpx_pdhg_iterate", "mpx_pdlp_scale_problem_batched", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
@@ -41,6 +42,10 @@ def lib():
     L.mpx_pdlp_solve_batch_shared.argtypes = [C.POINTER(MpxLp), C.c_int32, C.c_void_p,
                                               C.c_void_p, C.POINTER(MpxPdlpOptions),
                                               C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
+    L.mpx_pdlp_solve_batch_shared_warm.argtypes = [C.POINTER(MpxLp), C.c_int32, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p, C.c_void_p,
+                                                   C.POINTER(MpxPdlpOptions), C.c_void_p,
+                                                   C.c_void_p, C.POINTER(MpxPdlpResult)]
     L.mpx_pdlp_solve_batch.argtypes = [C.POINTER(MpxLp), C.c_int32, C.POINTER(MpxPdlpOptions),
                                        C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
     L.mpx_server_create.argtypes = [C.POINTER(MpxPdlpOptions), C.c_double, C.c_int32,
@@ -49,6 +54,8 @@ def lib():
     L.mpx_server_submit.argtypes = [C.c_void_p, C.POINTER(MpxLp), C.POINTER(C.c_int64)]
     L.mpx_server_submit_bounds.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                            C.POINTER(C.c_int64)]
+    L.mpx_server_submit_bounds_warm.argtypes = [C.c_void_p, C.c_int32] + [C.c_void_p] * 4 + [
+        C.POINTER(C.c_int64)]
     L.mpx_server_get_stats.argtypes = [C.c_void_p, C.POINTER(MpxServerStats)]
     L.mpx_server_poll.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
     L.mpx_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
@@ -115,8 +122,9 @@ def solve(lp, warm=None, **opts):
     return d, x, y
 
 
-def solve_batch_shared(lp, col_lower=None, col_upper=None, **opts):
-    """count = rows of col_lower; returns (list of result dicts, X[count,n], Y[count,m])."""
+def solve_batch_shared(lp, col_lower=None, col_upper=None, warm=None, **opts):
+    """count = rows of col_lower; returns (list of result dicts, X[count,n], Y[count,m]).
+    warm: optional (WX[count,n], WY[count,m]) original-space warm starts."""
     o = opts.pop("_options", None) or make_options(**opts)
     count = opts.pop("count", None)
     if col_lower is not None:
@@ -127,10 +135,18 @@ def solve_batch_shared(lp, col_lower=None, col_upper=None, **opts):
     X = np.zeros((count, lp.num_cols))
     Y = np.zeros((count, lp.num_rows))
     res = (MpxPdlpResult * count)()
-    rc = lib().mpx_pdlp_solve_batch_shared(
-        C.byref(clp), count, None if col_lower is None else col_lower.ctypes.data,
-        None if col_upper is None else col_upper.ctypes.data, C.byref(o), X.ctypes.data,
-        Y.ctypes.data, res)
+    if warm is None:
+        rc = lib().mpx_pdlp_solve_batch_shared(
+            C.byref(clp), count, None if col_lower is None else col_lower.ctypes.data,
+            None if col_upper is None else col_upper.ctypes.data, C.byref(o), X.ctypes.data,
+            Y.ctypes.data, res)
+    else:
+        wx = np.ascontiguousarray(warm[0], dtype=np.float64).reshape(count, lp.num_cols)
+        wy = np.ascontiguousarray(warm[1], dtype=np.float64).reshape(count, lp.num_rows)
+        rc = lib().mpx_pdlp_solve_batch_shared_warm(
+            C.byref(clp), count, None if col_lower is None else col_lower.ctypes.data,
+            None if col_upper is None else col_upper.ctypes.data, wx.ctypes.data,
+            wy.ctypes.data, C.byref(o), X.ctypes.data, Y.ctypes.data, res)
     _check(rc, "mpx_pdlp_solve_batch_shared")
     return [r.as_dict() for r in res], X, Y
 
@@ -279,12 +295,22 @@ class Server:
         self._dims[t.value] = (lp.num_cols, lp.num_rows)
         return t.value
 
-    def submit_bounds(self, base_id, col_lower, col_upper):
+    def submit_bounds(self, base_id, col_lower, col_upper, warm=None):
+        """warm: optional (x, y) in original space (a parent node's solution)."""
         lo = np.ascontiguousarray(col_lower, dtype=np.float64)
         hi = np.ascontiguousarray(col_upper, dtype=np.float64)
         t = C.c_int64(-1)
-        _check(lib().mpx_server_submit_bounds(self._h, base_id, lo.ctypes.data, hi.ctypes.data,
-                                              C.byref(t)), "mpx_server_submit_bounds")
+        if warm is None:
+            _check(lib().mpx_server_submit_bounds(self._h, base_id, lo.ctypes.data,
+                                                  hi.ctypes.data, C.byref(t)),
+                   "mpx_server_submit_bounds")
+        else:
+            wx = np.ascontiguousarray(warm[0], dtype=np.float64)
+            wy = np.ascontiguousarray(warm[1], dtype=np.float64)
+            _check(lib().mpx_server_submit_bounds_warm(self._h, base_id, lo.ctypes.data,
+                                                       hi.ctypes.data, wx.ctypes.data,
+                                                       wy.ctypes.data, C.byref(t)),
+                   "mpx_server_submit_bounds_warm")
         self._dims[t.value] = self._base_dims[base_id]
         return t.value
 

@@ -106,3 +106,47 @@ def test_batched_setup_vs_oracle(which):
             assert np.array_equal(v, again[k]), k
         else:
             assert v == again[k], k
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_shared_warm_start_vs_oracle(precision):
+    """Branch-and-bound feed: node LPs (bounds variants of one base)
+    warm-started from the root LP's solution.  Each instance matches the
+    oracle's warm-started solve of its own LP (setColBounds + setWarmStart,
+    pdlp.cpp:2890-2898, 915-930); in FP64 the iteration count follows the
+    oracle's.  Restarting the instances from their own solutions takes fewer
+    iterations in total than the cold solves (the reference's warm-start test,
+    tests/test_pdlp.cpp:1076-1113, for one LP; per instance the reference
+    itself sometimes needs more)."""
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    count = 12
+    lo = np.empty((count, 800))
+    hi = np.empty((count, 800))
+    for i in range(count):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    r0, x0, y0 = native.solve(g.lp, **TIGHT)
+    assert r0["status"] == 0
+    opts = dict(precision=precision, max_iter=200000, **TIGHT)
+    warm = (np.tile(x0, (count, 1)), np.tile(y0, (count, 1)))
+    res, X, Y = native.solve_batch_shared(g.lp, lo, hi, warm=warm, **opts)
+    cold, _, _ = native.solve_batch_shared(g.lp, lo, hi, **opts)
+    print("warm iters", [r["iterations"] for r in res], "cold", [r["iterations"] for r in cold])
+    both = []
+    for i in range(count):
+        lp = g.lp.with_col_bounds(lo[i], hi[i])
+        ro, _, _ = ob.solve(lp, warm=(x0, y0), **TIGHT)
+        assert res[i]["status"] == ro["status"] == cold[i]["status"], (i, res[i], ro["status"])
+        if ro["status"] == 0:
+            rel = abs(res[i]["objective"] - ro["objective"]) / (1 + abs(ro["objective"]))
+            assert rel <= (5e-6 if precision else 1e-6), (i, res[i]["objective"], ro["objective"])
+            if not precision:
+                assert abs(res[i]["iterations"] - ro["iterations"]) <= 0.1 * ro["iterations"], i
+            k = ob.kkt_check(lp, X[i], Y[i])
+            assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, (i, k)
+            both.append(i)
+    assert both
+    _, XC, YC = native.solve_batch_shared(g.lp, lo, hi, **opts)
+    again, _, _ = native.solve_batch_shared(g.lp, lo, hi, warm=(XC, YC), **opts)
+    print("re-warm iters", [again[i]["iterations"] for i in both])
+    assert all(again[i]["status"] == 0 for i in both)
+    assert sum(again[i]["iterations"] for i in both) < sum(cold[i]["iterations"] for i in both)

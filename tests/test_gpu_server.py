@@ -165,3 +165,28 @@ def test_server_statuses_and_time_limit():
     r = _drain(srv, [t])[t][0]
     srv.close()
     assert r["status"] == 4 and 0 < r["iterations"] <= 2 * 200, r
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_server_warm_variants_identical_to_solo_batch(precision):
+    """Warm-started bounds variants (children of a solved root) in a mix with
+    cold variants and a fresh LP: each result is bit-identical to the
+    warm-started shared batch of that request alone."""
+    opts = dict(precision=precision, max_iter=200000, **TIGHT)
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    r0, x0, y0 = native.solve(g.lp, **TIGHT)
+    bounds = [g.branch_bounds(lpgen.c2_instance_seed(i)) for i in range(6)]
+    srv = native.Server(max_slots=16, **opts)
+    base = srv.add_base(g.lp)
+    warm_t = {srv.submit_bounds(base, *bounds[i], warm=(x0, y0)): i for i in (0, 2, 4)}
+    cold_t = {srv.submit_bounds(base, *bounds[i]): i for i in (1, 3, 5)}
+    fresh = lpgen.c3_instance(7, 120).lp
+    tf = srv.submit(fresh)
+    got = _drain(srv, list(warm_t) + list(cold_t) + [tf])
+    srv.close()
+    for t, i in warm_t.items():
+        res, xs, ys = native.solve_batch_shared(g.lp, bounds[i][0][None], bounds[i][1][None],
+                                                warm=(x0[None], y0[None]), **opts)
+        _same(got[t], (res[0], xs[0], ys[0]), f"warm{i}")
+    for t, i in cold_t.items():
+        _same(got[t], _alone_shared(g, bounds[i], opts), f"cold{i}")
