@@ -275,3 +275,49 @@ def test_matrix_zero_tol_on_gpu():
     assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
     assert abs(r1["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
     assert r1["work_units"] == r1["iterations"] * (4.0 * int(keep.sum()) + 2.0 * (150 + 400))
+
+
+def test_concurrent_solves_and_race_stop():
+    """The concurrent-root race contract (mip_solver.cpp:4478-4500): solves on
+    their own threads share no state (own stream and memory per solve) and
+    give results identical to sequential solves; a shared stop flag set by the
+    first Optimal finisher cancels the other, which reports IterLimit with
+    work units = iterations * (4 nnz + 2 (m + n)), no wall-clock term."""
+    import threading
+    from mipx_amd._abi import STOP_FN
+    small = lpgen.c0_plain(seed=101, m=300, n=750).lp
+    big = lpgen.c0_plain(seed=0xC0, m=2000, n=5000).lp
+    seq = {k: native.solve(lp, **TIGHT) for k, lp in (("small", small), ("big", big))}
+
+    def both(stop_fn=None, on_done=None):
+        out = {}
+
+        def run(key, lp):
+            kw = dict(TIGHT)
+            if stop_fn is not None:
+                kw["stop_fn"] = stop_fn
+            out[key] = native.solve(lp, **kw)
+            if on_done:
+                on_done(out[key])
+        ts = [threading.Thread(target=run, args=kv) for kv in (("small", small), ("big", big))]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return out
+
+    par = both()
+    for k in ("small", "big"):
+        (ra, xa, ya), (rb, xb, yb) = par[k], seq[k]
+        assert (ra["status"], ra["iterations"], ra["objective"]) == \
+            (rb["status"], rb["iterations"], rb["objective"]), k
+        assert np.array_equal(xa, xb) and np.array_equal(ya, yb), k
+
+    flag = threading.Event()
+    cb = STOP_FN(lambda _u: 1 if flag.is_set() else 0)
+    race = both(cb, lambda r: flag.set() if r[0]["status"] == 0 else None)
+    rs, rbg = race["small"][0], race["big"][0]
+    assert rs["status"] == 0 and rs["iterations"] == seq["small"][0]["iterations"]
+    assert rbg["status"] == 3 and 0 <= rbg["iterations"] < seq["big"][0]["iterations"]
+    assert rbg["iterations"] % 200 == 0
+    assert rbg["work_units"] == rbg["iterations"] * (4.0 * big.nnz + 2.0 * (2000 + 5000))
