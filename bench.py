@@ -323,6 +323,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--handover", type=float, default=0.0,
                     help="mixed precision: FP32 -> FP64 handover KKT ratio (0: library default)")
+    ap.add_argument("--spmv-group", type=int, default=0,
+                    help="lanes per column / row of the packed kernels (0: per-LP choice)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -350,6 +352,8 @@ def main():
     cap = dict(max_iter=6 * 200) if a.profile_only else {}
     if a.handover > 0:
         cap["mixed_handover"] = a.handover
+    if a.spmv_group > 0:
+        cap["spmv_group"] = a.spmv_group
     if a.workload == "c3":
         lps = c3_workload(a.count)
         lp = lps[0]
