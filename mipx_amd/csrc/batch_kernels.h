@@ -46,6 +46,9 @@ struct BCtrl {
     int32_t power_iters, g_row, g_col, use_lds;
     int32_t round_iters, skip_eval;
     unsigned long long t_finish;  // device wall clock when the instance finished
+    // This round's evaluation sums (column fields, then row fields), written
+    // by kEvalFields for kEvalControl.
+    double evf[10];
 };
 
 enum BPhase { kPhaseFp32 = 0, kPhaseFp64 = 1, kPhasePolish = 2, kPhaseDone = 3 };
@@ -123,48 +126,80 @@ __device__ inline T bDot(const int* __restrict__ starts, const int* __restrict__
 }
 
 constexpr int kNumEval = static_cast<int>(kNumColFields) + static_cast<int>(kNumRowFields);
+// Columns (rows) per lane group and pass in the batched evaluation.
+constexpr int kEvalCW = 4;
+static_assert(kNumEval == 10, "BCtrl::evf holds the evaluation sums");
 
 struct BEval {
     double fpe, rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj, dx2, dy2;
 };
 
-// FP64 evaluation of (xh, yh) against (x0, y0), same formulas as the single
-// path (src/lp/pdlp.cpp:1076-1202).  Result valid in all threads.
+// FP64 evaluation sums of (xh, yh) against (x0, y0), same formulas as the
+// single path (src/lp/pdlp.cpp:1076-1202): the kNumColFields column fields
+// then the kNumRowFields row fields, stored to dst by thread 0.
 template <int GC, int GR, typename T>
-__device__ inline BEval bEvaluate(const BInst& in, const BVecs<T>& v, const BCtrl& c) {
+__device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCtrl& c,
+                                   double* dst) {
     const int tid = threadIdx.x;
+    // Each lane group takes kEvalCW columns (rows) per pass, PC (PR) apart,
+    // with their loads issued together: a thread still visits its columns in
+    // the order grp, grp + PC, grp + 2 PC, ... and sums each column's entries
+    // per lane in storage order, so the fields are the same bits as with one
+    // column per pass, with kEvalCW dependent load chains in flight.
+    constexpr int PC = kBlock / GC, PR = kBlock / GR;
+    const int lc = tid % GC, gc = tid / GC;
     double cf[kNumColFields] = {0, 0, 0, 0, 0, 0};
-    for (int base = 0; base < in.n; base += kBlock / GC) {
-        const int j = base + tid / GC, lane = tid % GC;
-        if (j < in.n) {
-            double aty = 0.0, atdy = 0.0;
-            for (int q = in.cs[j] + lane; q < in.cs[j + 1]; q += GC) {
-                const int i = in.ri[q];
-                const double a = in.valt[q];
-                const double yi = static_cast<double>(v.yh[i]);
-                aty += a * yi;
-                atdy += a * (yi - static_cast<double>(v.y0[i]));
+    for (int base = 0; base < in.n; base += kEvalCW * PC) {
+        int j[kEvalCW], qb[kEvalCW], qe[kEvalCW];
+        double aty[kEvalCW], atdy[kEvalCW], xj[kEvalCW], x0j[kEvalCW];
+        int most = 0;
+#pragma unroll
+        for (int w = 0; w < kEvalCW; ++w) {
+            j[w] = base + gc + w * PC;
+            const bool ok = j[w] < in.n;
+            qb[w] = ok ? in.cs[j[w]] + lc : 0;
+            qe[w] = ok ? in.cs[j[w] + 1] : 0;
+            xj[w] = ok ? static_cast<double>(v.xh[j[w]]) : 0.0;
+            x0j[w] = ok ? static_cast<double>(v.x0[j[w]]) : 0.0;
+            const int cnt = qe[w] > qb[w] ? (qe[w] - qb[w] + GC - 1) / GC : 0;
+            most = cnt > most ? cnt : most;
+            aty[w] = atdy[w] = 0.0;
+        }
+        for (int t = 0; t < most; ++t) {
+#pragma unroll
+            for (int w = 0; w < kEvalCW; ++w) {
+                const int q = qb[w] + t * GC;
+                if (q < qe[w]) {
+                    const int i = in.ri[q];
+                    const double a = in.valt[q];
+                    const double yi = static_cast<double>(v.yh[i]);
+                    aty[w] += a * yi;
+                    atdy[w] += a * (yi - static_cast<double>(v.y0[i]));
+                }
             }
-            aty = groupSum<GC>(aty);
-            atdy = groupSum<GC>(atdy);
-            if (lane == 0) {
-                const double xj = static_cast<double>(v.xh[j]);
-                const double dx = xj - static_cast<double>(v.x0[j]);
+        }
+#pragma unroll
+        for (int w = 0; w < kEvalCW; ++w) {
+            aty[w] = groupSum<GC>(aty[w]);
+            atdy[w] = groupSum<GC>(atdy[w]);
+            if (lc == 0 && j[w] < in.n) {
+                const int jj = j[w];
+                const double dx = xj[w] - x0j[w];
                 cf[kDx2] += dx * dx;
-                cf[kInter] += atdy * dx;
-                const double g = in.c[j] + aty;
-                const double lb = in.l[j], ub = in.u[j];
+                cf[kInter] += atdy[w] * dx;
+                const double g = in.c[jj] + aty[w];
+                const double lb = in.l[jj], ub = in.u[jj];
                 // Activity against the bounds the iteration projected onto.
                 const double lbi = static_cast<double>(static_cast<T>(lb));
                 const double ubi = static_cast<double>(static_cast<T>(ub));
                 double dr;
-                if (xj <= lbi + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
-                else if (xj >= ubi - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
+                if (xj[w] <= lbi + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
+                else if (xj[w] >= ubi - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
                 else dr = g;
-                const double d = dr / (c.os * in.col_scale[j]);
+                const double d = dr / (c.os * in.col_scale[jj]);
                 cf[kDsq] += d * d;
                 cf[kDmax] = maxRef(cf[kDmax], fabs(d));
-                cf[kPobj] += in.c[j] * xj;
+                cf[kPobj] += in.c[jj] * xj[w];
                 double gp = g;
                 if (!isfinite(lb)) gp = minRef(gp, 0.0);
                 if (!isfinite(ub)) gp = maxRef(gp, 0.0);
@@ -173,21 +208,44 @@ __device__ inline BEval bEvaluate(const BInst& in, const BVecs<T>& v, const BCtr
             }
         }
     }
+    const int lr = tid % GR, gr = tid / GR;
     double rf[kNumRowFields] = {0, 0, 0, 0};
-    for (int base = 0; base < in.m; base += kBlock / GR) {
-        const int i = base + tid / GR, lane = tid % GR;
-        if (i < in.m) {
-            const double ax = bDot<GR>(in.rs, in.ci, in.val, v.xh, i, lane);
-            if (lane == 0) {
-                const double lo = in.rl[i], hi = in.ru[i];
-                const double vs = ax - clampRef(ax, lo, hi);
-                const double p = vs / (c.cs * in.row_scale[i]);
+    for (int base = 0; base < in.m; base += kEvalCW * PR) {
+        int i[kEvalCW], qb[kEvalCW], qe[kEvalCW];
+        double ax[kEvalCW], yv[kEvalCW], y0v[kEvalCW];
+        int most = 0;
+#pragma unroll
+        for (int w = 0; w < kEvalCW; ++w) {
+            i[w] = base + gr + w * PR;
+            const bool ok = i[w] < in.m;
+            qb[w] = ok ? in.rs[i[w]] + lr : 0;
+            qe[w] = ok ? in.rs[i[w] + 1] : 0;
+            yv[w] = ok ? static_cast<double>(v.yh[i[w]]) : 0.0;
+            y0v[w] = ok ? static_cast<double>(v.y0[i[w]]) : 0.0;
+            const int cnt = qe[w] > qb[w] ? (qe[w] - qb[w] + GR - 1) / GR : 0;
+            most = cnt > most ? cnt : most;
+            ax[w] = 0.0;
+        }
+        for (int t = 0; t < most; ++t) {
+#pragma unroll
+            for (int w = 0; w < kEvalCW; ++w) {
+                const int q = qb[w] + t * GR;
+                if (q < qe[w]) ax[w] += in.val[q] * static_cast<double>(v.xh[in.ci[q]]);
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < kEvalCW; ++w) {
+            ax[w] = groupSum<GR>(ax[w]);
+            if (lr == 0 && i[w] < in.m) {
+                const int ii = i[w];
+                const double lo = in.rl[ii], hi = in.ru[ii];
+                const double vs = ax[w] - clampRef(ax[w], lo, hi);
+                const double p = vs / (c.cs * in.row_scale[ii]);
                 rf[kPsq] += p * p;
                 rf[kPmax] = maxRef(rf[kPmax], fabs(p));
-                const double yi = static_cast<double>(v.yh[i]);
-                const double dy = yi - static_cast<double>(v.y0[i]);
+                const double dy = yv[w] - y0v[w];
                 rf[kDy2] += dy * dy;
-                double yp = yi;
+                double yp = yv[w];
                 if (!isfinite(hi)) yp = minRef(yp, 0.0);
                 if (!isfinite(lo)) yp = maxRef(yp, 0.0);
                 if (yp > 0.0 && isfinite(hi)) rf[kDrow] -= yp * hi;
@@ -206,12 +264,15 @@ __device__ inline BEval bEvaluate(const BInst& in, const BVecs<T>& v, const BCtr
         mx[kNumColFields + f] = f == kPmax;
     }
     blockReduce<kNumEval>(all, mx, out);
-    __shared__ double sh[kNumEval];
     if (tid == 0)
-        for (int f = 0; f < kNumEval; ++f) sh[f] = out[f];
-    __syncthreads();
-    const double* mc = sh;
-    const double* mr = sh + kNumColFields;
+        for (int f = 0; f < kNumEval; ++f) dst[f] = out[f];
+}
+
+// The termination quantities from the evaluation sums f (column fields, then
+// row fields) and the instance's scalars.
+__device__ inline BEval bEvalFinish(const double* f, const BCtrl& c) {
+    const double* mc = f;
+    const double* mr = f + kNumColFields;
     BEval e;
     e.dx2 = mc[kDx2];
     e.dy2 = mr[kDy2];
@@ -226,7 +287,6 @@ __device__ inline BEval bEvaluate(const BInst& in, const BVecs<T>& v, const BCtr
     e.pobj = c.offset + c.sgn * mc[kPobj] * unscale;
     e.dobj = c.offset + c.sgn * (mc[kDcol] + mr[kDrow]) * unscale;
     e.gap = fabs(e.pobj - e.dobj) / (1.0 + fabs(e.pobj) + fabs(e.dobj));
-    __syncthreads();
     return e;
 }
 

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -459,7 +460,9 @@ __device__ void evalControl(const BInst& in, const BOptions& o) {
     }
     const bool polish = s_phase == kPhasePolish;
     const int ran = s_ran;
-    const BEval e = bEvaluate<GC, GR, T>(in, v, c);
+    // The sums come from kEvalFields (same round, same stream).
+    const BEval e = bEvalFinish(c.evf, c);
+    __syncthreads();
     if (tid == 0) {
         c.inner += ran;
         c.total += ran;
@@ -641,6 +644,41 @@ __device__ void evalControl(const BInst& in, const BOptions& o) {
     }
     if (tid == 0 && c.phase == kPhaseDone) c.round_iters = 0;
     if (tid == 0) stampDone(c);
+}
+
+// Evaluation sums of every instance that kEvalControl<T> will evaluate this
+// round (one workgroup per instance), into its BCtrl::evf.  Kept apart from
+// the control kernel, whose register footprint would otherwise limit the
+// SpMV passes to two or three waves per SIMD.
+template <int GC, int GR, typename T>
+__device__ inline void evalFields(const BInst& in) {
+    constexpr bool kFp32 = sizeof(T) == 4;
+    BCtrl& c = *in.ctrl;
+    __shared__ int s_go;
+    if (threadIdx.x == 0) {
+        const int ph = c.phase;
+        s_go = !(ph == kPhaseDone || kFp32 != (ph == kPhaseFp32) || c.round_iters <= 0 ||
+                 (!kFp32 && c.skip_eval));
+    }
+    __syncthreads();
+    if (!s_go) return;
+    bEvalFields<GC, GR, T>(in, pickVecs<T>(in), c, c.evf);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kEvalFields(const BInst* __restrict__ insts, int K) {
+    if (static_cast<int>(blockIdx.x) >= K) return;
+    const BInst& in = insts[blockIdx.x];
+    switch (in.preset) {
+        case 0: evalFields<1, 1, T>(in); break;
+        case 1: evalFields<2, 2, T>(in); break;
+        case 2: evalFields<2, 4, T>(in); break;
+        case 3: evalFields<4, 4, T>(in); break;
+        case 4: evalFields<4, 8, T>(in); break;
+        case 5: evalFields<8, 8, T>(in); break;
+        case 6: evalFields<8, 16, T>(in); break;
+        default: evalFields<16, 16, T>(in); break;
+    }
 }
 
 template <typename T>
@@ -1225,6 +1263,14 @@ struct IterPreset {
     int gc, gr;
 };
 
+// MPX_GRAPHS=0 in the environment makes the batch engines and the server
+// launch their rounds directly instead of replaying captured graphs (same
+// kernels and results; for tools that cannot follow graph launches).
+bool graphsDisabled() {
+    const char* v = std::getenv("MPX_GRAPHS");
+    return v != nullptr && v[0] == '0';
+}
+
 // Device copy of the options the per-instance control logic reads.
 BOptions makeBOptions(const mpx_pdlp_options& o) {
     BOptions b{};
@@ -1268,6 +1314,7 @@ public:
         MPX_HIP(hipEventCreate(&ev1_));
         bo_ = makeBOptions(o);
         mixed_ = o.precision == MPX_PREC_MIXED;
+        graph_failed_ = graphsDisabled();
     }
     ~BatchEngine() {
         for (auto& kv : graphs_) kv.second.reset();
@@ -1595,7 +1642,11 @@ void BatchEngine::enqueueRound(const IterPreset& p, bool timed) {
         kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
     }
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
-    if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    if (mixed_) {
+        kEvalFields<float><<<K, kBlock, 0, st_>>>(di, K);
+        kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    }
+    kEvalFields<double><<<K, kBlock, 0, st_>>>(di, K);
     kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
 }
 
@@ -1611,7 +1662,11 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     for (int k = 1; k <= bo_.N; ++k) ilIterate(st_, grid_, di, w, il_, f32, cwp_, cwd_, k);
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
     ilScatter(st_, grid_, di, w, il_, f32);
-    if (f32) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    if (f32) {
+        kEvalFields<float><<<K, kBlock, 0, st_>>>(di, K);
+        kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    }
+    kEvalFields<double><<<K, kBlock, 0, st_>>>(di, K);
     kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
 }
 
@@ -2088,6 +2143,7 @@ public:
         MPX_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         bo_ = makeBOptions(o);
         mixed_ = o.precision == MPX_PREC_MIXED;
+        graph_failed_ = graphsDisabled();
         size_t freeb = 0, total = 0;
         MPX_HIP(hipMemGetInfo(&freeb, &total));
         const size_t want = budget > 0.0 ? static_cast<size_t>(budget) : freeb / 2;
@@ -2564,7 +2620,11 @@ private:
             for (auto& b : bases_) ilIterate(st_, grid_, di, b->owner, b->il, f32, 4, 1, k);
         }
         for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, f32);
-        if (mixed_) kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+        if (mixed_) {
+            kEvalFields<float><<<K, kBlock, 0, st_>>>(di, K);
+            kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
+        }
+        kEvalFields<double><<<K, kBlock, 0, st_>>>(di, K);
         kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
     }
 

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -756,7 +757,13 @@ void SingleSolver::runBlock(int batch, Graph& graph, bool allowGraph) {
     hipStream_t st = stream_.s;
     pushScalars();
     const int N = o_.termination_eval_frequency;
-    if (allowGraph && batch == N) {
+    // MPX_GRAPHS=0: direct launches (same kernels), for tools that cannot
+    // follow graph launches.
+    static const bool graphsOff = [] {
+        const char* v = std::getenv("MPX_GRAPHS");
+        return v != nullptr && v[0] == '0';
+    }();
+    if (allowGraph && batch == N && !graphsOff) {
         if (graph.e == nullptr) {
             bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
