@@ -46,9 +46,6 @@ struct BCtrl {
     int32_t power_iters, g_row, g_col, use_lds;
     int32_t round_iters, skip_eval;
     unsigned long long t_finish;  // device wall clock when the instance finished
-    // This round's evaluation sums (column fields, then row fields), written
-    // by kEvalFields for kEvalControl.
-    double evf[10];
 };
 
 enum BPhase { kPhaseFp32 = 0, kPhaseFp64 = 1, kPhasePolish = 2, kPhaseDone = 3 };
@@ -128,18 +125,30 @@ __device__ inline T bDot(const int* __restrict__ starts, const int* __restrict__
 constexpr int kNumEval = static_cast<int>(kNumColFields) + static_cast<int>(kNumRowFields);
 // Columns (rows) per lane group and pass in the batched evaluation.
 constexpr int kEvalCW = 4;
-static_assert(kNumEval == 10, "BCtrl::evf holds the evaluation sums");
+// An instance's evaluation is split into parts of about kEvalChunk matrix
+// entries plus vector elements (at most kEvalMaxParts), each a workgroup;
+// the part count depends only on the instance's size, so its sums do not
+// depend on the rest of the batch.
+constexpr int kEvalMaxParts = 64;
+constexpr long long kEvalChunk = 1 << 17;
+
+__device__ inline int evalParts(const BInst& in) {
+    const long long work = static_cast<long long>(in.nnz) + in.n + in.m;
+    const long long p = (work + kEvalChunk - 1) / kEvalChunk;
+    return static_cast<int>(p < 1 ? 1 : (p > kEvalMaxParts ? kEvalMaxParts : p));
+}
 
 struct BEval {
     double fpe, rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj, dx2, dy2;
 };
 
-// FP64 evaluation sums of (xh, yh) against (x0, y0), same formulas as the
-// single path (src/lp/pdlp.cpp:1076-1202): the kNumColFields column fields
-// then the kNumRowFields row fields, stored to dst by thread 0.
+// FP64 evaluation sums of (xh, yh) against (x0, y0) over the columns
+// [jb, je) and rows [ib, ie), same formulas as the single path
+// (src/lp/pdlp.cpp:1076-1202): the kNumColFields column fields then the
+// kNumRowFields row fields, stored to dst by thread 0.
 template <int GC, int GR, typename T>
 __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCtrl& c,
-                                   double* dst) {
+                                   double* dst, int jb, int je, int ib, int ie) {
     const int tid = threadIdx.x;
     // Each lane group takes kEvalCW columns (rows) per pass, PC (PR) apart,
     // with their loads issued together: a thread still visits its columns in
@@ -149,14 +158,14 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
     constexpr int PC = kBlock / GC, PR = kBlock / GR;
     const int lc = tid % GC, gc = tid / GC;
     double cf[kNumColFields] = {0, 0, 0, 0, 0, 0};
-    for (int base = 0; base < in.n; base += kEvalCW * PC) {
+    for (int base = jb; base < je; base += kEvalCW * PC) {
         int j[kEvalCW], qb[kEvalCW], qe[kEvalCW];
         double aty[kEvalCW], atdy[kEvalCW], xj[kEvalCW], x0j[kEvalCW];
         int most = 0;
 #pragma unroll
         for (int w = 0; w < kEvalCW; ++w) {
             j[w] = base + gc + w * PC;
-            const bool ok = j[w] < in.n;
+            const bool ok = j[w] < je;
             qb[w] = ok ? in.cs[j[w]] + lc : 0;
             qe[w] = ok ? in.cs[j[w] + 1] : 0;
             xj[w] = ok ? static_cast<double>(v.xh[j[w]]) : 0.0;
@@ -182,7 +191,7 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
         for (int w = 0; w < kEvalCW; ++w) {
             aty[w] = groupSum<GC>(aty[w]);
             atdy[w] = groupSum<GC>(atdy[w]);
-            if (lc == 0 && j[w] < in.n) {
+            if (lc == 0 && j[w] < je) {
                 const int jj = j[w];
                 const double dx = xj[w] - x0j[w];
                 cf[kDx2] += dx * dx;
@@ -210,14 +219,14 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
     }
     const int lr = tid % GR, gr = tid / GR;
     double rf[kNumRowFields] = {0, 0, 0, 0};
-    for (int base = 0; base < in.m; base += kEvalCW * PR) {
+    for (int base = ib; base < ie; base += kEvalCW * PR) {
         int i[kEvalCW], qb[kEvalCW], qe[kEvalCW];
         double ax[kEvalCW], yv[kEvalCW], y0v[kEvalCW];
         int most = 0;
 #pragma unroll
         for (int w = 0; w < kEvalCW; ++w) {
             i[w] = base + gr + w * PR;
-            const bool ok = i[w] < in.m;
+            const bool ok = i[w] < ie;
             qb[w] = ok ? in.rs[i[w]] + lr : 0;
             qe[w] = ok ? in.rs[i[w] + 1] : 0;
             yv[w] = ok ? static_cast<double>(v.yh[i[w]]) : 0.0;
@@ -236,7 +245,7 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
 #pragma unroll
         for (int w = 0; w < kEvalCW; ++w) {
             ax[w] = groupSum<GR>(ax[w]);
-            if (lr == 0 && i[w] < in.m) {
+            if (lr == 0 && i[w] < ie) {
                 const int ii = i[w];
                 const double lo = in.rl[ii], hi = in.ru[ii];
                 const double vs = ax[w] - clampRef(ax[w], lo, hi);

@@ -586,23 +586,35 @@ public:
         // Power iteration in captured groups; the host checks after each.
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
+        SuArgs pa = a_;  // tile lists of the power steps (compacted below)
         bool graphed =
-            graphs && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            graphs && !graphsDisabled() &&
+            hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (graphed) {
-            powerGroup(st, k);
+            powerGroup(st, k, pa);
             graphed = hipStreamEndCapture(st, &g) == hipSuccess && g != nullptr &&
                       hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
             if (!graphed) (void)hipGetLastError();
         }
-        int left = 1;
+        int left = 1, running = k;
         int maxIter = 0;
         for (const BOwner& w : owners_) maxIter = std::max(maxIter, w.sv_max_iter);
+        std::vector<SuPower> hp;
         for (int done = 0; left > 0 && done <= maxIter + kPowerGroup; done += kPowerGroup) {
             if (graphed) MPX_HIP(hipGraphLaunch(ge, st));
-            else powerGroup(st, k);
-            kPwCount<<<1, kBlock, 0, st>>>(a_, k);
+            else powerGroup(st, k, pa);
+            kPwCount<<<1, kBlock, 0, st>>>(pa, k);
             MPX_HIP(hipMemcpyAsync(&left, a_.remaining, sizeof(int), hipMemcpyDeviceToHost, st));
             MPX_HIP(hipStreamSynchronize(st));
+            // Once at most half of the listed matrices still iterate, the
+            // power steps get tile lists of those alone (same tiles in the
+            // same order per matrix, so every matrix's sums are unchanged).
+            if (!graphed && left > 0 && 2 * left <= running) {
+                hp.resize(static_cast<size_t>(k));
+                MPX_HIP(hipMemcpyAsync(hp.data(), a_.pw, sizeof(SuPower) * k, hipMemcpyDeviceToHost, st));
+                MPX_HIP(hipStreamSynchronize(st));
+                running = compactPower(st, k, hp, pa);
+            }
         }
         if (ge) (void)hipGraphExecDestroy(ge);
         if (g) (void)hipGraphDestroy(g);
@@ -612,16 +624,52 @@ public:
     }
 
 private:
-    void powerGroup(hipStream_t st, int k) {
+    void powerGroup(hipStream_t st, int k, const SuArgs& a) {
         auto grid = [](int tiles) { return std::max(1, std::min(tiles, 4096)); };
         for (int s = 0; s < kPowerGroup; ++s) {
-            kPwRows<<<grid(a_.nprowT), kBlock, 0, st>>>(a_);
-            kPwCols<<<std::max(1, a_.npcolT), kBlock, 0, st>>>(a_);
-            kPwDot<<<k, kBlock, 0, st>>>(a_);
-            kPwResid<<<std::max(1, a_.ncolT), kBlock, 0, st>>>(a_);
-            kPwNorm<<<k, kBlock, 0, st>>>(a_);
-            kPwScale<<<grid(a_.ncolT), kBlock, 0, st>>>(a_);
+            kPwRows<<<grid(a.nprowT), kBlock, 0, st>>>(a);
+            kPwCols<<<std::max(1, a.npcolT), kBlock, 0, st>>>(a);
+            kPwDot<<<k, kBlock, 0, st>>>(a);
+            kPwResid<<<std::max(1, a.ncolT), kBlock, 0, st>>>(a);
+            kPwNorm<<<k, kBlock, 0, st>>>(a);
+            kPwScale<<<grid(a.ncolT), kBlock, 0, st>>>(a);
         }
+    }
+
+    // Rewrites the power steps' tile lists (row-product, column-product and
+    // column tiles) and the per-matrix ranges for the matrices still in
+    // kPwRun, in cohort order; the full lists are uploaded again by the next
+    // run().  Returns the number of those matrices.
+    int compactPower(hipStream_t st, int k, const std::vector<SuPower>& hp, SuArgs& pa) {
+        std::vector<SuTile> prow, pcol, col;
+        std::vector<SuRange> rng = ranges_;
+        int running = 0;
+        for (int o = 0; o < k; ++o) {
+            if (hp[static_cast<size_t>(o)].status != kPwRun) continue;
+            ++running;
+            SuRange& r = rng[static_cast<size_t>(o)];
+            const SuRange& f = ranges_[static_cast<size_t>(o)];
+            r.prow0 = static_cast<int32_t>(prow.size());
+            prow.insert(prow.end(), prowT_.begin() + f.prow0, prowT_.begin() + f.prow0 + f.nprow);
+            r.pcol0 = static_cast<int32_t>(pcol.size());
+            pcol.insert(pcol.end(), pcolT_.begin() + f.pcol0, pcolT_.begin() + f.pcol0 + f.npcol);
+            r.col0 = static_cast<int32_t>(col.size());
+            col.insert(col.end(), colT_.begin() + f.col0, colT_.begin() + f.col0 + f.ncol);
+        }
+        auto up = [&](const void* dst, const void* src, size_t bytes) {
+            if (bytes)
+                MPX_HIP(hipMemcpyAsync(const_cast<void*>(dst), src, bytes, hipMemcpyHostToDevice, st));
+        };
+        up(pa.prowT, prow.data(), sizeof(SuTile) * prow.size());
+        up(pa.pcolT, pcol.data(), sizeof(SuTile) * pcol.size());
+        up(pa.colT, col.data(), sizeof(SuTile) * col.size());
+        up(pa.ranges, rng.data(), sizeof(SuRange) * rng.size());
+        // The copies come from pageable vectors that end with this call.
+        MPX_HIP(hipStreamSynchronize(st));
+        pa.nprowT = static_cast<int>(prow.size());
+        pa.npcolT = static_cast<int>(pcol.size());
+        pa.ncolT = static_cast<int>(col.size());
+        return running;
     }
 
     std::vector<BOwner> owners_;

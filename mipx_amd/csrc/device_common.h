@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -34,6 +36,18 @@ const char* lastError();
 struct DeviceError {};
 
 inline int ceilDiv(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
+
+// MPX_GRAPHS=0 in the environment: every captured launch sequence (PDHG
+// blocks, power-iteration groups, batch and server rounds) is launched
+// directly instead -- same kernels, same results -- for tools that cannot
+// follow graph launches.
+inline bool graphsDisabled() {
+    static const bool off = [] {
+        const char* v = std::getenv("MPX_GRAPHS");
+        return v != nullptr && v[0] == '0';
+    }();
+    return off;
+}
 
 // std::clamp(v, lo, hi) of libstdc++: (v < lo) ? lo : (hi < v) ? hi : v.
 template <typename T>

@@ -435,7 +435,7 @@ __global__ void kTileFill(const BInst* __restrict__ insts, int K, const int* __r
 // reference's loop (src/lp/pdlp.cpp:2228-2469) and the polish phase switch
 // (2477-2676); then sets the instance's iteration count for the next round.
 template <int GC, int GR, typename T>
-__device__ void evalControl(const BInst& in, const BOptions& o) {
+__device__ void evalControl(const BInst& in, const BOptions& o, const double* parts) {
     BCtrl& c = *in.ctrl;
     const int tid = threadIdx.x;
     constexpr bool kFp32 = sizeof(T) == 4;
@@ -460,8 +460,21 @@ __device__ void evalControl(const BInst& in, const BOptions& o) {
     }
     const bool polish = s_phase == kPhasePolish;
     const int ran = s_ran;
-    // The sums come from kEvalFields (same round, same stream).
-    const BEval e = bEvalFinish(c.evf, c);
+    // The sums of the instance's evaluation parts (kEvalParts, same round),
+    // combined in part order.
+    __shared__ double fs[kNumEval];
+    if (tid == 0) {
+        const int P = evalParts(in);
+        for (int f = 0; f < kNumEval; ++f) fs[f] = parts[f];
+        for (int p = 1; p < P; ++p)
+            for (int f = 0; f < kNumEval; ++f) {
+                const double x = parts[p * kNumEval + f];
+                const bool mx = f == kDmax || f == kNumColFields + kPmax;
+                fs[f] = mx ? maxRef(fs[f], x) : fs[f] + x;
+            }
+    }
+    __syncthreads();
+    const BEval e = bEvalFinish(fs, c);
     __syncthreads();
     if (tid == 0) {
         c.inner += ran;
@@ -646,56 +659,113 @@ __device__ void evalControl(const BInst& in, const BOptions& o) {
     if (tid == 0) stampDone(c);
 }
 
-// Evaluation sums of every instance that kEvalControl<T> will evaluate this
-// round (one workgroup per instance), into its BCtrl::evf.  Kept apart from
-// the control kernel, whose register footprint would otherwise limit the
-// SpMV passes to two or three waves per SIMD.
-template <int GC, int GR, typename T>
-__device__ inline void evalFields(const BInst& in) {
+// ---- evaluation in parts ------------------------------------------------------
+// Whether kEvalControl<T> evaluates the instance this round.
+template <typename T>
+__device__ inline bool evaluatedNow(const BCtrl& c) {
     constexpr bool kFp32 = sizeof(T) == 4;
-    BCtrl& c = *in.ctrl;
-    __shared__ int s_go;
-    if (threadIdx.x == 0) {
-        const int ph = c.phase;
-        s_go = !(ph == kPhaseDone || kFp32 != (ph == kPhaseFp32) || c.round_iters <= 0 ||
-                 (!kFp32 && c.skip_eval));
-    }
-    __syncthreads();
-    if (!s_go) return;
-    bEvalFields<GC, GR, T>(in, pickVecs<T>(in), c, c.evf);
+    const int ph = c.phase;
+    return !(ph == kPhaseDone || kFp32 != (ph == kPhaseFp32) || c.round_iters <= 0 ||
+             (!kFp32 && c.skip_eval));
 }
 
+// off[i] = first part of instance i among the instances kEvalControl<T>
+// evaluates this round, off[K] = all parts.  One block, instance order.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void kEvalFields(const BInst* __restrict__ insts, int K) {
-    if (static_cast<int>(blockIdx.x) >= K) return;
-    const BInst& in = insts[blockIdx.x];
-    switch (in.preset) {
-        case 0: evalFields<1, 1, T>(in); break;
-        case 1: evalFields<2, 2, T>(in); break;
-        case 2: evalFields<2, 4, T>(in); break;
-        case 3: evalFields<4, 4, T>(in); break;
-        case 4: evalFields<4, 8, T>(in); break;
-        case 5: evalFields<8, 8, T>(in); break;
-        case 6: evalFields<8, 16, T>(in); break;
-        default: evalFields<16, 16, T>(in); break;
+__global__ __launch_bounds__(1024) void kEvalScan(const BInst* __restrict__ insts, int K,
+                                                  int* __restrict__ off) {
+    __shared__ int part[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < K; base += 1024) {
+        const int k = base + static_cast<int>(threadIdx.x);
+        int p = 0;
+        if (k < K && evaluatedNow<T>(*insts[k].ctrl)) p = evalParts(insts[k]);
+        part[threadIdx.x] = p;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const int a = threadIdx.x >= static_cast<unsigned>(d) ? part[threadIdx.x - d] : 0;
+            __syncthreads();
+            part[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (k < K) off[k] = carry + part[threadIdx.x] - p;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += part[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) off[K] = carry;
+}
+
+// Part p of P: columns and rows in P equal slices.
+template <int GC, int GR, typename T>
+__device__ inline void evalPart(const BInst& in, int p, int P, double* dst) {
+    const long long cspan = (in.n + P - 1) / P, rspan = (in.m + P - 1) / P;
+    const int jb = static_cast<int>(min(static_cast<long long>(in.n), p * cspan));
+    const int je = static_cast<int>(min(static_cast<long long>(in.n), jb + cspan));
+    const int ib = static_cast<int>(min(static_cast<long long>(in.m), p * rspan));
+    const int ie = static_cast<int>(min(static_cast<long long>(in.m), ib + rspan));
+    bEvalFields<GC, GR, T>(in, pickVecs<T>(in), *in.ctrl, dst, jb, je, ib, ie);
+}
+
+// Every part of this round's evaluations, over a fixed grid: part t belongs
+// to the last instance i with off[i] <= t; its sums go to
+// parts[(i * kEvalMaxParts + p) * kNumEval ...].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kEvalParts(const BInst* __restrict__ insts, int K,
+                                                     const int* __restrict__ off,
+                                                     double* __restrict__ parts) {
+    const int total = off[K];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = K - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (off[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BInst& in = insts[lo];
+        const int p = t - off[lo], P = evalParts(in);
+        double* dst = parts + (static_cast<size_t>(lo) * kEvalMaxParts + p) * kNumEval;
+        switch (in.preset) {
+            case 0: evalPart<1, 1, T>(in, p, P, dst); break;
+            case 1: evalPart<2, 2, T>(in, p, P, dst); break;
+            case 2: evalPart<2, 4, T>(in, p, P, dst); break;
+            case 3: evalPart<4, 4, T>(in, p, P, dst); break;
+            case 4: evalPart<4, 8, T>(in, p, P, dst); break;
+            case 5: evalPart<8, 8, T>(in, p, P, dst); break;
+            case 6: evalPart<8, 16, T>(in, p, P, dst); break;
+            default: evalPart<16, 16, T>(in, p, P, dst); break;
+        }
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kEvalControl(const BInst* __restrict__ insts, int K,
-                                                        BOptions o) {
+                                                        BOptions o, const double* __restrict__ parts) {
     if (static_cast<int>(blockIdx.x) >= K) return;
     const BInst& in = insts[blockIdx.x];
+    const double* mine = parts + static_cast<size_t>(blockIdx.x) * kEvalMaxParts * kNumEval;
     switch (in.preset) {
-        case 0: evalControl<1, 1, T>(in, o); break;
-        case 1: evalControl<2, 2, T>(in, o); break;
-        case 2: evalControl<2, 4, T>(in, o); break;
-        case 3: evalControl<4, 4, T>(in, o); break;
-        case 4: evalControl<4, 8, T>(in, o); break;
-        case 5: evalControl<8, 8, T>(in, o); break;
-        case 6: evalControl<8, 16, T>(in, o); break;
-        default: evalControl<16, 16, T>(in, o); break;
+        case 0: evalControl<1, 1, T>(in, o, mine); break;
+        case 1: evalControl<2, 2, T>(in, o, mine); break;
+        case 2: evalControl<2, 4, T>(in, o, mine); break;
+        case 3: evalControl<4, 4, T>(in, o, mine); break;
+        case 4: evalControl<4, 8, T>(in, o, mine); break;
+        case 5: evalControl<8, 8, T>(in, o, mine); break;
+        case 6: evalControl<8, 16, T>(in, o, mine); break;
+        default: evalControl<16, 16, T>(in, o, mine); break;
     }
+}
+
+// The evaluation and control of one round for the instances in precision T:
+// part offsets, the parts, then the per-instance control.
+template <typename T>
+void enqueueEval(hipStream_t st, int grid, const BInst* di, int K, const BOptions& bo, int* off,
+                 double* parts) {
+    kEvalScan<T><<<1, 1024, 0, st>>>(di, K, off);
+    kEvalParts<T><<<grid, kBlock, 0, st>>>(di, K, off, parts);
+    kEvalControl<T><<<K, kBlock, 0, st>>>(di, K, bo, parts);
 }
 
 // Counts unfinished instances (one thread per instance, sum by atomics on an
@@ -860,6 +930,20 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() {
         if (p) (void)hipFree(p);
+    }
+};
+
+// Part offsets and part sums of the per-round evaluation of K instances.
+struct EvalBufs {
+    std::unique_ptr<DevBuf> buf;
+    int* off = nullptr;
+    double* parts = nullptr;
+    void alloc(int K) {
+        const size_t offb = (static_cast<size_t>(K + 1) * sizeof(int) + 255) & ~size_t(255);
+        const size_t partb = static_cast<size_t>(K) * kEvalMaxParts * kNumEval * sizeof(double);
+        buf = std::make_unique<DevBuf>(offb + partb);
+        off = static_cast<int*>(buf->p);
+        parts = reinterpret_cast<double*>(static_cast<char*>(buf->p) + offb);
     }
 };
 
@@ -1263,14 +1347,6 @@ struct IterPreset {
     int gc, gr;
 };
 
-// MPX_GRAPHS=0 in the environment makes the batch engines and the server
-// launch their rounds directly instead of replaying captured graphs (same
-// kernels and results; for tools that cannot follow graph launches).
-bool graphsDisabled() {
-    const char* v = std::getenv("MPX_GRAPHS");
-    return v != nullptr && v[0] == '0';
-}
-
 // Device copy of the options the per-instance control logic reads.
 BOptions makeBOptions(const mpx_pdlp_options& o) {
     BOptions b{};
@@ -1430,6 +1506,7 @@ private:
     // Lane-per-instance mode (every instance shares owner 0).
     bool interleaved_ = false;
     std::unique_ptr<DevBuf> d_il_buf_;
+    EvalBufs ev_;
     IlSet il_;
     void enqueueInterleavedRound(bool timed);
 };
@@ -1448,6 +1525,7 @@ void BatchEngine::setup() {
         bytes += instBytes(lp.num_rows, lp.num_cols, inst_lo_[i] != nullptr);
     }
     arena_ = std::make_unique<DevBuf>(bytes);
+    ev_.alloc(std::max(1, ni));
     Carve cv{static_cast<char*>(arena_->p), 0, bytes};
     owners_.resize(static_cast<size_t>(no));
     owner_preset_.resize(static_cast<size_t>(no));
@@ -1642,12 +1720,8 @@ void BatchEngine::enqueueRound(const IterPreset& p, bool timed) {
         kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
     }
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
-    if (mixed_) {
-        kEvalFields<float><<<K, kBlock, 0, st_>>>(di, K);
-        kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
-    }
-    kEvalFields<double><<<K, kBlock, 0, st_>>>(di, K);
-    kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    if (mixed_) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
+    enqueueEval<double>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
 }
 
 void BatchEngine::enqueueInterleavedRound(bool timed) {
@@ -1662,12 +1736,8 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     for (int k = 1; k <= bo_.N; ++k) ilIterate(st_, grid_, di, w, il_, f32, cwp_, cwd_, k);
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
     ilScatter(st_, grid_, di, w, il_, f32);
-    if (f32) {
-        kEvalFields<float><<<K, kBlock, 0, st_>>>(di, K);
-        kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
-    }
-    kEvalFields<double><<<K, kBlock, 0, st_>>>(di, K);
-    kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
+    if (f32) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
+    enqueueEval<double>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
 }
 
 void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us, int* slots) {
@@ -2151,6 +2221,7 @@ public:
         const size_t S = static_cast<size_t>(S_);
         tab_ = std::make_unique<DevBuf>(sizeof(BInst) * S);
         ctrl_ = std::make_unique<DevBuf>(sizeof(BCtrl) * S);
+        ev_.alloc(S_);
         cnt_ = std::make_unique<DevBuf>(sizeof(int) * 4 * S);
         recs_ = std::make_unique<DevBuf>(pad(S, sizeof(PRec<float>)) + pad(S, sizeof(PRec<double>)));
         rec32_ = static_cast<PRec<float>*>(recs_->p);
@@ -2620,12 +2691,8 @@ private:
             for (auto& b : bases_) ilIterate(st_, grid_, di, b->owner, b->il, f32, 4, 1, k);
         }
         for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, f32);
-        if (mixed_) {
-            kEvalFields<float><<<K, kBlock, 0, st_>>>(di, K);
-            kEvalControl<float><<<K, kBlock, 0, st_>>>(di, K, bo_);
-        }
-        kEvalFields<double><<<K, kBlock, 0, st_>>>(di, K);
-        kEvalControl<double><<<K, kBlock, 0, st_>>>(di, K, bo_);
+        if (mixed_) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
+        enqueueEval<double>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
     }
 
     void launchRound() {
@@ -2775,6 +2842,7 @@ private:
     hipStream_t st_ = nullptr;
     std::unique_ptr<DevicePool> pool_;
     std::unique_ptr<DevBuf> tab_, ctrl_, cnt_, newi_, newa_, news_, ptrs_, recs_;
+    EvalBufs ev_;
     PRec<float>* rec32_ = nullptr;
     PRec<double>* rec64_ = nullptr;
     void* tiles_mem_ = nullptr;

@@ -667,7 +667,7 @@ void SingleSolver::powerIteration() {
     // Steps run in captured groups; one host check per group.
     const int per = 16;
     Graph g;
-    bool useGraph = true;
+    bool useGraph = !graphsDisabled();
     PowerState hs{};
     int launched = 0;
     while (launched < o_.sv_max_iter) {
@@ -757,13 +757,7 @@ void SingleSolver::runBlock(int batch, Graph& graph, bool allowGraph) {
     hipStream_t st = stream_.s;
     pushScalars();
     const int N = o_.termination_eval_frequency;
-    // MPX_GRAPHS=0: direct launches (same kernels), for tools that cannot
-    // follow graph launches.
-    static const bool graphsOff = [] {
-        const char* v = std::getenv("MPX_GRAPHS");
-        return v != nullptr && v[0] == '0';
-    }();
-    if (allowGraph && batch == N && !graphsOff) {
+    if (allowGraph && batch == N && !graphsDisabled()) {
         if (graph.e == nullptr) {
             bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
