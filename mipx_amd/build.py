@@ -67,10 +67,11 @@ def build_host():
     flags = ["-std=c++20", "-O2", "-Wall", "-fPIC"]
     _run(["g++"] + flags + inc + ["-shared", "-o", os.path.join(PKG, "libmipx_host.so"),
                                   os.path.join(src, "pdlp.cpp"), os.path.join(src, "sparse_matrix.cpp"),
-                                  "-L" + PKG, "-lmipx_pdlp", "-Wl,-rpath,$ORIGIN"])
+                                  os.path.join(src, "mps_io.cpp"),
+                                  "-L" + PKG, "-lmipx_pdlp", "-lz", "-Wl,-rpath,$ORIGIN"])
     bind = os.path.join(PKG, "bin")
     os.makedirs(bind, exist_ok=True)
-    for exe, extra in (("mipx_solve", ["-lmipx_lpgen"]), ("host_tests", [])):
+    for exe, extra in (("mipx_solve", ["-lmipx_lpgen"]), ("host_tests", []), ("io_tests", ["-lz"])):
         _run(["g++"] + flags + inc + ["-o", os.path.join(bind, exe), os.path.join(src, exe + ".cpp"),
                                       "-L" + PKG, "-lmipx_host", "-lmipx_pdlp"] + extra +
              ["-Wl,-rpath,$ORIGIN/.."])

@@ -469,7 +469,8 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         for (int p = 1; p < P; ++p)
             for (int f = 0; f < kNumEval; ++f) {
                 const double x = parts[p * kNumEval + f];
-                const bool mx = f == kDmax || f == kNumColFields + kPmax;
+                const bool mx = f == static_cast<int>(kDmax) ||
+                                f == static_cast<int>(kNumColFields) + static_cast<int>(kPmax);
                 fs[f] = mx ? maxRef(fs[f], x) : fs[f] + x;
             }
     }

@@ -4,12 +4,15 @@
 // 491-558, 572-609): builds a PdlpSolver with verbose / time limit /
 // optimality norm, prints "PDLP backend: GPU" and the stable summary lines
 // (Status / Objective / Iterations / Work units / Time) that the reference's
-// tests/perf scripts parse.  Input is a seeded synthetic LP (the library's
-// generator); file input is outside this build.
+// tests/perf scripts parse.  Input is an MPS file (the reference CLI's
+// positional argument; .mps or .mps.gz, read as its LP relaxation) or a
+// seeded synthetic LP from the library's generator; --write-mps stores the
+// problem that was solved.
 //
-//   mipx_solve --gen c0|c1|c2|c3 [--seed S] [--rows M] [--cols N] [--pdlp]
-//              [--time-limit S] [--pdlp-opt-norm l2|linf] [--precision fp64|mixed]
-//              [--tol T] [--quiet] [--print-backend]
+//   mipx_solve [FILE.mps[.gz] | --gen c0|c1|c2|c3 [--seed S] [--rows M] [--cols N]]
+//              [--pdlp] [--time-limit S] [--pdlp-opt-norm l2|linf]
+//              [--precision fp64|mixed] [--tol T] [--quiet] [--print-backend]
+//              [--write-mps OUT.mps] [--max-iter N]
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -18,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "mipx/io.h"
 #include "mipx/pdlp.h"
 
 extern "C" {
@@ -102,11 +106,12 @@ static mipx::LpProblem generate(const std::string& kind, uint64_t seed, int m, i
 }
 
 int main(int argc, char** argv) {
-    std::string kind = "c0", precision = "fp64";
+    std::string kind = "c0", precision = "fp64", mps, write_mps;
     uint64_t seed = 0xC0;
     int m = 0, n = 0;
     bool verbose = true, print_backend = false;
     double time_limit = -1.0, tol = 1e-6;
+    int max_iter = 0;
     mipx::PdlpOptimalityNorm norm = mipx::PdlpOptimalityNorm::L2;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -134,12 +139,22 @@ int main(int argc, char** argv) {
             }
         } else if (a == "--quiet") verbose = false;
         else if (a == "--print-backend") print_backend = true;
+        else if (a == "--write-mps") write_mps = next();
+        else if (a == "--max-iter") max_iter = std::atoi(next().c_str());
+        else if (a.size() > 4 && a[0] != '-') mps = a;
         else {
             std::fprintf(stderr, "Error: unknown option %s\n", a.c_str());
             return 1;
         }
     }
-    const mipx::LpProblem lp = generate(kind, seed, m, n);
+    mipx::LpProblem lp;
+    try {
+        lp = mps.empty() ? generate(kind, seed, m, n) : mipx::readMps(mps);
+        if (!write_mps.empty()) mipx::writeMps(write_mps, lp);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "Error: %s\n", e.what());
+        return 1;
+    }
     if (verbose)
         std::printf("LP %s: %d rows, %d cols, %d nonzeros\n", lp.name.c_str(), lp.num_rows,
                     lp.num_cols, lp.matrix.numNonzeros());
@@ -150,6 +165,7 @@ int main(int argc, char** argv) {
     o.max_solve_seconds = time_limit;
     o.optimality_norm = norm;
     o.primal_tol = o.dual_tol = o.optimality_tol = tol;
+    if (max_iter > 0) o.max_iter = max_iter;
     o.precision = precision == "mixed" ? mipx::PdlpPrecision::Mixed : mipx::PdlpPrecision::FP64;
     solver.setOptions(o);
     solver.load(lp);

@@ -36,3 +36,47 @@ def test_cli_stable_lines_match_oracle():
     assert abs(obj - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
     it = int(lines["Iterations"])
     assert float(lines["Work units"]) == pytest.approx(it * (4.0 * lp.nnz + 2.0 * (300 + 750)))
+
+
+def _stable_lines(out):
+    return dict(re.findall(r"^(PDLP backend|Status|Objective|Iterations|Work units): (.*)$", out,
+                           flags=re.M))
+
+
+def test_cli_mps_round_trip(tmp_path):
+    """A generated LP written as MPS (--write-mps) and solved from the file
+    gives the same stable lines as solving it directly."""
+    mps = str(tmp_path / "c0.mps")
+    base = [os.path.join(BIN, "mipx_solve"), "--pdlp", "--print-backend", "--quiet"]
+    r1 = subprocess.run(base + ["--gen", "c0", "--seed", "101", "--rows", "300", "--cols", "750",
+                                "--write-mps", mps], capture_output=True, text=True, timeout=120)
+    r2 = subprocess.run(base + [mps], capture_output=True, text=True, timeout=120)
+    print(r1.stdout, r2.stdout)
+    assert r1.returncode == 0 and r2.returncode == 0
+    a, b = _stable_lines(r1.stdout), _stable_lines(r2.stdout)
+    assert a["Status"] == "Optimal" and a == b
+
+
+def test_cli_reads_reference_tiny_mps():
+    """tests/data/tiny.mps of the reference read as its LP relaxation (the
+    fields the reference's MPS test expects): x2 + x3 = 7 cannot hold with
+    x2 <= 1 and x3 <= 5; neither the oracle nor the GPU path finds a
+    certificate within 20,000 iterations, and both stop at the limit."""
+    import numpy as np
+    from mipx_amd.lp import from_dense
+    from lp_cases import GOLDEN
+    r = subprocess.run([os.path.join(BIN, "mipx_solve"), os.path.join(GOLDEN, "tiny.mps"),
+                        "--pdlp", "--print-backend", "--quiet", "--max-iter", "20000"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0
+    lines = _stable_lines(r.stdout)
+    inf = float("inf")
+    lp = from_dense(np.array([[2.0, 1.0, 0.0], [1.0, 0.0, 1.0], [0.0, 1.0, 1.0]]),
+                    [1.0, 3.0, 2.0], [0.0, 0.0, 1.0], [8.0, 1.0, 5.0],
+                    [-inf, 5.0, 7.0], [10.0, inf, 7.0])
+    ro, _, _ = ob.solve(lp, verbose=0, max_iter=20000, primal_tol=1e-6, dual_tol=1e-6,
+                        optimality_tol=1e-6)
+    names = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "Iteration limit"}
+    assert lines["Status"] == names[ro["status"]] == "Iteration limit"
+    assert int(lines["Iterations"]) == ro["iterations"] == 20000
