@@ -1,4 +1,5 @@
-// batch_kernels.h -- batched PDLP: one workgroup per LP instance.
+// batch_kernels.h -- batched PDLP: per-instance state, evaluation sums and
+// certificates (the control logic runs one workgroup per LP instance).
 //
 // Many LPs (branch-and-bound node relaxations sharing one A, or independent
 // LPs of uneven size) run in one launch; each 256-thread workgroup owns one

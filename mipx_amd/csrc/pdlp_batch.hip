@@ -1,18 +1,26 @@
-// pdlp_batch.hip -- batched PDLP (many LPs, one workgroup each, one launch per
-// round) and the continuous-batching serving loop.
+// pdlp_batch.hip -- batched PDLP (many LPs per launch) and the
+// continuous-batching serving loop.
 //
-// Device work per round: one launch of kBatchIterate<float> (instances in
-// their FP32 phase) and one of kBatchIterate<double> (FP64 refinement and
-// polish), each running up to `blocks_per_round` evaluation blocks of N
-// iterations per instance; the pair is captured once in a hipGraph and
-// replayed.  The only host synchronisation per round is a 4-byte read of the
-// number of unfinished instances.
+// A round is N = termination_eval_frequency iterations of every running
+// instance followed by its evaluation and control, captured once in a
+// hipGraph and replayed:
+//   * instances that share one constraint matrix (branch-and-bound node
+//     relaxations) iterate in the lane-per-instance kernels of
+//     batch_interleaved.h: 64 instances per group, one lane each, so a wave
+//     reads the shared matrix once for 64 instances;
+//   * independent LPs iterate in packed tiles (instance, block of columns or
+//     rows) over a fixed grid, each with its own lane-group preset;
+//   * kEvalScan / kEvalParts / kEvalControl evaluate every instance in
+//     size-based parts and run the reference host loop's termination,
+//     certificate, restart, primal-weight and polish decisions per instance
+//     (src/lp/pdlp.cpp:2228-2676) on the device.
+// The host reads one count of unfinished instances per round.
 //
-// Setup is also on the device, one workgroup per distinct matrix: Ruiz,
-// objective/bound scaling, Pock-Chambolle bases, bound/objective rescaling
-// (bit-identical per element to the reference's buildScaledProblem,
-// src/lp/pdlp.cpp:469-669) and the power iteration (pdlp.cpp:748-795).
-// Branch-and-bound instances that share A share all of it and only scale
+// Setup is on the device too: the scaling (Ruiz, Pock-Chambolle,
+// bound/objective rescaling; bit-identical per element to the reference's
+// buildScaledProblem, pdlp.cpp:469-669) and the power iteration
+// (pdlp.cpp:748-795) run as tile passes over a cohort of matrices
+// (batch_setup.h).  Instances that share A share all of it and only scale
 // their own column bounds.
 
 #include <hip/hip_runtime.h>
