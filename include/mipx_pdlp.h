@@ -139,6 +139,13 @@ typedef struct mpx_pdlp_options {
                                          dual/dual_tol, gap/optimality_tol),
                                          evaluated in FP64, is at most this
                                          (default 3), or when FP32 stalls */
+    int32_t spmv_chunk_cols;          /* single LP: the dual step's A x_r in
+                                         column chunks of this many columns,
+                                         each chunk's x_r slice served from
+                                         L2 (partial sums added chunk by
+                                         chunk); 0 = automatic (chunks of
+                                         2^19 columns when n > 2^20), < 0 =
+                                         never */
 } mpx_pdlp_options;
 
 /* Mirrors `mipx::LpResult` (include/mipx/lp_solver.h:11-16) plus the final

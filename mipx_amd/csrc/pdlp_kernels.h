@@ -190,6 +190,35 @@ __global__ __launch_bounds__(kBlock) void kDualStepLong(
     dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, sc, k);
 }
 
+// Dual half step with A x_r in column chunks: chunk c of C holds, row by
+// row, the entries whose columns lie in chunk c (rows in order, the entries
+// of a row in column order; row starts rs_c).  A chunk's slice of x_r fits
+// in L2, so the gathers of one launch hit L2 instead of going to memory.
+// Chunks 0..C-2 leave their row sums in `part` (first writes, the others
+// add); the last adds its own and finishes the row: A x_r = ((p0 + p1) +
+// ...) + p_{C-1}.
+template <int G, typename T, bool STORE_HAT>
+__global__ __launch_bounds__(kBlock) void kDualChunk(
+    int m, const int* __restrict__ rs_c, const int* __restrict__ ci_c, const T* __restrict__ val_c,
+    const T* __restrict__ xr, T* __restrict__ part, int c, int C, T* __restrict__ y,
+    const T* __restrict__ y0, T* __restrict__ yh, const T* __restrict__ sigma_base,
+    const T* __restrict__ rl, const T* __restrict__ ru, const IterScalars* __restrict__ sc,
+    int k) {
+    const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int i = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
+    if (i >= m) return;
+    const T acc = groupDotBE<G>(rs_c[i], rs_c[i + 1], ci_c, val_c, xr, lane);
+    if (lane != 0) return;
+    if (c == 0 && C > 1) {
+        part[i] = acc;
+    } else if (c < C - 1) {
+        part[i] = part[i] + acc;
+    } else {
+        const T axr = C > 1 ? part[i] + acc : acc;
+        dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, sc, k);
+    }
+}
+
 // ---- evaluation (src/lp/pdlp.cpp:1076-1202) --------------------------------
 // Column pass: A'y^ and A'(y^ - y0) per column, then the per-column terms of
 // |dx|^2, <A'dy, dx>, the dual residual (sum of squares and max, original

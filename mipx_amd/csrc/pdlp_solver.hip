@@ -298,6 +298,12 @@ private:
     void powerIteration();
     void convertIterates64to32();
     void convertIterates32to64();
+    // FP32 copy of the chunk-major values (mixed precision).
+    void convertChunks() {
+        if (nch_ && valch32_ && nnz_ > 0)
+            kConvert<float, double><<<ceilDiv(nnz_, kBlock), kBlock, 0, stream_.s>>>(nnz_, valch64_,
+                                                                                     valch32_);
+    }
     template <typename T>
     void adaptStep(double lb, double ub);
     template <typename T>
@@ -341,6 +347,13 @@ private:
     bool adaptive_ = false, have_prev_ = false;
     double *prev_ax_ = nullptr, *prev_x_ = nullptr, *acc_part_ = nullptr, *acc_out_ = nullptr;
     int nbn_ = 1;
+    // Column chunks of the dual step's SpMV (0: one kDualStep launch): chunk-
+    // major CSR (row starts per chunk, then the entries of chunk 0, 1, ...),
+    // its position map into the scaled CSR values, and the partial sums.
+    int nch_ = 0, gch_ = 1;
+    int *rsch_ = nullptr, *cich_ = nullptr, *permch_ = nullptr;
+    double *valch64_ = nullptr, *part64_ = nullptr;
+    float *valch32_ = nullptr, *part32_ = nullptr;
     // Anderson acceleration (pdlp.cpp:293-437): window of aa_w_ restart
     // candidates and residuals in HBM, aa_count_ stored so far.
     int aa_w_ = 0, aa_count_ = 0, nbaa_ = 1;
@@ -387,6 +400,29 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     const size_t pairs = static_cast<size_t>(aa_w_) * static_cast<size_t>(aa_w_ + 1) / 2;
     const size_t accParts = std::max({static_cast<size_t>(nbr_), static_cast<size_t>(nbn_),
                                       static_cast<size_t>(nbaa_) * std::max<size_t>(pairs, 1)});
+    // Chunked dual SpMV: forced by spmv_chunk_cols > 0, automatic (2^19
+    // columns) when x_r is larger than 8 MB in FP64; not with rows long
+    // enough for the one-wave row kernel.
+    {
+        long long cc = o_.spmv_chunk_cols;
+        if (cc == 0) cc = n_ > (1 << 20) ? (1 << 19) : -1;
+        bool longRows = false;
+        if (o_.spmv_group <= 0 && gr_ < kWave) {
+            long long inLong = 0;
+            for (int i = 0; i < m_; ++i) {
+                const int len = lp_.row_starts[i + 1] - lp_.row_starts[i];
+                if (len > 4 * gr_) inLong += len;
+            }
+            longRows = inLong > 0 && 20 * inLong >= static_cast<long long>(nnz_);
+        }
+        if (cc > 0 && cc < n_ && !longRows && m_ > 0) {
+            nch_ = static_cast<int>((n_ + cc - 1) / cc);
+            gch_ = pickGroup(nnz_ / nch_, m_, 0);
+            bytes += padBytes(static_cast<size_t>(nch_) * (m + 1), 4) + 2 * padBytes(nnz, 4) +
+                     padBytes(nnz, 8) + padBytes(m, 8);
+            if (mixed_) bytes += padBytes(nnz, 4) + padBytes(m, 4);
+        }
+    }
     if (adaptive_) bytes += padBytes(m, 8) + padBytes(n, 8);
     if (aa_w_) bytes += 2 * padBytes(static_cast<size_t>(aa_w_) * dim, 8) + padBytes(aa_w_, 8);
     if (adaptive_ || aa_w_) bytes += padBytes(accParts, 8) + padBytes(std::max<size_t>(pairs, 2), 8);
@@ -436,6 +472,17 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     if (adaptive_) {
         prev_ax_ = arena_.take<double>(m);
         prev_x_ = arena_.take<double>(n);
+    }
+    if (nch_) {
+        rsch_ = arena_.take<int>(static_cast<size_t>(nch_) * (m + 1));
+        cich_ = arena_.take<int>(nnz);
+        permch_ = arena_.take<int>(nnz);
+        valch64_ = arena_.take<double>(nnz);
+        part64_ = arena_.take<double>(m);
+        if (mixed_) {
+            valch32_ = arena_.take<float>(nnz);
+            part32_ = arena_.take<float>(m);
+        }
     }
     if (aa_w_) {
         aa_z_ = arena_.take<double>(static_cast<size_t>(aa_w_) * dim);
@@ -511,6 +558,36 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
         MPX_HIP(hipMemcpyAsync(v64_.val, lp_.values, nnz * 8, hipMemcpyHostToDevice, st));
     }
     MPX_HIP(hipMemcpyAsync(cs_, cs.data(), (n + 1) * 4, hipMemcpyHostToDevice, st));
+    // Chunk-major copy of the CSR structure for the chunked dual SpMV (host
+    // index work; the values follow through permch_ after scaling).
+    std::vector<int> rsch, cich, permch;
+    if (nch_ && n_long_r_ > 0) nch_ = 0;  // never with the one-wave row kernel
+    if (nch_) {
+        const long long cw = (static_cast<long long>(n_) + nch_ - 1) / nch_;
+        std::vector<long long> pos(static_cast<size_t>(nch_) + 1, 0);
+        for (size_t q = 0; q < nnz; ++q) ++pos[static_cast<size_t>(lp_.col_indices[q] / cw) + 1];
+        for (int c = 0; c < nch_; ++c) pos[static_cast<size_t>(c) + 1] += pos[static_cast<size_t>(c)];
+        rsch.resize(static_cast<size_t>(nch_) * (m + 1));
+        cich.resize(nnz);
+        permch.resize(nnz);
+        for (int i = 0; i < m_; ++i) {
+            for (int c = 0; c < nch_; ++c)
+                rsch[static_cast<size_t>(c) * (m + 1) + i] = static_cast<int>(pos[static_cast<size_t>(c)]);
+            for (int q = lp_.row_starts[i]; q < lp_.row_starts[i + 1]; ++q) {
+                const size_t c = static_cast<size_t>(lp_.col_indices[q] / cw);
+                const long long at = pos[c]++;
+                cich[static_cast<size_t>(at)] = lp_.col_indices[q];
+                permch[static_cast<size_t>(at)] = q;
+            }
+        }
+        for (int c = 0; c < nch_; ++c)
+            rsch[static_cast<size_t>(c) * (m + 1) + m] = static_cast<int>(pos[static_cast<size_t>(c)]);
+        MPX_HIP(hipMemcpyAsync(rsch_, rsch.data(), rsch.size() * 4, hipMemcpyHostToDevice, st));
+        if (nnz) {
+            MPX_HIP(hipMemcpyAsync(cich_, cich.data(), nnz * 4, hipMemcpyHostToDevice, st));
+            MPX_HIP(hipMemcpyAsync(permch_, permch.data(), nnz * 4, hipMemcpyHostToDevice, st));
+        }
+    }
     MPX_HIP(hipMemcpyAsync(obj_o_, lp_.obj, n * 8, hipMemcpyHostToDevice, st));
     MPX_HIP(hipMemcpyAsync(l_o_, lp_.col_lower, n * 8, hipMemcpyHostToDevice, st));
     MPX_HIP(hipMemcpyAsync(u_o_, lp_.col_upper, n * 8, hipMemcpyHostToDevice, st));
@@ -545,6 +622,7 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
                                           v64_.u);
     if (m_) kScaleRows<<<gm, kBlock, 0, st>>>(m_, row_scale_, rl_o_, ru_o_, v64_.rl, v64_.ru);
     if (nnz_) kGather<<<gz, kBlock, 0, st>>>(nnz_, perm_, v64_.val, v64_.valt);
+    if (nnz_ && nch_) kGather<<<gz, kBlock, 0, st>>>(nnz_, permch_, v64_.val, valch64_);
     // Pock-Chambolle (pdlp.cpp:581-604).
     if (o_.do_pock_chambolle_scaling && m_ > 0 && n_ > 0) {
         kPockChambolle<<<gm, kBlock, 0, st>>>(m_, rs_, v64_.val, v64_.sig);
@@ -727,6 +805,34 @@ void SingleSolver::launchIteration(int k, bool storeHat) {
             kPrimalStepLong<T, false><<<g, kBlock, 0, st>>>(n_long_c_, long_c_, cs_, ri_, v.valt,
                                                             v.y, v.x, v.x0, v.xr, v.xh, v.c,
                                                             v.tau, v.l, v.u, sc_dev_, k);
+    }
+    if (nch_) {
+        // A x_r chunk by chunk (no long rows in this mode).
+        const T* valc;
+        T* part;
+        if constexpr (std::is_same_v<T, double>) {
+            valc = valch64_;
+            part = part64_;
+        } else {
+            valc = valch32_;
+            part = part32_;
+        }
+        const int gridCh = std::max(1, ceilDiv(static_cast<long long>(m_) * gch_, kBlock));
+        withGroup(gch_, [&](auto GI) {
+            constexpr int G = decltype(GI)::value;
+            for (int c = 0; c < nch_; ++c) {
+                const int* rsc = rsch_ + static_cast<size_t>(c) * (m_ + 1);
+                if (storeHat)
+                    kDualChunk<G, T, true><<<gridCh, kBlock, 0, st>>>(
+                        m_, rsc, cich_, valc, v.xr, part, c, nch_, v.y, v.y0, v.yh, v.sig, v.rl,
+                        v.ru, sc_dev_, k);
+                else
+                    kDualChunk<G, T, false><<<gridCh, kBlock, 0, st>>>(
+                        m_, rsc, cich_, valc, v.xr, part, c, nch_, v.y, v.y0, v.yh, v.sig, v.rl,
+                        v.ru, sc_dev_, k);
+            }
+        });
+        return;
     }
     withGroup(gr_, [&](auto GI) {
         constexpr int G = decltype(GI)::value;
@@ -1099,6 +1205,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
     double best_kkt = std::numeric_limits<double>::infinity();
     if (fp32_phase) {
         convertProblem<float>(stream_.s, v32_, v64_, m_, n_, nnz_);
+        convertChunks();
         convertIterates64to32();
     }
     // Mixed precision: the FP64 phase runs to refine_tol (default: the solve's
@@ -1355,6 +1462,7 @@ void SingleSolver::iterateFrom(const mpx_scaled_problem* sp, int iters, int inne
             throw DeviceError();
         }
         convertProblem<float>(st, v32_, v64_, m_, n_, nnz_);
+        convertChunks();
         auto cv = [&](float* d, const double* s, int count) {
             if (count > 0)
                 kConvert<float, double><<<ceilDiv(count, kBlock), kBlock, 0, st>>>(count, s, d);
@@ -1459,6 +1567,7 @@ void mpx_pdlp_default_options(mpx_pdlp_options* o) {
     o->spmv_group = 0;
     o->device = 0;
     o->mixed_handover = 3.0;
+    o->spmv_chunk_cols = 0;
 }
 
 const char* mpx_last_error(void) { return mpx::lastError(); }

@@ -69,6 +69,7 @@ struct PdlpOptions {
     Int spmv_group = 0;
     Int device = 0;
     Real mixed_handover = 3.0;  // FP32 -> FP64 at this KKT ratio (see mipx_pdlp.h)
+    Int spmv_chunk_cols = 0;    // column chunks of the dual step's SpMV (see mipx_pdlp.h)
 };
 
 class PdlpSolver : public LpSolver {

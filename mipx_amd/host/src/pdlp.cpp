@@ -73,6 +73,7 @@ mpx_pdlp_options toC(const PdlpOptions& p) {
     o.spmv_group = p.spmv_group;
     o.device = p.device;
     o.mixed_handover = p.mixed_handover;
+    o.spmv_chunk_cols = p.spmv_chunk_cols;
     return o;
 }
 

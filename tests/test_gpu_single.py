@@ -321,3 +321,23 @@ def test_concurrent_solves_and_race_stop():
     assert rbg["status"] == 3 and 0 <= rbg["iterations"] < seq["big"][0]["iterations"]
     assert rbg["iterations"] % 200 == 0
     assert rbg["work_units"] == rbg["iterations"] * (4.0 * big.nnz + 2.0 * (2000 + 5000))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_chunked_dual_spmv(precision):
+    """The dual step's A x_r in column chunks (automatic for n > 2^20, forced
+    here with spmv_chunk_cols): same status and objective as the one-launch
+    SpMV and as the oracle, KKT certified."""
+    lp = lpgen.c0_plain(seed=0xC0, m=2000, n=5000).lp
+    base = dict(TIGHT, precision=precision)
+    r1, x1, y1 = native.solve(lp, spmv_chunk_cols=1300, **base)   # 4 chunks
+    r0, _, _ = native.solve(lp, spmv_chunk_cols=-1, **base)
+    ro, _, _ = ob.solve(lp, **TIGHT)
+    print(r1["iterations"], r0["iterations"], ro["iterations"])
+    assert r1["status"] == r0["status"] == ro["status"] == 0
+    tol = 1e-6 if precision == 0 else 5e-6
+    assert abs(r1["objective"] - ro["objective"]) <= tol * (1 + abs(ro["objective"]))
+    assert abs(r1["objective"] - r0["objective"]) <= tol * (1 + abs(r0["objective"]))
+    assert abs(r1["iterations"] - r0["iterations"]) <= 0.25 * r0["iterations"]
+    k = ob.kkt_check(lp, x1, y1)
+    assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
