@@ -333,8 +333,9 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(instances shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path here "
+              "(batched and served instances would shard across GPUs with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
 
     from mipx_amd import native
