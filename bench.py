@@ -108,6 +108,40 @@ def cpu_baseline(lp, lo, hi, budget_s):
                 sample=f"{done} C2 instances solved back to back on one host core in {el:.1f} s")
 
 
+def _cpu_worker(args):
+    """One host process of cpu_parallel: C2 instances wid, wid + nw, ...
+    solved back to back until budget_s seconds are used."""
+    wid, nw, count, budget_s = args
+    from oracle import binding as ob
+    from mipx_amd import lpgen
+    g = lpgen.c2_shared()
+    solve = ob.ref_solve if ob.ref_available() else ob.solve
+    t0 = time.perf_counter()
+    done = 0
+    i = wid
+    while time.perf_counter() - t0 < budget_s and i < count:
+        lo, hi = g.branch_bounds(lpgen.c2_instance_seed(i))
+        solve(g.lp.with_col_bounds(lo, hi), **TOL)
+        done += 1
+        i += nw
+    return done, time.perf_counter() - t0
+
+
+def cpu_parallel(count, budget_s, workers):
+    """The same CPU PDLP on `workers` host processes at once (one LP each,
+    as the reference would run independent node LPs): aggregate solves/s of
+    the host's share of cores, beside the single-core cpu_baseline."""
+    import multiprocessing as mp
+    from oracle import binding as ob
+    kind = "reference" if ob.ref_available() else "port"
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_cpu_worker, [(w, workers, count, budget_s) for w in range(workers)])
+    done = sum(d for d, _ in res)
+    el = max(e for _, e in res)
+    return dict(value=done / el, unit="LP solves/s", cores=workers, kind=kind,
+                sample=f"{done} C2 instances on {workers} host processes in {el:.1f} s")
+
+
 def run_c4(a, prec):
     """Serving benchmark: Poisson arrivals into mpx_server_*; a producer
     thread generates requests ahead of their arrival times."""
@@ -327,6 +361,8 @@ def main():
                     help="lanes per column / row of the packed kernels (0: per-LP choice)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host processes of the parallel CPU figure (C2)")
     ap.add_argument("--profile-only", action="store_true",
                     help="upload the workload and time --launches launches of the iteration "
                          "kernels one by one (for rocprofv3), then a 1 GiB device copy")
@@ -427,11 +463,14 @@ def main():
     plain = a.plain_sample / (time.perf_counter() - t)
     log(f"plain fp64 single-instance: {plain:.2f} solves/s")
 
-    cpu = None
+    cpu = cpu_par = None
     if not a.no_cpu:
         cpu = (cpu_baseline_lps(lps, a.cpu_seconds) if lps is not None
                else cpu_baseline(lp, lo, hi, a.cpu_seconds))
         log(f"cpu baseline: {cpu}")
+        if lps is None and a.cpu_workers > 1:
+            cpu_par = cpu_parallel(a.count, a.cpu_seconds, a.cpu_workers)
+            log(f"cpu parallel: {cpu_par}")
 
     out = {
         "metric": METRIC,
@@ -464,6 +503,7 @@ def main():
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "instances_in_launch": prof["profiled_slots"]},
         "cpu_baseline": cpu,
+        "cpu_parallel": cpu_par,
         "plain_fp64_gpu_solves_per_s": round(plain, 3),
         "speedup_vs_plain": round(value / plain, 2),
         "latency_ms": {"p50": round(st["latency_p50_ms"], 2), "p99": round(st["latency_p99_ms"], 2),
