@@ -263,6 +263,18 @@ __device__ inline void laneDotN(const int* __restrict__ idx, const T* __restrict
     }
 }
 
+// A lane's per-iteration scalars in the iteration precision: the round's
+// last iteration, the Halpern weight k/(k+1) of iteration `kiter` (formed in
+// FP64, then rounded), step size and primal weight.
+template <typename T>
+__device__ inline void laneScalars(const ILane& ln, int kiter, int& lim, T& lam, T& step, T& pw) {
+    lim = ln.lim;
+    const int kk = ln.inner + kiter;
+    lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
+    step = static_cast<T>(ln.step);
+    pw = static_cast<T>(ln.pw);
+}
+
 // CW = columns (rows) per wave in the iteration kernels: their epilogue
 // operands and gathers are issued together so that several memory round trips
 // are in flight per wave; each column still sums its entries in storage
@@ -282,10 +294,11 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
     const T* __restrict__ IU, int kiter) {
     const int nslots = *count;
     const int per_group = (n + 4 * kCW - 1) / (4 * kCW);
-    // A lane's scalars are the same for every column of its group: loaded
-    // when the group changes (work items come group by group).
-    int curg = -1;
-    ILane ln{1.0, 1.0, 0, 0};
+    // A lane's scalars are the same for every column of its group: loaded,
+    // and the Halpern weight k/(k+1) formed, when the group changes (work
+    // items come group by group).
+    int curg = -1, lim = 0;
+    T lam = T(0), step = T(1), pw = T(1);
     forGroupItems(nslots, per_group, [&](int g, int jb) {
         // Wave-uniform column: in a scalar register, so the column's matrix
         // entries, c and tau come through the scalar cache.
@@ -296,9 +309,9 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         const int s = g * kLanes + lane;
         if (g != curg) {
             curg = g;
-            ln = s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+            laneScalars(s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0}, kiter, lim, lam, step, pw);
         }
-        if (kiter > ln.lim) return;  // lane idle this iteration
+        if (kiter > lim) return;  // lane idle this iteration
         const T* Y = IY + static_cast<size_t>(g) * m * kLanes + lane;
         const size_t base = (static_cast<size_t>(g) * n + j0) * kLanes + lane;
         T xv[kCW], x0v[kCW], lv[kCW], uv[kCW], acc[kCW];
@@ -317,10 +330,7 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         }
         if constexpr (kCW == 1) acc[0] = laneDot(ri, valt, qb[0], qe[0], Y);
         else laneDotN<T, kCW>(ri, valt, qb, qe, Y, acc);
-        const int kk = ln.inner + kiter;
-        const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
-        const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
-        const bool last = kiter == ln.lim;
+        const bool last = kiter == lim;
         T* xh = last ? vecsOf<T>(insts[slots[s]]).xh : nullptr;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
@@ -347,8 +357,8 @@ __global__ __launch_bounds__(kBlock) void kILDual(
     T* __restrict__ IY, const T* __restrict__ IY0, int kiter) {
     const int nslots = *count;
     const int per_group = (m + 4 * kCW - 1) / (4 * kCW);
-    int curg = -1;
-    ILane ln{1.0, 1.0, 0, 0};
+    int curg = -1, lim = 0;
+    T lam = T(0), step = T(1), pw = T(1);
     forGroupItems(nslots, per_group, [&](int g, int ib) {
         const int i0 = __builtin_amdgcn_readfirstlane(
             (ib * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW);
@@ -357,9 +367,9 @@ __global__ __launch_bounds__(kBlock) void kILDual(
         const int s = g * kLanes + lane;
         if (g != curg) {
             curg = g;
-            ln = s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0};
+            laneScalars(s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0}, kiter, lim, lam, step, pw);
         }
-        if (kiter > ln.lim) return;
+        if (kiter > lim) return;
         const T* X = IXR + static_cast<size_t>(g) * n * kLanes + lane;
         const size_t base = (static_cast<size_t>(g) * m + i0) * kLanes + lane;
         T yv[kCW], y0v[kCW], acc[kCW];
@@ -376,10 +386,7 @@ __global__ __launch_bounds__(kBlock) void kILDual(
         }
         if constexpr (kCW == 1) acc[0] = laneDot(ci, val, qb[0], qe[0], X);
         else laneDotN<T, kCW>(ci, val, qb, qe, X, acc);
-        const int kk = ln.inner + kiter;
-        const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
-        const T step = static_cast<T>(ln.step), pw = static_cast<T>(ln.pw);
-        const bool last = kiter == ln.lim;
+        const bool last = kiter == lim;
         T* yh = last ? vecsOf<T>(insts[slots[s]]).yh : nullptr;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
