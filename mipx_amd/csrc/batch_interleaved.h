@@ -299,6 +299,7 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
     // items come group by group).
     int curg = -1, lim = 0;
     T lam = T(0), step = T(1), pw = T(1);
+    T* xh = nullptr;  // x^ of a lane whose round ends at this iteration
     forGroupItems(nslots, per_group, [&](int g, int jb) {
         // Wave-uniform column: in a scalar register, so the column's matrix
         // entries, c and tau come through the scalar cache.
@@ -310,6 +311,9 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         if (g != curg) {
             curg = g;
             laneScalars(s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0}, kiter, lim, lam, step, pw);
+            // Loaded here, not beside the column's update: a conditional load
+            // there would make every column wait for all of its loads.
+            xh = s < nslots && kiter == lim ? vecsOf<T>(insts[slots[s]]).xh : nullptr;
         }
         if (kiter > lim) return;  // lane idle this iteration
         const T* Y = IY + static_cast<size_t>(g) * m * kLanes + lane;
@@ -331,7 +335,6 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         if constexpr (kCW == 1) acc[0] = laneDot(ri, valt, qb[0], qe[0], Y);
         else laneDotN<T, kCW>(ri, valt, qb, qe, Y, acc);
         const bool last = kiter == lim;
-        T* xh = last ? vecsOf<T>(insts[slots[s]]).xh : nullptr;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             const int j = j0 + cc;
@@ -359,6 +362,7 @@ __global__ __launch_bounds__(kBlock) void kILDual(
     const int per_group = (m + 4 * kCW - 1) / (4 * kCW);
     int curg = -1, lim = 0;
     T lam = T(0), step = T(1), pw = T(1);
+    T* yh = nullptr;  // y^ of a lane whose round ends at this iteration
     forGroupItems(nslots, per_group, [&](int g, int ib) {
         const int i0 = __builtin_amdgcn_readfirstlane(
             (ib * 4 + static_cast<int>(threadIdx.x) / kLanes) * kCW);
@@ -368,6 +372,7 @@ __global__ __launch_bounds__(kBlock) void kILDual(
         if (g != curg) {
             curg = g;
             laneScalars(s < nslots ? lanes[s] : ILane{1.0, 1.0, 0, 0}, kiter, lim, lam, step, pw);
+            yh = s < nslots && kiter == lim ? vecsOf<T>(insts[slots[s]]).yh : nullptr;
         }
         if (kiter > lim) return;
         const T* X = IXR + static_cast<size_t>(g) * n * kLanes + lane;
@@ -387,7 +392,6 @@ __global__ __launch_bounds__(kBlock) void kILDual(
         if constexpr (kCW == 1) acc[0] = laneDot(ci, val, qb[0], qe[0], X);
         else laneDotN<T, kCW>(ci, val, qb, qe, X, acc);
         const bool last = kiter == lim;
-        T* yh = last ? vecsOf<T>(insts[slots[s]]).yh : nullptr;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
             const int i = i0 + cc;
