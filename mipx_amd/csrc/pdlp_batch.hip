@@ -1307,14 +1307,23 @@ void ilGather(hipStream_t st, int grid, const BInst* di, int K, const BOwner& w,
                                                  nullptr, nullptr);
 }
 
+// Columns (rows) per wave of the FP64 kernels of a mixed-precision batch.
+// The FP32 pair is tuned in the first rounds, when every instance is still
+// in FP32; one column (row) per wave is the fastest FP64 choice on C2 at
+// full occupancy (primal 372 / 400 / 408 us for 1 / 2 / 4 columns, dual
+// 220 / 250 / 293 us) and gives the most work items once few instances
+// are left.
+constexpr int kIlFp64PerWave = 1;
+
 // Iteration k (primal then dual; FP32 instances first) with `cwp` columns
-// per wave in the primal kernels and `cwd` rows per wave in the dual ones.
+// per wave in the primal kernels and `cwd` rows per wave in the dual ones
+// (`cwp64` / `cwd64` for the FP64 kernels).
 void ilIterate(hipStream_t st, int grid_base, const BInst* di, const BOwner& w, IlSet& s,
-               bool f32, int cwp, int cwd, int k) {
+               bool f32, int cwp, int cwd, int cwp64, int cwd64, int k) {
     const int n = w.n, m = w.m;
     const int grid = ilGrid(grid_base);
     auto primal = [&](bool fp32) {
-        withCW(cwp, [&](auto CWI) {
+        withCW(fp32 ? cwp : cwp64, [&](auto CWI) {
             constexpr int CW = decltype(CWI)::value;
             if (fp32)
                 kILPrimal<float, CW><<<grid, kBlock, 0, st>>>(
@@ -1327,7 +1336,7 @@ void ilIterate(hipStream_t st, int grid_base, const BInst* di, const BOwner& w, 
         });
     };
     auto dual = [&](bool fp32) {
-        withCW(cwd, [&](auto CWI) {
+        withCW(fp32 ? cwd : cwd64, [&](auto CWI) {
             constexpr int CW = decltype(CWI)::value;
             if (fp32)
                 kILDual<float, CW><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, s.lanes32, n,
@@ -1750,7 +1759,10 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     const bool f32 = mixed_ && !fp32_gone_;
     ilGather(st_, grid_, di, K, w, il_, f32);
     if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
-    for (int k = 1; k <= bo_.N; ++k) ilIterate(st_, grid_, di, w, il_, f32, cwp_, cwd_, k);
+    // FP64 kernels: the tuned pair when the batch is FP64 only.
+    const int cwp64 = mixed_ ? kIlFp64PerWave : cwp_, cwd64 = mixed_ ? kIlFp64PerWave : cwd_;
+    for (int k = 1; k <= bo_.N; ++k)
+        ilIterate(st_, grid_, di, w, il_, f32, cwp_, cwd_, cwp64, cwd64, k);
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
     ilScatter(st_, grid_, di, w, il_, f32);
     if (f32) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
@@ -2703,9 +2715,11 @@ private:
             }
             kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tc64_, totals_ + 2, k);
             kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
-            // Four columns per wave in the primal, one row per wave in the
-            // dual: the pair the batched engine's autotuner settles on.
-            for (auto& b : bases_) ilIterate(st_, grid_, di, b->owner, b->il, f32, 4, 1, k);
+            // FP32: four columns per wave in the primal, one row per wave in
+            // the dual, the pair the batched engine's autotuner settles on.
+            for (auto& b : bases_)
+                ilIterate(st_, grid_, di, b->owner, b->il, f32, 4, 1, kIlFp64PerWave,
+                          kIlFp64PerWave, k);
         }
         for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, f32);
         if (mixed_) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
