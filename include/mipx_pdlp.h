@@ -342,7 +342,8 @@ int mpx_server_create(const mpx_pdlp_options* opts, double hbm_budget_bytes,
                       int32_t max_slots, mpx_server** out);
 /* Uploads a base LP whose scaled matrix, objective, row bounds and power
  * iteration are shared by the bounds variants submitted against it (the
- * branch-and-bound node relaxations of one MIP).  Kept until destroy. */
+ * branch-and-bound node relaxations of one MIP).  Kept until destroy.  A
+ * base has at most 4,194,303 rows and columns (MPX_ERR_UNSUPPORTED beyond). */
 int mpx_server_add_base(mpx_server* srv, const mpx_lp* base, int32_t* base_id);
 /* Copies the LP; returns a ticket >= 0 in *ticket. */
 int mpx_server_submit(mpx_server* srv, const mpx_lp* lp, int64_t* ticket);

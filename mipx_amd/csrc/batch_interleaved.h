@@ -205,37 +205,85 @@ __device__ inline void stNT(T* p, T v) {
     __builtin_nontemporal_store(v, p);
 }
 
-// Gathered dot product sum_q val[q] * V[idx[q] * 64] with four entries in
-// flight; the additions stay in storage order.
+// Buffer-resource view of one group's interleaved vector (len elements x 64
+// lanes of T).  Element i of lane l sits at byte i * 64 * sizeof(T) +
+// l * sizeof(T): the element offset goes into the load's scalar offset and the
+// lane offset is a constant vector register, so a gather costs one scalar
+// shift and no vector address arithmetic.  Offsets stay below 2^31 bytes
+// (ilFits); 0x00020000 is the gfx9 descriptor word for raw dword access.
+constexpr int kIlMaxLen = 0x7fffffff / (kLanes * 8);  // 4,194,303
+__host__ __device__ constexpr bool ilFits(int n, int m) { return n <= kIlMaxLen && m <= kIlMaxLen; }
+
 template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t groupRsrc(const T* base, int len) {
+    const int bytes = len * kLanes * static_cast<int>(sizeof(T));
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), 0, bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ inline T gatherLane(__amdgpu_buffer_rsrc_t r, int voff, int i) {
+    const int soff = i * kLanes * static_cast<int>(sizeof(T));
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// Gathered dot product sum_q val[q] * V[idx[q]] of the lane's column of the
+// group view `r` (lane byte offset `voff`), G entries in flight; the
+// additions stay in storage order.  The last 1..G-1 entries go in one round
+// trip: indices past the end repeat the last entry and are not added.  With
+// G = 8 a C2 row (16 entries on average) takes two round trips and a column
+// (6 entries) one: these kernels wait on memory three quarters of the time
+// at the full 8 waves per SIMD, so entries in flight per wave set the pace.
+template <typename T, int G = 8>
 __device__ inline T laneDot(const int* __restrict__ idx, const T* __restrict__ val, int qb, int qe,
-                            const T* __restrict__ V) {
+                            __amdgpu_buffer_rsrc_t r, int voff) {
     T acc = T(0);
     int q = qb;
-    for (; q + 4 <= qe; q += 4) {
-        const int i0 = idx[q], i1 = idx[q + 1], i2 = idx[q + 2], i3 = idx[q + 3];
-        const T a0 = val[q], a1 = val[q + 1], a2 = val[q + 2], a3 = val[q + 3];
-        const T v0 = V[static_cast<size_t>(i0) * kLanes], v1 = V[static_cast<size_t>(i1) * kLanes];
-        const T v2 = V[static_cast<size_t>(i2) * kLanes], v3 = V[static_cast<size_t>(i3) * kLanes];
-        acc += a0 * v0;
-        acc += a1 * v1;
-        acc += a2 * v2;
-        acc += a3 * v3;
+    for (; q + G <= qe; q += G) {
+        int ix[G];
+        T a[G], v[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            ix[u] = idx[q + u];
+            a[u] = val[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) v[u] = gatherLane<T>(r, voff, ix[u]);
+#pragma unroll
+        for (int u = 0; u < G; ++u) acc += a[u] * v[u];
     }
-    for (; q < qe; ++q) acc += val[q] * V[static_cast<size_t>(idx[q]) * kLanes];
+    if (q < qe) {
+        int ix[G - 1];
+        T a[G - 1], v[G - 1];
+#pragma unroll
+        for (int u = 0; u < G - 1; ++u) {
+            const int qc = min(q + u, qe - 1);
+            ix[u] = idx[qc];
+            a[u] = val[qc];
+        }
+#pragma unroll
+        for (int u = 0; u < G - 1; ++u) v[u] = gatherLane<T>(r, voff, ix[u]);
+#pragma unroll
+        for (int u = 0; u < G - 1; ++u)
+            if (q + u < qe) acc += a[u] * v[u];
+    }
     return acc;
 }
 
-// Gathered dot products acc[c] = sum_q val[q] * V[idx[q] * 64] over the
-// entries [qb[c], qe[c]) of CW columns (rows) at once: entries go in chunks
-// of four per column, so up to 4 * CW gathers are in flight and a short tail
-// costs one round trip, not one per entry.  Each column's additions stay in
-// storage order (an entry past the end is skipped, not added as zero).
-// (Measured slower than laneDot for one column per wave; used for CW > 1.)
+// Gathered dot products acc[c] = sum_q val[q] * V[idx[q]] over the entries
+// [qb[c], qe[c]) of CW columns (rows) at once: entries go in chunks of four
+// per column, so up to 4 * CW gathers are in flight and a short tail costs
+// one round trip, not one per entry.  Loads are unconditional (an index past
+// a column's end repeats its last entry; an empty column reads entry 0) and
+// each column's additions stay in storage order (an entry past the end is
+// skipped, not added as zero).  Callers guarantee nnz > 0 when any column has
+// entries.
 template <typename T, int CW>
 __device__ inline void laneDotN(const int* __restrict__ idx, const T* __restrict__ val,
                                 const int (&qb)[CW], const int (&qe)[CW],
-                                const T* __restrict__ V, T (&acc)[CW]) {
+                                __amdgpu_buffer_rsrc_t r, int voff, T (&acc)[CW]) {
     int most = 0;
 #pragma unroll
     for (int c = 0; c < CW; ++c) most = max(most, qe[c] - qb[c]);
@@ -247,14 +295,14 @@ __device__ inline void laneDotN(const int* __restrict__ idx, const T* __restrict
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int q = qb[c] + t + u;
-                const bool ok = q < qe[c];
-                ix[c][u] = ok ? idx[q] : 0;
-                a[c][u] = ok ? val[q] : T(0);
+                const int qc = q < qe[c] ? q : qe[c] > qb[c] ? qe[c] - 1 : 0;
+                ix[c][u] = idx[qc];
+                a[c][u] = val[qc];
             }
 #pragma unroll
         for (int c = 0; c < CW; ++c)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[c][u] = V[static_cast<size_t>(ix[c][u]) * kLanes];
+            for (int u = 0; u < 4; ++u) v[c][u] = gatherLane<T>(r, voff, ix[c][u]);
 #pragma unroll
         for (int c = 0; c < CW; ++c)
 #pragma unroll
@@ -316,7 +364,8 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
             xh = s < nslots && kiter == lim ? vecsOf<T>(insts[slots[s]]).xh : nullptr;
         }
         if (kiter > lim) return;  // lane idle this iteration
-        const T* Y = IY + static_cast<size_t>(g) * m * kLanes + lane;
+        const __amdgpu_buffer_rsrc_t ry = groupRsrc(IY + static_cast<size_t>(g) * m * kLanes, m);
+        const int voff = lane * static_cast<int>(sizeof(T));
         const size_t base = (static_cast<size_t>(g) * n + j0) * kLanes + lane;
         T xv[kCW], x0v[kCW], lv[kCW], uv[kCW], acc[kCW];
         int qb[kCW], qe[kCW];
@@ -332,8 +381,8 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
             qe[cc] = in ? cs[j0 + cc + 1] : 0;
             acc[cc] = T(0);
         }
-        if constexpr (kCW == 1) acc[0] = laneDot(ri, valt, qb[0], qe[0], Y);
-        else laneDotN<T, kCW>(ri, valt, qb, qe, Y, acc);
+        if constexpr (kCW == 1) acc[0] = laneDot(ri, valt, qb[0], qe[0], ry, voff);
+        else laneDotN<T, kCW>(ri, valt, qb, qe, ry, voff, acc);
         const bool last = kiter == lim;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
@@ -375,7 +424,8 @@ __global__ __launch_bounds__(kBlock) void kILDual(
             yh = s < nslots && kiter == lim ? vecsOf<T>(insts[slots[s]]).yh : nullptr;
         }
         if (kiter > lim) return;
-        const T* X = IXR + static_cast<size_t>(g) * n * kLanes + lane;
+        const __amdgpu_buffer_rsrc_t rx = groupRsrc(IXR + static_cast<size_t>(g) * n * kLanes, n);
+        const int voff = lane * static_cast<int>(sizeof(T));
         const size_t base = (static_cast<size_t>(g) * m + i0) * kLanes + lane;
         T yv[kCW], y0v[kCW], acc[kCW];
         int qb[kCW], qe[kCW];
@@ -389,8 +439,8 @@ __global__ __launch_bounds__(kBlock) void kILDual(
             qe[cc] = in ? rs[i0 + cc + 1] : 0;
             acc[cc] = T(0);
         }
-        if constexpr (kCW == 1) acc[0] = laneDot(ci, val, qb[0], qe[0], X);
-        else laneDotN<T, kCW>(ci, val, qb, qe, X, acc);
+        if constexpr (kCW == 1) acc[0] = laneDot(ci, val, qb[0], qe[0], rx, voff);
+        else laneDotN<T, kCW>(ci, val, qb, qe, rx, voff, acc);
         const bool last = kiter == lim;
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {

@@ -1597,7 +1597,7 @@ void BatchEngine::setup() {
                                                  pad(nr, sizeof(PRec<float>)));
     }
     // Lane-per-instance mode when every instance shares one matrix.
-    interleaved_ = allow_interleaved_ && no == 1 && ni >= 1;
+    interleaved_ = allow_interleaved_ && no == 1 && ni >= 1 && ilFits(owners_[0].n, owners_[0].m);
     if (interleaved_) {
         const size_t bytes_il = IlSet::bytes(ni, owners_[0].n, owners_[0].m, mixed_);
         d_il_buf_ = std::make_unique<DevBuf>(bytes_il);
@@ -3215,6 +3215,10 @@ int mpx_server_add_base(mpx_server* srv, const mpx_lp* base, int32_t* base_id) {
         return MPX_ERR_INVALID;
     }
     if (srv->broken) return MPX_ERR_DEVICE;
+    if (!ilFits(base->num_cols, base->num_rows)) {
+        setError("mpx_server_add_base: a base LP has at most 4194303 rows and columns");
+        return MPX_ERR_UNSUPPORTED;
+    }
     try {
         *base_id = srv->s->addBase(*base);
     } catch (const Server::OOM&) {
