@@ -81,14 +81,17 @@ def primal_kernel_bytes(lp, slots, sv):
     return slots * (6 * n + m) * sv + 2 * n * sv + nnz * (sv + 4) + (n + 1) * 4
 
 
-def read_traffic(workload, precision):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), committed under profiles/ when available."""
+def read_traffic(workload, precision, kernel=None):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE), committed under profiles/ when available."""
     path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        d = json.load(f)
+    if "kernels" in d:
+        return d["kernels"].get(kernel)
+    return d.get("bytes_per_launch")
 
 
 def cpu_baseline(lp, lo, hi, budget_s):
@@ -446,6 +449,8 @@ def main():
     bytes_launch = (packed_primal_bytes(lps, sv) if lps is not None
                     else primal_kernel_bytes(lp, prof["profiled_slots"], sv))
     achieved = bytes_launch / (prof["primal_kernel_us"] * 1e-6) / 1e9
+    primal_kernel = (None if lps is not None else
+                     f"kILPrimal<{'float' if prec == 1 else 'double'},{prof['columns_per_wave']}>")
     log(f"primal kernel {prof['primal_kernel_us']:.1f} us for {prof['profiled_slots']} instances "
         f"-> {achieved:.0f} GB/s")
     batch.close()
@@ -494,11 +499,9 @@ def main():
                    "kernels": "lane-per-instance" if st["interleaved"] else "packed"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
-                     "traffic": read_traffic(a.workload, a.precision),
+                     "traffic": read_traffic(a.workload, a.precision, primal_kernel),
                      "kernel": ("kPackedPrimal (primal half step, packed tiles)" if lps is not None
-                                else f"kILPrimal<{'float' if prec == 1 else 'double'},"
-                                     f"{prof['columns_per_wave']}> (primal half step, "
-                                     "lane-per-instance)"),
+                                else f"{primal_kernel} (primal half step, lane-per-instance)"),
                      "kernel_us": round(prof["primal_kernel_us"], 2),
                      "algorithmic_bytes_per_launch": int(bytes_launch),
                      "instances_in_launch": prof["profiled_slots"]},
