@@ -190,3 +190,23 @@ def test_server_warm_variants_identical_to_solo_batch(precision):
         _same(got[t], (res[0], xs[0], ys[0]), f"warm{i}")
     for t, i in cold_t.items():
         _same(got[t], _alone_shared(g, bounds[i], opts), f"cold{i}")
+
+
+def test_server_refuses_base_beyond_lane_offsets():
+    """A base's lane-per-instance slices are addressed with 31-bit byte
+    offsets: a base with more than 4,194,303 columns is refused with
+    MPX_ERR_UNSUPPORTED (code 4) before anything is allocated, and the
+    server keeps working."""
+    from mipx_amd.lp import LpProblem
+    n = 4_194_304
+    big = LpProblem(1, n, np.array([0, 1]), np.array([0]), np.array([1.0]), np.zeros(n),
+                    np.zeros(n), np.ones(n), np.array([0.0]), np.array([1.0]))
+    srv = native.Server(max_slots=4, **TIGHT)
+    with pytest.raises(RuntimeError, match=r"code 4\)"):
+        srv.add_base(big)
+    g = lpgen.c2_shared(seed=0xC2, m=60, n=150)
+    base = srv.add_base(g.lp)
+    t = srv.submit_bounds(base, g.lp.col_lower, g.lp.col_upper)
+    r = _drain(srv, [t])[t][0]
+    srv.close()
+    assert r["status"] == 0, r
