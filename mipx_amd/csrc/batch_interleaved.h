@@ -6,8 +6,10 @@
 // interleaved: V[group][row][lane].  A wave then handles one column (or row)
 // for 64 instances at once: the matrix entries of that column are uniform
 // (scalar loads, L2-resident), and every gather of y[row] / x_r[col] is one
-// contiguous 64-element load.  Per-instance data (primal weight, inner count,
-// iteration limit of the round) sits in per-lane registers.
+// contiguous 64-element buffer load whose scalar offset carries the element
+// index.  Per-instance data (step, primal weight, Halpern weight, iteration
+// limit of the round) sits in per-lane registers, set when a wave's work
+// moves to another group.
 //
 // Round structure (driven by BatchEngine):
 //   kSlotLists      compacts the active instances of each precision into slots
