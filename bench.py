@@ -355,7 +355,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--drain-seconds", type=float, default=120.0)
     ap.add_argument("--max-iter", type=int, default=400000)
-    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--count", type=int, default=None,
+                    help="instances (default: 4096 for c2, 1024 for c3)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--handover", type=float, default=0.0,
@@ -371,6 +372,8 @@ def main():
                          "kernels one by one (for rocprofv3), then a 1 GiB device copy")
     ap.add_argument("--launches", type=int, default=20)
     a = ap.parse_args()
+    if a.count is None:
+        a.count = 1024 if a.workload == "c3" else 4096
     if a.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path here "
               "(batched and served instances would shard across GPUs with no exchange)",
