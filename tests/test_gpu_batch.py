@@ -150,3 +150,24 @@ def test_batch_shared_warm_start_vs_oracle(precision):
     print("re-warm iters", [again[i]["iterations"] for i in both])
     assert all(again[i]["status"] == 0 for i in both)
     assert sum(again[i]["iterations"] for i in both) < sum(cold[i]["iterations"] for i in both)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_preconditioner_refresh(precision):
+    """The batched path applies the same refresh rule: forced refreshes
+    (ratio 1.01) leave every instance's run bit-identical."""
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    count = 8
+    lo = np.empty((count, 800))
+    hi = np.empty((count, 800))
+    for i in range(count):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    base = dict(TIGHT, precision=precision, max_iter=200000, primal_weight=1.0)
+    r0, X0, Y0 = native.solve_batch_shared(g.lp, lo, hi, **base)
+    r1, X1, Y1 = native.solve_batch_shared(g.lp, lo, hi, preconditioner_refresh=1,
+                                           preconditioner_refresh_ratio=1.01, **base)
+    for a, b in zip(r0, r1):
+        assert a["status"] == b["status"] == 0
+        assert (a["iterations"], a["objective"]) == (b["iterations"], b["objective"])
+    assert np.array_equal(X0, X1) and np.array_equal(Y0, Y1)
+    assert any(abs(r["primal_weight"] - 1.0) > 0.01 for r in r1)

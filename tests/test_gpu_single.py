@@ -341,3 +341,18 @@ def test_chunked_dual_spmv(precision):
     assert abs(r1["iterations"] - r0["iterations"]) <= 0.25 * r0["iterations"]
     k = ob.kkt_check(lp, x1, y1)
     assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
+
+
+def test_preconditioner_refresh_single_path():
+    """pdlp.cpp:1325-1340 / 713-742: a refresh recomputes sigma/tau from the
+    unchanged scaled matrix, which leaves them as they were (pinned against
+    the reference build in test_oracle_preconditioner_refresh), so runs with
+    refreshes forced (ratio 1.01) are bit-identical to runs without them."""
+    lp = lpgen.c0_plain(seed=3, m=120, n=300).lp
+    base = dict(TIGHT, primal_weight=1.0)
+    r0, x0, y0 = native.solve(lp, **base)
+    r1, x1, y1 = native.solve(lp, preconditioner_refresh=1, preconditioner_refresh_ratio=1.01,
+                              **base)
+    assert r1["status"] == 0 and r1["restarts"] > 0 and abs(r1["primal_weight"] - 1.0) > 0.01
+    assert (r1["iterations"], r1["objective"]) == (r0["iterations"], r0["objective"])
+    assert np.array_equal(x1, x0) and np.array_equal(y1, y0)

@@ -310,3 +310,29 @@ def test_oracle_closed_form_with_optional_features(opts):
             r, x, y = solve(lp, **o)
             assert r["status"] == 0, c["name"]
             assert abs(r["objective"] - c["objective"]) <= max(c["tol"], 1e-4), c["name"]
+
+
+def test_oracle_preconditioner_refresh():
+    """pdlp.cpp:1325-1340 / 713-742: on a restart whose primal weight drifted
+    past the ratio the reference recomputes Pock-Chambolle sigma/tau from the
+    scaled matrix's row and column 2-norms.  Those are the values setup
+    computed from the same matrix, so a refresh only moves the recorded
+    weight: runs with refreshes forced (ratio 1.01) equal runs without them,
+    and equal the reference build's."""
+    for seed, (m, n) in [(3, (120, 300)), (0xC0, (600, 1500))]:
+        lp = lpgen.c0_plain(seed=seed, m=m, n=n).lp
+        base = dict(TIGHT, primal_weight=1.0)
+        on = dict(base, preconditioner_refresh=1, preconditioner_refresh_ratio=1.01)
+        r1, x1, y1 = ob.solve(lp, **on)
+        r0, x0, y0 = ob.solve(lp, **base)
+        assert r1["status"] == 0
+        # The weight moves only at restarts and ends more than 1% from its
+        # start, so at least one restart refreshed.
+        assert r1["restarts"] > 0 and abs(r1["primal_weight"] - 1.0) > 0.01
+        assert (r1["iterations"], r1["objective"]) == (r0["iterations"], r0["objective"])
+        assert np.array_equal(x1, x0) and np.array_equal(y1, y0)
+        if ob.ref_available():
+            r2, x2, y2 = ob.ref_solve(lp, **on)
+            assert (r1["status"], r1["iterations"]) == (r2["status"], r2["iterations"])
+            assert r1["objective"] == r2["objective"]
+            assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
