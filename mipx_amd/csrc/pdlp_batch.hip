@@ -1239,8 +1239,14 @@ struct IlSet {
     IlBufs<double> d;
     int *slots32 = nullptr, *slots64 = nullptr, *counts = nullptr;
     ILane *lanes32 = nullptr, *lanes64 = nullptr;
+    // FP64 bound flags per (group, tile of 64 columns): bit j % 64 set when
+    // some lane's l or u at column j differs from the owner's; the other
+    // columns read the owner's bound (one cached value) instead of 64 lanes'
+    // copies.
+    unsigned long long* bf64 = nullptr;
     int owner = -1;  // instances taking part (-1: every instance of the table)
 
+    static size_t boundTiles(int n) { return static_cast<size_t>((n + kLanes - 1) / kLanes); }
     static size_t bytes(int cap, int n, int m, bool mixed) {
         const size_t ng = static_cast<size_t>((cap + kLanes - 1) / kLanes);
         const size_t en = ng * kLanes * static_cast<size_t>(n);
@@ -1248,7 +1254,8 @@ struct IlSet {
         size_t b = 5 * pad(en, 8) + 2 * pad(em, 8);
         if (mixed) b += 5 * pad(en, 4) + 2 * pad(em, 4);
         const size_t c = static_cast<size_t>(cap);
-        return b + 2 * pad(c, 4) + 2 * pad(c, sizeof(ILane)) + pad(4, 4);
+        return b + 2 * pad(c, 4) + 2 * pad(c, sizeof(ILane)) + pad(4, 4) +
+               pad(ng * boundTiles(n), 8);
     }
     void carve(Carve& c, int cap, int n, int m, bool mixed) {
         const size_t ng = static_cast<size_t>((cap + kLanes - 1) / kLanes);
@@ -1270,6 +1277,7 @@ struct IlSet {
         lanes32 = c.take<ILane>(static_cast<size_t>(cap));
         lanes64 = c.take<ILane>(static_cast<size_t>(cap));
         counts = c.take<int>(4);
+        bf64 = c.take<unsigned long long>(ng * boundTiles(n));
     }
 };
 
@@ -1297,14 +1305,14 @@ void ilGather(hipStream_t st, int grid, const BInst* di, int K, const BOwner& w,
                                    s.counts);
     if (f32) {
         kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, n, 0, s.f.x, s.f.x0,
-                                                    s.f.l, s.f.u);
+                                                    s.f.l, s.f.u, nullptr, nullptr, nullptr);
         kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, m, 1, s.f.y, s.f.y0,
-                                                    nullptr, nullptr);
+                                                    nullptr, nullptr, nullptr, nullptr, nullptr);
     }
     kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x, s.d.x0,
-                                                 s.d.l, s.d.u);
+                                                 s.d.l, s.d.u, w.l, w.u, s.bf64);
     kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y, s.d.y0,
-                                                 nullptr, nullptr);
+                                                 nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 // Columns (rows) per wave of the FP64 kernels of a mixed-precision batch.
@@ -1328,11 +1336,11 @@ void ilIterate(hipStream_t st, int grid_base, const BInst* di, const BOwner& w, 
             if (fp32)
                 kILPrimal<float, CW><<<grid, kBlock, 0, st>>>(
                     di, s.slots32, s.counts, s.lanes32, n, m, w.cs, w.ri, w.valtf, w.cf, w.tauf,
-                    s.f.y, s.f.x, s.f.x0, s.f.xr, s.f.l, s.f.u, k);
+                    s.f.y, s.f.x, s.f.x0, s.f.xr, s.f.l, s.f.u, nullptr, nullptr, nullptr, k);
             else
                 kILPrimal<double, CW><<<grid, kBlock, 0, st>>>(
                     di, s.slots64, s.counts + 1, s.lanes64, n, m, w.cs, w.ri, w.valt, w.c, w.tau,
-                    s.d.y, s.d.x, s.d.x0, s.d.xr, s.d.l, s.d.u, k);
+                    s.d.y, s.d.x, s.d.x0, s.d.xr, s.d.l, s.d.u, w.l, w.u, s.bf64, k);
         });
     };
     auto dual = [&](bool fp32) {
@@ -1789,11 +1797,13 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
                 if (f32)
                     kILPrimal<float, CW><<<ilGrid(grid_), kBlock, 0, st_>>>(
                         di, il_.slots32, il_.counts, il_.lanes32, w.n, w.m, w.cs, w.ri, w.valtf,
-                        w.cf, w.tauf, il_.f.y, il_.f.x, il_.f.x0, il_.f.xr, il_.f.l, il_.f.u, k);
+                        w.cf, w.tauf, il_.f.y, il_.f.x, il_.f.x0, il_.f.xr, il_.f.l, il_.f.u,
+                        nullptr, nullptr, nullptr, k);
                 else
                     kILPrimal<double, CW><<<ilGrid(grid_), kBlock, 0, st_>>>(
                         di, il_.slots64, il_.counts + 1, il_.lanes64, w.n, w.m, w.cs, w.ri, w.valt,
-                        w.c, w.tau, il_.d.y, il_.d.x, il_.d.x0, il_.d.xr, il_.d.l, il_.d.u, k);
+                        w.c, w.tau, il_.d.y, il_.d.x, il_.d.x0, il_.d.xr, il_.d.l, il_.d.u,
+                        w.l, w.u, il_.bf64, k);
             });
             MPX_HIP(hipEventRecord(e[1], st_));
             withCW(cwd_, [&](auto CWI) {
