@@ -1,0 +1,45 @@
+"""CPU checks of bench.py's reporting contract: one GPU only, the roofline's
+algorithmic byte count, and the measured HBM traffic it reports for the
+kernel the autotuner picked."""
+import os
+import subprocess
+import sys
+from types import SimpleNamespace
+
+import bench
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr
+
+
+def test_primal_kernel_bytes_c2():
+    """4,096 instances of m=3,000 n=8,000 nnz=48,000: per instance 6n + m
+    values (x, x0, l, u read; x, x_r written; y once), shared 2n values and
+    A' in CSR."""
+    lp = SimpleNamespace(num_rows=3000, num_cols=8000, nnz=48000)
+    assert bench.primal_kernel_bytes(lp, 4096, 4) == 836_064_004
+    assert bench.primal_kernel_bytes(lp, 4096, 8) == 1_671_904_004
+    # one instance, by hand: 6n + m values, then the shared part
+    assert bench.primal_kernel_bytes(lp, 1, 8) == (6 * 8000 + 3000) * 8 + 2 * 8000 * 8 \
+        + 48000 * 12 + 8001 * 4
+
+
+def test_traffic_per_tuned_kernel():
+    """Every primal variant the autotuner can pick has a measured HBM figure,
+    above the algorithmic bytes (part of the gathers is refetched); unknown
+    kernels and workloads give None."""
+    for prec, sv in (("mixed", 4), ("fp64", 8)):
+        t = "float" if prec == "mixed" else "double"
+        for cw in (1, 2, 4):
+            v = bench.read_traffic("c2", prec, f"kILPrimal<{t},{cw}>")
+            assert isinstance(v, int) and v > 0, (prec, cw, v)
+            lp = SimpleNamespace(num_rows=3000, num_cols=8000, nnz=48000)
+            assert v >= bench.primal_kernel_bytes(lp, 4096, sv)
+    assert bench.read_traffic("c2", "mixed", "kILPrimal<float,3>") is None
+    assert bench.read_traffic("c9", "mixed", "kILPrimal<float,1>") is None
