@@ -404,9 +404,10 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         }
         // FP64: the kCW columns lie in one 64-column tile (kCW divides 64); a
         // column whose bounds match the owner's in every lane reads the
-        // owner's value (all lanes load one cached address).  FP32 streams
-        // its own copies: there the flag word's wait costs more than the
-        // saved bytes (primal 180 -> 189 us with flags; FP64 365 -> 346 us).
+        // owner's value (all lanes load one cached address): primal 365 ->
+        // 346 us on C2.  FP32 streams its own copies: with the flags its
+        // primal measured slower (180 -> 189 us, 193 us with each group's
+        // flags staged in LDS instead of a scalar load per column).
         unsigned long long fw = ~0ull;
         if constexpr (sizeof(T) == 8) fw = bflags[static_cast<size_t>(g) * ntiles + (j0 >> 6)];
 #pragma unroll
