@@ -94,55 +94,125 @@ def read_traffic(workload, precision, kernel=None):
     return d.get("bytes_per_launch")
 
 
+def host_info():
+    """CPU model, logical CPUs usable by this process and physical cores of
+    the host (lscpu), recorded beside the CPU baselines."""
+    import subprocess
+    info = {"nproc": len(os.sched_getaffinity(0)), "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "1") or 1)
+        cores = int(kv.get("Core(s) per socket", "0") or 0)
+        info["physical_cores"] = sockets * cores if cores else None
+    except (OSError, ValueError, subprocess.SubprocessError):
+        info["model"] = info["physical_cores"] = None
+    return info
+
+
+def _pin(slot):
+    """Pin this process to the slot-th CPU it is allowed to run on."""
+    cpus = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {cpus[slot % len(cpus)]})
+    return cpus[slot % len(cpus)]
+
+
 def cpu_baseline(lp, lo, hi, budget_s):
-    """The reference's own CPU PDLP (oracle/_ref, built from its sources) or,
-    when that was not built, the oracle restatement, single-threaded, on
-    instances of the same workload until `budget_s` seconds are used."""
+    """The CPU restatement of the reference's PDLP (the oracle, bit-identical
+    to the reference's CPU path), single-threaded and pinned to one core, on
+    instances of the same workload until `budget_s` seconds are used.  The
+    instances it solves are compared with the GPU results afterwards."""
+    import threading
     from oracle import binding as ob
-    kind = "reference" if ob.ref_available() else "port"
-    solve = ob.ref_solve if kind == "reference" else ob.solve
-    t0 = time.perf_counter()
-    done = 0
-    while time.perf_counter() - t0 < budget_s and done < lo.shape[0]:
-        r, _, _ = solve(lp.with_col_bounds(lo[done], hi[done]), **TOL)
-        done += 1
-    el = time.perf_counter() - t0
-    return dict(value=done / el, unit="LP solves/s", cores=1, kind=kind,
-                sample=f"{done} C2 instances solved back to back on one host core in {el:.1f} s")
+    out = {}
+
+    def work():
+        out["cpu"] = _pin(0)
+        t0 = time.perf_counter()
+        done = []
+        while time.perf_counter() - t0 < budget_s and len(done) < lo.shape[0]:
+            i = len(done)
+            r, _, _ = ob.solve(lp.with_col_bounds(lo[i], hi[i]), **TOL)
+            done.append(r)
+        out["el"] = time.perf_counter() - t0
+        out["done"] = done
+    th = threading.Thread(target=work)
+    th.start()
+    th.join()
+    done, el = out["done"], out["el"]
+    return dict(value=len(done) / el, unit="LP solves/s", cores=1, kind="port",
+                sample=f"the first {len(done)} C2 instances solved back to back on one host "
+                       f"core (cpu {out['cpu']}) in {el:.1f} s"), done
 
 
 def _cpu_worker(args):
-    """One host process of cpu_parallel: C2 instances wid, wid + nw, ...
-    solved back to back until budget_s seconds are used."""
+    """One host process of cpu_parallel, pinned to its own core: C2
+    instances wid, wid + nw, ... solved back to back until budget_s seconds
+    are used."""
     wid, nw, count, budget_s = args
+    _pin(wid)
     from oracle import binding as ob
     from mipx_amd import lpgen
     g = lpgen.c2_shared()
-    solve = ob.ref_solve if ob.ref_available() else ob.solve
     t0 = time.perf_counter()
     done = 0
     i = wid
     while time.perf_counter() - t0 < budget_s and i < count:
         lo, hi = g.branch_bounds(lpgen.c2_instance_seed(i))
-        solve(g.lp.with_col_bounds(lo, hi), **TOL)
+        ob.solve(g.lp.with_col_bounds(lo, hi), **TOL)
         done += 1
         i += nw
     return done, time.perf_counter() - t0
 
 
 def cpu_parallel(count, budget_s, workers):
-    """The same CPU PDLP on `workers` host processes at once (one LP each,
-    as the reference would run independent node LPs): aggregate solves/s of
-    the host's share of cores, beside the single-core cpu_baseline."""
+    """The same CPU PDLP on `workers` host processes at once, each pinned to
+    its own core (one LP each, as the reference would run independent node
+    LPs): aggregate solves/s of the host's share of cores, beside the
+    single-core cpu_baseline."""
     import multiprocessing as mp
-    from oracle import binding as ob
-    kind = "reference" if ob.ref_available() else "port"
+    workers = max(1, min(workers, len(os.sched_getaffinity(0))))
     with mp.get_context("spawn").Pool(workers) as pool:
         res = pool.map(_cpu_worker, [(w, workers, count, budget_s) for w in range(workers)])
     done = sum(d for d, _ in res)
     el = max(e for _, e in res)
-    return dict(value=done / el, unit="LP solves/s", cores=workers, kind=kind,
-                sample=f"{done} C2 instances on {workers} host processes in {el:.1f} s")
+    return dict(value=done / el, unit="LP solves/s", cores=workers, kind="port",
+                sample=f"{done} C2 instances on {workers} host processes, one pinned core "
+                       f"each, in {el:.1f} s")
+
+
+def copy_ceiling(repeats=5):
+    """Best-of rate of a plain 1 GiB device-to-device streaming copy (read +
+    write bytes / kernel time): the measured bandwidth ceiling beside the
+    8 TB/s nominal peak."""
+    from mipx_amd import native
+    best = 0.0
+    for _ in range(repeats):
+        nb = native.copy_bench(1 << 30)
+        best = max(best, 2.0 * nb / (native.copy_bench_ms * 1e-3) / 1e9)
+    return round(best, 1)
+
+
+def iter_bytes(m, n, nnz, sv):
+    """Algorithmic bytes of one PDHG iteration of one LP:
+    2 nnz (s_v + 4) + (m + n + 2) 4 + 9 n s_v + 7 m s_v."""
+    return 2 * nnz * (sv + 4) + (m + n + 2) * 4 + 9 * n * sv + 7 * m * sv
+
+
+def whole_solve_bytes_shared(lp, res, rounds, N):
+    """Algorithmic bytes of a whole shared-A batch solve: per instance and
+    iteration (7 n + 4 m) s_v at the precision it ran in (FP32 iterations at
+    4 bytes, the rest at 8), plus per batched iteration the matrix in both
+    orientations and the shared vectors (c, tau, sigma, rl, ru) once, counted
+    at FP32 width (a lower bound)."""
+    m, n, nnz = lp.num_rows, lp.num_cols, lp.nnz
+    per = lambda sv: (7 * n + 4 * m) * sv
+    inst = sum(r["iterations_fp32"] * per(4) + (r["iterations"] - r["iterations_fp32"]) * per(8)
+               for r in res)
+    shared = rounds * N * (2 * nnz * 8 + (m + n + 2) * 4 + (2 * n + 3 * m) * 4)
+    return inst + shared
 
 
 def run_c4(a, prec):
@@ -281,17 +351,26 @@ def run_c4(a, prec):
 
 def cpu_baseline_lps(lps, budget_s):
     """As cpu_baseline, on the LPs of a heterogeneous batch in order."""
+    import threading
     from oracle import binding as ob
-    kind = "reference" if ob.ref_available() else "port"
-    solve = ob.ref_solve if kind == "reference" else ob.solve
-    t0 = time.perf_counter()
-    done = 0
-    while time.perf_counter() - t0 < budget_s and done < len(lps):
-        solve(lps[done], **TOL)
-        done += 1
-    el = time.perf_counter() - t0
-    return dict(value=done / el, unit="LP solves/s", cores=1, kind=kind,
-                sample=f"the first {done} C3 LPs solved back to back on one host core in {el:.1f} s")
+    out = {}
+
+    def work():
+        out["cpu"] = _pin(0)
+        t0 = time.perf_counter()
+        done = []
+        while time.perf_counter() - t0 < budget_s and len(done) < len(lps):
+            r, _, _ = ob.solve(lps[len(done)], **TOL)
+            done.append(r)
+        out["el"] = time.perf_counter() - t0
+        out["done"] = done
+    th = threading.Thread(target=work)
+    th.start()
+    th.join()
+    done, el = out["done"], out["el"]
+    return dict(value=len(done) / el, unit="LP solves/s", cores=1, kind="port",
+                sample=f"the first {len(done)} C3 LPs solved back to back on one host core "
+                       f"(cpu {out['cpu']}) in {el:.1f} s"), done
 
 
 def run_c1(a):
@@ -440,12 +519,23 @@ def main():
     native.synchronize()
     elapsed = time.perf_counter() - t0
     st = batch.stats()
-    res, _, _ = batch.results(want_vectors=False)
+    want_vec = not a.no_cpu
+    res, X, Y = batch.results(want_vectors=want_vec)
     status = np.bincount([r["status"] for r in res], minlength=7).tolist()
     iters = np.array([r["iterations"] for r in res])
     kkt = max(max(r["rel_primal"], r["rel_dual"], r["rel_gap"]) for r in res if r["status"] == 0)
     value = a.count * a.steps / elapsed
     log(f"batched: {value:.1f} solves/s, status counts {status}, mean iters {iters.mean():.0f}")
+
+    # Whole-solve algorithmic bytes per second (one step = one full solve).
+    N = 200
+    if lps is not None:
+        ws_bytes = sum(r["iterations_fp32"] * iter_bytes(l.num_rows, l.num_cols, l.nnz, 4) +
+                       (r["iterations"] - r["iterations_fp32"]) *
+                       iter_bytes(l.num_rows, l.num_cols, l.nnz, 8) for l, r in zip(lps, res))
+    else:
+        ws_bytes = whole_solve_bytes_shared(lp, res, st["rounds"], N)
+    ws_gbs = ws_bytes / (elapsed / a.steps) / 1e9
 
     prof = batch.profile(launches=50)
     sv = 4 if prec == 1 else 8
@@ -457,6 +547,24 @@ def main():
     log(f"primal kernel {prof['primal_kernel_us']:.1f} us for {prof['profiled_slots']} instances "
         f"-> {achieved:.0f} GB/s")
     batch.close()
+    kernels = {primal_kernel or "kPackedPrimal": {
+        "us": round(prof["primal_kernel_us"], 2), "GBs": round(achieved, 1),
+        "frac": round(achieved / PEAK_GBS, 4)}}
+    if prec == 1 and lps is None:
+        # The FP64 primal kernel of the same instances, timed the same way
+        # (an FP64 batch, every instance active).
+        b64 = native.ResidentBatch(lp, lo, hi, precision=0, max_iter=6 * N, **TOL)
+        b64.run()
+        p64 = b64.profile(launches=50)
+        b64.close()
+        by64 = primal_kernel_bytes(lp, p64["profiled_slots"], 8)
+        g64 = by64 / (p64["primal_kernel_us"] * 1e-6) / 1e9
+        kernels[f"kILPrimal<double,{p64['columns_per_wave']}>"] = {
+            "us": round(p64["primal_kernel_us"], 2), "GBs": round(g64, 1),
+            "frac": round(g64 / PEAK_GBS, 4), "dual_us": round(p64["dual_kernel_us"], 2)}
+        log(f"fp64 primal kernel {p64['primal_kernel_us']:.1f} us -> {g64:.0f} GB/s")
+    ceiling = copy_ceiling()
+    log(f"streaming copy ceiling {ceiling} GB/s")
 
     # Plain single-instance FP64 GPU PDLP on a sample of the same instances.
     t = time.perf_counter()
@@ -471,11 +579,26 @@ def main():
     plain = a.plain_sample / (time.perf_counter() - t)
     log(f"plain fp64 single-instance: {plain:.2f} solves/s")
 
-    cpu = cpu_par = None
+    cpu = cpu_par = vs_oracle = None
     if not a.no_cpu:
-        cpu = (cpu_baseline_lps(lps, a.cpu_seconds) if lps is not None
-               else cpu_baseline(lp, lo, hi, a.cpu_seconds))
+        cpu, cpu_res = (cpu_baseline_lps(lps, a.cpu_seconds) if lps is not None
+                        else cpu_baseline(lp, lo, hi, a.cpu_seconds))
+        cpu["host"] = host_info()
         log(f"cpu baseline: {cpu}")
+        # Each GPU result of the instances the CPU leg solved, against the
+        # oracle's FP64 solve of the same LP and the long-double certificate.
+        from oracle import binding as ob
+        errs, certs = [], []
+        for i, ro in enumerate(cpu_res):
+            one = lps[i] if lps is not None else lp.with_col_bounds(lo[i], hi[i])
+            if ro["status"] == 0 and res[i]["status"] == 0:
+                errs.append(abs(res[i]["objective"] - ro["objective"]) / (1 + abs(ro["objective"])))
+                k = ob.kkt_check(one, X[i], Y[i])
+                certs.append(max(k["rel_primal"], k["rel_dual"], k["rel_gap"]))
+        vs_oracle = {"instances": len(errs),
+                     "max_rel_objective_error": max(errs) if errs else None,
+                     "max_certified_rel_kkt": max(certs) if certs else None}
+        log(f"vs oracle: {vs_oracle}")
         if lps is None and a.cpu_workers > 1:
             cpu_par = cpu_parallel(a.count, a.cpu_seconds, a.cpu_workers)
             log(f"cpu parallel: {cpu_par}")
@@ -507,9 +630,17 @@ def main():
                                 else f"{primal_kernel} (primal half step, lane-per-instance)"),
                      "kernel_us": round(prof["primal_kernel_us"], 2),
                      "algorithmic_bytes_per_launch": int(bytes_launch),
-                     "instances_in_launch": prof["profiled_slots"]},
+                     "instances_in_launch": prof["profiled_slots"],
+                     "kernels": kernels,
+                     "whole_solve": {"achieved": round(ws_gbs, 1), "frac": round(ws_gbs / PEAK_GBS, 4),
+                                     "algorithmic_bytes": int(ws_bytes),
+                                     "definition": "sum over instances of iterations x bytes "
+                                                   "per iteration at the precision run / wall "
+                                                   "time of one solve"},
+                     "copy_ceiling_GBs": ceiling},
         "cpu_baseline": cpu,
         "cpu_parallel": cpu_par,
+        "vs_oracle": vs_oracle,
         "plain_fp64_gpu_solves_per_s": round(plain, 3),
         "speedup_vs_plain": round(value / plain, 2),
         "latency_ms": {"p50": round(st["latency_p50_ms"], 2), "p99": round(st["latency_p99_ms"], 2),

@@ -15,6 +15,11 @@
 //   * x0 ~ U(0, xhi), b = A x0; <= rows get ru = b + U(0, slack), rl = -inf;
 //     equality rows rl = ru = b;
 //   * y0 ~ N(0, 1) with y0 <= 0 on <= rows, s0 = |N(0, 1)|, c = A' y0 + s0.
+//   * optional column kinds (fractions; 0 keeps every column boxed and the
+//     LP bit-identical to a generator without them): free columns
+//     (-inf < x < inf, c = A' y0), lower-only (0 <= x, c = A' y0 + s0) and
+//     upper-only (x <= xhi, c = A' y0 - s0), drawn from a second stream, so
+//     that x0 stays feasible and y0 dual feasible.
 // CSR output has sorted column indices in every row.
 
 #include <algorithm>
@@ -76,6 +81,9 @@ typedef struct mpxgen_params {
     double slack;         // <= row slack ~ U(0, slack)
     double eq_fraction;   // leading fraction of equality rows
     uint64_t seed;
+    double free_fraction;        // columns with no bounds
+    double lower_only_fraction;  // columns 0 <= x
+    double upper_only_fraction;  // columns x <= xhi
 } mpxgen_params;
 
 struct MpxGen {
@@ -209,7 +217,27 @@ void* mpxgen_create(const mpxgen_params* p) {
     g->obj.assign(static_cast<size_t>(n), 0.0);
     for (int32_t i = 0; i < m; ++i)
         for (int32_t k = g->rs[i]; k < g->rs[i + 1]; ++k) g->obj[g->ci[k]] += g->val[k] * y0[i];
-    for (int32_t j = 0; j < n; ++j) g->obj[j] += std::abs(rng.normal());
+    std::vector<double> s0(static_cast<size_t>(n));
+    for (int32_t j = 0; j < n; ++j) s0[j] = std::abs(rng.normal());
+    const double f_free = p->free_fraction, f_lo = p->lower_only_fraction,
+                 f_up = p->upper_only_fraction;
+    if (f_free > 0.0 || f_lo > 0.0 || f_up > 0.0) {
+        Rng kinds(p->seed ^ 0x0B0DDEDC01ull);
+        for (int32_t j = 0; j < n; ++j) {
+            const double u = kinds.uniform();
+            if (u < f_free) {
+                g->cl[j] = -inf;
+                g->cu[j] = inf;
+                s0[j] = 0.0;
+            } else if (u < f_free + f_lo) {
+                g->cu[j] = inf;
+            } else if (u < f_free + f_lo + f_up) {
+                g->cl[j] = -inf;
+                s0[j] = -s0[j];
+            }
+        }
+    }
+    for (int32_t j = 0; j < n; ++j) g->obj[j] += s0[j];
     return g;
 }
 

@@ -1,11 +1,15 @@
 // pdlp.h -- PdlpSolver with the reference's interface (include/mipx/pdlp.h:14-174),
 // running every solve on the MI355X through the C-ABI of include/mipx_pdlp.h.
 //
-// Options keep the reference's names and defaults.  use_gpu, gpu_min_rows and
-// gpu_min_nnz are kept for source compatibility: this class is the device
-// backend and always solves on the GPU (the reference routes small LPs to its
-// CPU loop before reaching solveGpu(), pdlp.cpp:862-869).  Extensions:
-// `precision` (FP64 or mixed FP32/FP64) and `spmv_group`.
+// Options keep the reference's names and defaults.  The reference's GPU gate
+// (pdlp.cpp:862-869: use_gpu && rows >= gpu_min_rows && kept nonzeros >=
+// gpu_min_nnz) is honoured: an LP the gate would send to the reference's CPU
+// loop gets Status::Error with lastError() saying so, and usedGpu() is false,
+// because that CPU loop is not part of this library (a caller that wants it
+// keeps the reference's own CPU branch, as the binding of the reference does).
+// The trivial n == 0 / m == 0 cases are solved in closed form before the
+// gate, as in the reference.  Extensions: `precision` (FP64 or mixed
+// FP32/FP64), `spmv_group`, `device`, `mixed_handover`, `spmv_chunk_cols`.
 #pragma once
 
 #include <atomic>
@@ -103,6 +107,8 @@ public:
     [[nodiscard]] bool usedGpu() const { return used_gpu_; }
     // Text of the last device error (empty when none).
     [[nodiscard]] const std::string& lastError() const { return last_error_; }
+    // Whether solve() would run this LP on the GPU (the reference's gate).
+    [[nodiscard]] bool passesGpuGate() const;
 
 private:
     PdlpOptions options_{};

@@ -38,16 +38,105 @@ static LpProblem lp2(std::vector<Real> obj, std::vector<Real> lo, std::vector<Re
     return p;
 }
 
+// The closed-form LPs are far below the reference's GPU gate (512 rows,
+// 10,000 nonzeros), so the device tests open it.
 static PdlpOptions quiet(Int max_iter = 50000) {
     PdlpOptions o;
     o.verbose = false;
     o.max_iter = max_iter;
+    o.gpu_min_rows = 0;
+    o.gpu_min_nnz = 0;
     return o;
 }
 
-int main() {
+// Reduced costs in original space, c + A'y, by a dense loop over the
+// triplets (independent of SparseMatrix::multiplyTranspose).
+static std::vector<Real> denseReducedCosts(const std::vector<Real>& c,
+                                           const std::vector<Triplet>& t,
+                                           const std::vector<Real>& y) {
+    std::vector<Real> rc = c;
+    for (const auto& e : t) rc[static_cast<size_t>(e.col)] += e.val * y[static_cast<size_t>(e.row)];
+    return rc;
+}
+
+// The reference's GPU gate (pdlp.cpp:862-869) in the mirror.  No device call:
+// every LP here is either rejected by the gate before the library is used,
+// or only asked whether it would pass.
+static void gateTests(const LpProblem& simple) {
+    {  // default options: 1 row < 512, so the reference would run its CPU loop
+        PdlpSolver s;
+        PdlpOptions o;
+        o.verbose = false;
+        s.setOptions(o);
+        s.load(simple);
+        CHECK(!s.passesGpuGate());
+        const LpResult r = s.solve();
+        CHECK(r.status == Status::Error && r.objective == 0.0 && r.iterations == 0);
+        CHECK(!s.usedGpu());
+        CHECK(s.lastError().find("GPU gate") != std::string::npos);
+        CHECK(s.getPrimalValues().size() == 2 && s.getPrimalValues()[0] == 0.0);
+    }
+    {  // thresholds open, use_gpu off
+        PdlpSolver s;
+        PdlpOptions o = quiet();
+        o.use_gpu = false;
+        s.setOptions(o);
+        s.load(simple);
+        CHECK(!s.passesGpuGate());
+        CHECK(s.solve().status == Status::Error && !s.usedGpu());
+        o.use_gpu = true;
+        s.setOptions(o);
+        CHECK(s.passesGpuGate());
+    }
+    {  // rows and nonzeros at the thresholds pass; one below fails
+        PdlpSolver s;
+        PdlpOptions o = quiet();
+        o.gpu_min_rows = 1;
+        o.gpu_min_nnz = 2;
+        s.setOptions(o);
+        s.load(simple);
+        CHECK(s.passesGpuGate());
+        o.gpu_min_nnz = 3;
+        s.setOptions(o);
+        CHECK(!s.passesGpuGate());
+        o.gpu_min_nnz = 2;
+        o.gpu_min_rows = 2;
+        s.setOptions(o);
+        CHECK(!s.passesGpuGate());
+    }
+    {  // the gate counts entries that survive matrix_zero_tol (pdlp.cpp:499-518)
+        PdlpSolver s;
+        PdlpOptions o = quiet();
+        o.gpu_min_nnz = 2;
+        o.matrix_zero_tol = 1e-3;
+        s.setOptions(o);
+        s.load(lp2({-1, -1}, {0, 0}, {kInf, kInf}, {-kInf}, {4.0}, {{0, 0, 1.0}, {0, 1, 1e-4}}));
+        CHECK(!s.passesGpuGate());
+        o.matrix_zero_tol = 0.0;
+        s.setOptions(o);
+        CHECK(s.passesGpuGate());
+    }
+    {  // trivial LPs (no rows or no columns) are closed forms before the gate
+        PdlpSolver s;
+        PdlpOptions o;
+        o.use_gpu = false;
+        s.setOptions(o);
+        s.load(lp2({1.0}, {1.0}, {3.0}, {}, {}, {}));
+        CHECK(s.passesGpuGate());
+        LpProblem empty;
+        s.load(empty);
+        CHECK(s.passesGpuGate());
+    }
+}
+
+int main(int argc, char** argv) {
     const LpProblem simple = lp2({-1, -1}, {0, 0}, {kInf, kInf}, {-kInf}, {4.0},
                                  {{0, 0, 1.0}, {0, 1, 1.0}});
+    gateTests(simple);
+    if (argc > 1 && std::string(argv[1]) == "--gate-only") {
+        std::printf("host_tests: %d passed, %d failed\n", g_pass, g_fail);
+        return g_fail == 0 ? 0 : 1;
+    }
     {  // test_pdlp.cpp:88-107
         PdlpSolver s;
         s.setOptions(quiet(10000));
@@ -59,8 +148,14 @@ int main() {
         const auto x = s.getPrimalValues();
         CHECK(x.size() == 2 && x[0] >= -1e-6 && x[1] >= -1e-6 && x[0] + x[1] <= 4.0 + 1e-3);
         CHECK(s.usedGpu());
+        // getReducedCosts = c + A'y in original space (pdlp.cpp:2808-2819)
         const auto rc = s.getReducedCosts();
-        CHECK(rc.size() == 2);
+        const auto want = denseReducedCosts({-1, -1}, {{0, 0, 1.0}, {0, 1, 1.0}}, s.getDualValues());
+        CHECK(rc.size() == 2 && rc == want);
+        // at the optimum of min -x0 - x1 s.t. x0 + x1 <= 4 the row is tight
+        // and both reduced costs vanish with y = 1 (mipx's sign convention)
+        CHECK(std::abs(rc[0]) <= 1e-4 && std::abs(rc[1]) <= 1e-4);
+        CHECK(std::abs(s.getDualValues()[0] - 1.0) <= 1e-4);
     }
     {  // bounds-only, test_pdlp.cpp:109-124
         PdlpSolver s;
@@ -102,6 +197,11 @@ int main() {
         s.load(lp2({-1, -2}, {0, 0}, {3, 5}, {-kInf}, {100.0}, {{0, 0, 1.0}, {0, 1, 1.0}}));
         LpResult r = s.solve();
         CHECK(r.status == Status::Optimal && std::abs(r.objective + 13.0) <= 1e-3);
+        // both columns sit at their upper bounds with a slack row: y = 0 and
+        // the reduced costs are the objective (-1, -2)
+        const auto rc = s.getReducedCosts();
+        CHECK(rc == denseReducedCosts({-1, -2}, {{0, 0, 1.0}, {0, 1, 1.0}}, s.getDualValues()));
+        CHECK(std::abs(rc[0] + 1.0) <= 1e-4 && std::abs(rc[1] + 2.0) <= 1e-4);
         s.setColBounds(1, 0.0, 4.0);  // optimum moves to -11
         r = s.solve();
         CHECK(r.status == Status::Optimal && std::abs(r.objective + 11.0) <= 1e-3);

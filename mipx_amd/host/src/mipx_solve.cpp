@@ -13,6 +13,12 @@
 //              [--pdlp] [--time-limit S] [--pdlp-opt-norm l2|linf]
 //              [--precision fp64|mixed] [--tol T] [--quiet] [--print-backend]
 //              [--write-mps OUT.mps] [--max-iter N]
+//              [--gpu|--no-gpu] [--gpu-min-rows N] [--gpu-min-nnz N]
+//
+// The gate flags keep the reference CLI's defaults (512 rows, 10,000
+// nonzeros); an LP below them reports "PDLP backend: CPU" and status Error,
+// since the reference's CPU loop is not part of this library.
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -112,6 +118,9 @@ int main(int argc, char** argv) {
     bool verbose = true, print_backend = false;
     double time_limit = -1.0, tol = 1e-6;
     int max_iter = 0;
+    // The reference CLI's GPU gate flags and defaults (main.cpp:75-80, 284-293).
+    bool use_gpu = true;
+    int gpu_min_rows = 512, gpu_min_nnz = 10000;
     mipx::PdlpOptimalityNorm norm = mipx::PdlpOptimalityNorm::L2;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -141,6 +150,10 @@ int main(int argc, char** argv) {
         else if (a == "--print-backend") print_backend = true;
         else if (a == "--write-mps") write_mps = next();
         else if (a == "--max-iter") max_iter = std::atoi(next().c_str());
+        else if (a == "--gpu") use_gpu = true;
+        else if (a == "--no-gpu") use_gpu = false;
+        else if (a == "--gpu-min-rows") gpu_min_rows = std::max(0, std::atoi(next().c_str()));
+        else if (a == "--gpu-min-nnz") gpu_min_nnz = std::max(0, std::atoi(next().c_str()));
         else if (a.size() > 4 && a[0] != '-') mps = a;
         else {
             std::fprintf(stderr, "Error: unknown option %s\n", a.c_str());
@@ -164,6 +177,9 @@ int main(int argc, char** argv) {
     o.verbose = verbose;
     o.max_solve_seconds = time_limit;
     o.optimality_norm = norm;
+    o.use_gpu = use_gpu;
+    o.gpu_min_rows = gpu_min_rows;
+    o.gpu_min_nnz = gpu_min_nnz;
     o.primal_tol = o.dual_tol = o.optimality_tol = tol;
     if (max_iter > 0) o.max_iter = max_iter;
     o.precision = precision == "mixed" ? mipx::PdlpPrecision::Mixed : mipx::PdlpPrecision::FP64;

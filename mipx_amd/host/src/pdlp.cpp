@@ -8,6 +8,7 @@
 #include "mipx/pdlp.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "mipx_pdlp.h"
@@ -127,6 +128,18 @@ LpResult PdlpSolver::solve() {
                       static_cast<Index>(warm_y_.size()) == problem_.num_rows;
     primal_.assign(static_cast<size_t>(problem_.num_cols), 0.0);
     dual_.assign(static_cast<size_t>(problem_.num_rows), 0.0);
+    if (!passesGpuGate()) {
+        // The reference runs its CPU loop for such LPs (pdlp.cpp:862-869);
+        // that loop is not part of this library.
+        used_gpu_ = false;
+        last_error_ =
+            "LP is below the GPU gate (use_gpu, gpu_min_rows, gpu_min_nnz): the CPU PDLP loop "
+            "is not part of this library";
+        status_ = Status::Error;
+        objective_ = 0.0;
+        iterations_ = 0;
+        return {status_, 0.0, 0, 0.0};
+    }
     mpx_pdlp_result r{};
     const int rc = mpx_pdlp_solve(&lp, &o, warm ? warm_x_.data() : nullptr,
                                   warm ? warm_y_.data() : nullptr, primal_.data(), dual_.data(), &r);
@@ -144,6 +157,20 @@ LpResult PdlpSolver::solve() {
     objective_ = r.objective;
     iterations_ = r.iterations;
     return {status_, objective_, iterations_, r.work_units};
+}
+
+bool PdlpSolver::passesGpuGate() const {
+    // Trivial LPs are closed forms evaluated before the gate (pdlp.cpp:812-860).
+    if (problem_.num_cols == 0 || problem_.num_rows == 0) return true;
+    // The gate counts the entries that survive matrix_zero_tol (pdlp.cpp:499-518, 863-864).
+    Index kept = problem_.matrix.numNonzeros();
+    if (options_.matrix_zero_tol > 0.0) {
+        kept = 0;
+        for (const Real v : problem_.matrix.csr_values())
+            if (std::abs(v) >= options_.matrix_zero_tol) ++kept;
+    }
+    return options_.use_gpu && problem_.num_rows >= options_.gpu_min_rows &&
+           kept >= options_.gpu_min_nnz;
 }
 
 std::vector<Real> PdlpSolver::getReducedCosts() const {

@@ -32,6 +32,8 @@ class GenParams(C.Structure):
         ("value_kind", C.c_int32), ("vlo", C.c_double), ("vhi", C.c_double),
         ("xhi", C.c_double), ("slack", C.c_double), ("eq_fraction", C.c_double),
         ("seed", C.c_uint64),
+        ("free_fraction", C.c_double), ("lower_only_fraction", C.c_double),
+        ("upper_only_fraction", C.c_double),
     ]
 
 
@@ -94,8 +96,10 @@ class Generated:
 
 
 def params(m, n, colnnz_kind=0, lo=5, hi=5, mean=0.0, value_kind=0, vlo=0.1, vhi=1.0,
-           seed=0, xhi=10.0, slack=1.0, eq_fraction=0.5):
+           seed=0, xhi=10.0, slack=1.0, eq_fraction=0.5, free=0.0, lower_only=0.0,
+           upper_only=0.0):
     p = GenParams()
+    p.free_fraction, p.lower_only_fraction, p.upper_only_fraction = free, lower_only, upper_only
     p.m, p.n = int(m), int(n)
     p.colnnz_kind, p.colnnz_lo, p.colnnz_hi, p.colnnz_mean = colnnz_kind, lo, hi, mean
     p.value_kind, p.vlo, p.vhi = value_kind, vlo, vhi
@@ -104,9 +108,12 @@ def params(m, n, colnnz_kind=0, lo=5, hi=5, mean=0.0, value_kind=0, vlo=0.1, vhi
     return p
 
 
-def c0_plain(seed=0xC0, m=2000, n=5000, per_col=5):
+def c0_plain(seed=0xC0, m=2000, n=5000, per_col=5, free=0.0, lower_only=0.0, upper_only=0.0):
+    """C0 family; `free`, `lower_only`, `upper_only` turn that fraction of
+    the columns into free / 0 <= x / x <= 10 columns (dual kept feasible)."""
     return Generated(params(m, n, 0, per_col, per_col, value_kind=0, vlo=0.1, vhi=1.0,
-                            seed=seed))
+                            seed=seed, free=free, lower_only=lower_only,
+                            upper_only=upper_only))
 
 
 def c1_large(seed=0xC1, m=1_000_000, n=2_000_000):

@@ -43,3 +43,31 @@ def test_traffic_per_tuned_kernel():
             assert v >= bench.primal_kernel_bytes(lp, 4096, sv)
     assert bench.read_traffic("c2", "mixed", "kILPrimal<float,3>") is None
     assert bench.read_traffic("c9", "mixed", "kILPrimal<float,1>") is None
+
+
+def test_iteration_bytes_match_baseline_table():
+    """B_iter = 2 nnz (s_v + 4) + (m + n + 2) 4 + 9 n s_v + 7 m s_v: C0 is
+    1.10 MB (FP64) / 0.66 MB (FP32), C1 500 MB / 304 MB per iteration."""
+    assert round(bench.iter_bytes(2000, 5000, 25000, 8) / 1e6, 2) == 1.10
+    assert round(bench.iter_bytes(2000, 5000, 25000, 4) / 1e6, 2) == 0.66
+    assert round(bench.iter_bytes(1_000_000, 2_000_000, 12_000_000, 8) / 1e6) == 500
+    assert round(bench.iter_bytes(1_000_000, 2_000_000, 12_000_000, 4) / 1e6) == 304
+
+
+def test_whole_solve_bytes_shared():
+    """Per instance (7 n + 4 m) s_v per iteration at the precision run, plus
+    the shared matrix and vectors once per batched iteration."""
+    lp = SimpleNamespace(num_rows=3000, num_cols=8000, nnz=48000)
+    res = [dict(iterations=100, iterations_fp32=60), dict(iterations=50, iterations_fp32=0)]
+    per4, per8 = (7 * 8000 + 4 * 3000) * 4, (7 * 8000 + 4 * 3000) * 8
+    shared = 2 * 48000 * 8 + (3000 + 8000 + 2) * 4 + (2 * 8000 + 3 * 3000) * 4
+    assert bench.whole_solve_bytes_shared(lp, res, 1, 200) == \
+        60 * per4 + 40 * per8 + 50 * per8 + 200 * shared
+
+
+def test_cpu_baseline_runs_the_restatement_only():
+    """The CPU legs time the oracle (the restatement), never the reference's
+    own build: that build stays in this container."""
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert "ref_solve" not in src and "ref_available" not in src
+    assert '"reference"' not in src

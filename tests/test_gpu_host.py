@@ -10,6 +10,9 @@ from oracle import binding as ob
 
 pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mipx_amd", "bin")
+# The test LPs are below the reference CLI's default GPU gate (512 rows,
+# 10,000 nonzeros, src/cli/main.cpp:79-80).
+OPEN_GATE = ["--gpu-min-rows", "0", "--gpu-min-nnz", "0"]
 
 
 def test_cpp_host_tests():
@@ -22,7 +25,7 @@ def test_cpp_host_tests():
 
 def test_cli_stable_lines_match_oracle():
     r = subprocess.run([os.path.join(BIN, "mipx_solve"), "--gen", "c0", "--seed", "101",
-                        "--rows", "300", "--cols", "750", "--pdlp", "--print-backend", "--quiet"],
+                        "--rows", "300", "--cols", "750", "--pdlp", "--print-backend", "--quiet"] + OPEN_GATE,
                        capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0
@@ -47,7 +50,7 @@ def test_cli_mps_round_trip(tmp_path):
     """A generated LP written as MPS (--write-mps) and solved from the file
     gives the same stable lines as solving it directly."""
     mps = str(tmp_path / "c0.mps")
-    base = [os.path.join(BIN, "mipx_solve"), "--pdlp", "--print-backend", "--quiet"]
+    base = [os.path.join(BIN, "mipx_solve"), "--pdlp", "--print-backend", "--quiet"] + OPEN_GATE
     r1 = subprocess.run(base + ["--gen", "c0", "--seed", "101", "--rows", "300", "--cols", "750",
                                 "--write-mps", mps], capture_output=True, text=True, timeout=120)
     r2 = subprocess.run(base + [mps], capture_output=True, text=True, timeout=120)
@@ -66,7 +69,7 @@ def test_cli_reads_reference_tiny_mps():
     from mipx_amd.lp import from_dense
     from lp_cases import GOLDEN
     r = subprocess.run([os.path.join(BIN, "mipx_solve"), os.path.join(GOLDEN, "tiny.mps"),
-                        "--pdlp", "--print-backend", "--quiet", "--max-iter", "20000"],
+                        "--pdlp", "--print-backend", "--quiet", "--max-iter", "20000"] + OPEN_GATE,
                        capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0

@@ -336,3 +336,62 @@ def test_oracle_preconditioner_refresh():
             assert (r1["status"], r1["iterations"]) == (r2["status"], r2["iterations"])
             assert r1["objective"] == r2["objective"]
             assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+
+
+def test_oracle_reduced_costs_dense():
+    """orc_reduced_costs = c + A'y in original space (src/lp/pdlp.cpp:2808-2819),
+    against a dense product; a transposed or sign-flipped product fails it
+    (the matrix is not square and y has zero entries that the scatter skips)."""
+    lp = lpgen.c0_plain(seed=11, m=40, n=90, per_col=4).lp
+    rng = np.random.default_rng(3)
+    y = rng.standard_normal(lp.num_rows)
+    y[::5] = 0.0
+    rc = ob.reduced_costs(lp, y)
+    want = lp.obj + lp.dense().T @ y
+    np.testing.assert_allclose(rc, want, rtol=1e-13, atol=1e-13)
+    assert np.abs(rc - (lp.obj - lp.dense().T @ y)).max() > 1e-3
+    # At an optimum of a boxed LP, columns strictly inside their bounds have
+    # zero reduced cost, those at a lower (upper) bound a nonnegative
+    # (nonpositive) one, up to the solve's tolerance.
+    lp2 = lpgen.c0_plain(seed=5, m=50, n=120).lp
+    r, x, yy = ob.solve(lp2, **TIGHT)
+    rc2 = ob.reduced_costs(lp2, yy)
+    scale = 1e-4 * (1 + np.abs(lp2.obj).max())
+    inside = (x > lp2.col_lower + 1e-6) & (x < lp2.col_upper - 1e-6)
+    assert inside.any() and np.abs(rc2[inside]).max() <= scale
+    assert rc2[x <= lp2.col_lower + 1e-9].min() >= -scale
+    assert rc2[x >= lp2.col_upper - 1e-9].max() <= scale
+
+
+@pytest.mark.parametrize("which", ["boxed", "one_sided"])
+def test_certificate_agrees_with_oracle_report(which):
+    """The certify() helper used by the GPU tests holds for the oracle's own
+    results: reported residuals agree with the long-double certificate."""
+    from lp_cases import certify, one_sided_lp
+    lp = (one_sided_lp(seed=0x5D0, m=200, n=500) if which == "one_sided"
+          else lpgen.c0_plain(seed=101, m=300, n=750).lp)
+    r, x, y = ob.solve(lp, **TIGHT)
+    assert r["status"] == 0
+    k = certify(lp, r, x, y)
+    if which == "one_sided":
+        assert k["rel_dual"] > 0.0
+        # A planted dual error shows up in the dual residual itself.
+        bad = ob.kkt_check(lp, x, 1.01 * y)
+        assert bad["rel_dual"] > 1e-4
+
+
+def test_farkas_bounds_variant_detected_by_oracle():
+    """A bounds variant made infeasible by farkas_bounds is certified by the
+    reference's own rule (oracle/_ref when built) and by the oracle at the
+    first certificate check, with the ray on the violated row."""
+    from lp_cases import farkas_bounds
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    lo, hi = farkas_bounds(g.lp, *g.branch_bounds(lpgen.c2_instance_seed(7)), 17)
+    lp = g.lp.with_col_bounds(lo, hi)
+    r, x, y = ob.solve(lp, max_iter=100000, **TIGHT)
+    assert r["status"] == 1 and r["iterations"] == 2000
+    ray = y / np.linalg.norm(y)
+    assert abs(abs(ray[17]) - 1.0) <= 1e-3
+    if ob.ref_available():
+        rr, _, yr = ob.ref_solve(lp, max_iter=100000, **TIGHT)
+        assert rr["status"] == 1 and rr["iterations"] == 2000 and np.array_equal(y, yr)
