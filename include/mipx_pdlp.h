@@ -444,6 +444,26 @@ int mpx_pdhg_iterate(const mpx_lp* lp, const mpx_scaled_problem* sp,
                      double* current_x, double* initial_x, double* pdhg_x,
                      double* current_y, double* initial_y, double* pdhg_y);
 
+/*
+ * Relative KKT residuals of an original-space pair (x, y) on the device, in
+ * original units: the termination quantities of the reference's evaluation
+ * (src/lp/pdlp.cpp:1105-1202, GPU loop 2196-2269; norms of the original
+ * problem, 53-81), computed from the LP as given so that any returned
+ * solution can be certified where it lies.  `lp` is a host struct whose
+ * array pointers are DEVICE pointers (CSR); col_starts[num_cols + 1],
+ * row_indices[nnz], col_values[nnz] are A in CSC (device).  x[num_cols],
+ * y[num_rows] device.  out (device, 8 doubles) receives rel_primal (L2),
+ * rel_dual (L2: the part of sgn (c + A'y) that no finite bound absorbs),
+ * rel_gap, primal objective, dual objective, max bound violation of x, max
+ * row violation (absolute), max dual-sign violation of y.  Enqueued on
+ * `stream` (hipStream_t as void*, NULL = default); the per-block partial
+ * sums live in a stream-ordered allocation.  Deterministic (fixed-order
+ * reductions, one thread per row / column).
+ */
+int mpx_kkt_check_device(const mpx_lp* lp, const int32_t* col_starts,
+                         const int32_t* row_indices, const double* col_values,
+                         const double* x, const double* y, double* out, void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

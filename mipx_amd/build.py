@@ -15,7 +15,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 ROOT = os.path.dirname(PKG)
 
-HIP_SOURCES = ["pdlp_solver.hip", "pdlp_batch.hip"]
+HIP_SOURCES = ["pdlp_solver.hip", "pdlp_batch.hip", "pdlp_kkt.hip"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 

@@ -1452,6 +1452,10 @@ void SingleSolver::iterateFrom(const mpx_scaled_problem* sp, int iters, int inne
     MPX_HIP(hipMemcpyAsync(v64_.y, cy, m * 8, hipMemcpyHostToDevice, st));
     MPX_HIP(hipMemcpyAsync(v64_.y0, iy, m * 8, hipMemcpyHostToDevice, st));
     MPX_HIP(hipMemcpyAsync(v64_.yh, py, m * 8, hipMemcpyHostToDevice, st));
+    // The chunk-major copy of A used by the chunked dual step follows the
+    // caller's values too (setup gathered it from its own scaling).
+    if (nnz_ && nch_)
+        kGather<<<ceilDiv(nnz_, kBlock), kBlock, 0, st>>>(nnz_, permch_, v64_.val, valch64_);
     step_ = step;
     pw_ = pw;
     inner_ = inner;

@@ -23,6 +23,7 @@ EXPORTED = [
     "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
     "mpx_pdhg_iterate", "mpx_pdlp_scale_problem_batched", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
     "mpx_batch_results", "mpx_batch_get_stats", "mpx_batch_profile", "mpx_batch_destroy",
+    "mpx_kkt_check_device",
 ]
 
 
@@ -81,6 +82,7 @@ def lib():
     L.mpx_batch_get_stats.argtypes = [C.c_void_p, C.POINTER(MpxBatchStats)]
     L.mpx_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.POINTER(MpxBatchStats)]
     L.mpx_batch_destroy.argtypes = [C.c_void_p]
+    L.mpx_kkt_check_device.argtypes = [C.POINTER(MpxLp)] + [C.c_void_p] * 7
     _lib = L
     return L
 
@@ -358,6 +360,37 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+KKT_FIELDS = ["rel_primal", "rel_dual", "rel_gap", "primal_obj", "dual_obj", "bound_viol",
+              "row_viol_max", "dual_sign_viol"]
+
+
+def kkt_check_device(lp, x, y, entry="mpx_kkt_check_device"):
+    """Relative KKT residuals of (x, y) computed on the device
+    (mpx_kkt_check_device, or another entry point with its signature): the
+    LP, its CSC and the vectors are copied to device memory with torch, the
+    eight outputs come back as a dict."""
+    import scipy.sparse as sp
+    import torch
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    a = sp.csr_matrix((lp.values, lp.col_indices, lp.row_starts),
+                      shape=(lp.num_rows, lp.num_cols)).tocsc()
+    a.sort_indices()
+    keep = [dev(v) for v in (lp.row_starts, lp.col_indices, lp.values, lp.obj, lp.col_lower,
+                             lp.col_upper, lp.row_lower, lp.row_upper,
+                             a.indptr.astype(np.int32), a.indices.astype(np.int32), a.data,
+                             np.asarray(x, np.float64), np.asarray(y, np.float64))]
+    out = torch.zeros(8, dtype=torch.float64, device="cuda")
+    p = [t.data_ptr() for t in keep]
+    from ._abi import c_double_p, c_int32_p
+    ptrs = [C.cast(v, c_int32_p if k < 2 else c_double_p) for k, v in enumerate(p[:8])]
+    clp = MpxLp(lp.num_rows, lp.num_cols, lp.nnz, lp.sense, *ptrs, lp.obj_offset)
+    stream = torch.cuda.current_stream().cuda_stream
+    _check(getattr(lib(), entry)(C.byref(clp), p[8], p[9], p[10], p[11], p[12],
+                                 out.data_ptr(), stream), entry)
+    torch.cuda.synchronize()
+    return dict(zip(KKT_FIELDS, out.cpu().numpy().tolist()))
 
 
 def synchronize(device=0):

@@ -16,18 +16,20 @@ def _check_vs_oracle(lp, r, x, y, tag="", mixed=False):
     Mixed precision (FP32 iterations, FP64 evaluation and refinement) lands
     on a different point of the 1e-6 KKT region.  A 1e-6 relative KKT
     tolerance does not bound the objective error by 1e-6: the oracle itself
-    (bit-identical to the reference) is up to 2.6e-6 from the exact optimum
-    on these LP families.  Mixed results are held to that envelope: within
-    3e-6 of the exact optimum (scipy HiGHS), within 5e-6 of the oracle, and
-    KKT-certified by the independent long-double checker."""
+    (bit-identical to the reference) is up to 3.9e-6 from the exact optimum
+    on these LP families.  Mixed results at tolerance 1e-6 are therefore held
+    to 3e-6 of the exact optimum (scipy HiGHS) and KKT-certified by the
+    independent long-double checker; the 1e-6 bar against the oracle holds
+    when both solve to 1e-8 (test_mixed_refined_to_1e8_matches_oracle)."""
     ro, xo, yo = ob.solve(lp, **TIGHT)
     assert r["status"] == ro["status"], (tag, r, ro["status"])
     if ro["status"] == 0:
-        rel = abs(r["objective"] - ro["objective"]) / (1 + abs(ro["objective"]))
-        assert rel <= (5e-6 if mixed else 1e-6), (tag, r["objective"], ro["objective"], rel)
         if mixed:
             opt = highs_objective(lp)
             assert abs(r["objective"] - opt) <= 3e-6 * (1 + abs(opt)), (tag, r["objective"], opt)
+        else:
+            rel = abs(r["objective"] - ro["objective"]) / (1 + abs(ro["objective"]))
+            assert rel <= 1e-6, (tag, r["objective"], ro["objective"], rel)
         certify(lp, r, x, y, tag=tag)
     return ro
 
@@ -206,3 +208,38 @@ def test_batch_preconditioner_refresh(precision):
         assert (a["iterations"], a["objective"]) == (b["iterations"], b["objective"])
     assert np.array_equal(X0, X1) and np.array_equal(Y0, Y1)
     assert any(abs(r["primal_weight"] - 1.0) > 0.01 for r in r1)
+
+
+REFINED = dict(primal_tol=1e-8, dual_tol=1e-8, optimality_tol=1e-8, verbose=0)
+
+
+def test_mixed_refined_to_1e8_matches_oracle():
+    """Mixed precision refined to 1e-8 (FP32 iterations, FP64 phase to
+    refine_tol; BASELINE config 1's "final refinement to 1e-8") against the
+    oracle solved to 1e-8: objective within 1e-6 of the oracle's and of the
+    exact optimum, the returned point certified at 1e-8, on the single-LP
+    path, a shared-matrix batch (lane-per-instance kernels) and a packed
+    batch."""
+    opts = dict(TIGHT, precision=1, refine_tol=1e-8, max_iter=2000000)
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    count = 6
+    lo = np.empty((count, 800))
+    hi = np.empty((count, 800))
+    for i in range(count):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    res, X, Y = native.solve_batch_shared(g.lp, lo, hi, **opts)
+    cases = [(g.lp.with_col_bounds(lo[i], hi[i]), res[i], X[i], Y[i]) for i in range(count)]
+    lps = [lpgen.c3_instance(i, m).lp for i, m in enumerate([60, 150, 240])]
+    pr, xs, ys = native.solve_batch(lps, **opts)
+    cases += list(zip(lps, pr, xs, ys))
+    one = lpgen.c0_plain(seed=101, m=300, n=750).lp
+    cases.append((one,) + native.solve(one, **opts))
+    for k, (lp, r, x, y) in enumerate(cases):
+        ro, _, _ = ob.solve(lp, max_iter=2000000, **REFINED)
+        opt = highs_objective(lp)
+        print(k, r["objective"], ro["objective"], opt, r["iterations"], r["iterations_fp32"])
+        assert r["status"] == ro["status"] == 0, (k, r)
+        assert r["iterations_fp32"] > 0
+        assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"])), k
+        assert abs(r["objective"] - opt) <= 1e-6 * (1 + abs(opt)), k
+        certify(lp, r, x, y, tol=1e-8, tag=f"case{k}")

@@ -73,10 +73,13 @@ def test_c1_scaling_power_and_halpern_vs_oracle(c1):
     for k in ("values", "at_values", "at_row_starts", "at_col_indices", "row_scale",
               "col_scale", "sigma_base", "tau_base"):
         assert np.array_equal(got[k], want[k]), k
+    # The two global scales are sums over 2e6 / 1e6 terms, reduced in a
+    # fixed tree order on the device and sequentially in the oracle: their
+    # rounding differs by a few ulps times log2 of the length (measured 3e-14).
     for k in ("objective_scale", "constraint_scale"):
-        assert abs(got[k] - want[k]) <= 1e-14 * abs(want[k]), k
+        assert abs(got[k] - want[k]) <= 1e-12 * abs(want[k]), k
     for k in ("cscaled", "col_lower", "col_upper", "row_lower", "row_upper"):
-        np.testing.assert_allclose(got[k], want[k], rtol=1e-14, atol=0)
+        np.testing.assert_allclose(got[k], want[k], rtol=1e-12, atol=0)
     assert got["power_iterations"] == want["power_iterations"] == 30
     print("spectral norm", got["spectral_norm"], want["spectral_norm"], flush=True)
     assert abs(got["spectral_norm"] - want["spectral_norm"]) <= 1e-9 * want["spectral_norm"]
