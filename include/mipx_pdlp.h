@@ -464,6 +464,21 @@ int mpx_kkt_check_device(const mpx_lp* lp, const int32_t* col_starts,
                          const int32_t* row_indices, const double* col_values,
                          const double* x, const double* y, double* out, void* stream);
 
+/*
+ * mpx_kkt_check_device without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The partial sums live in a fixed 1 MiB device array per GPU, held until
+ * the process ends, so the LP may have at most 26,214 blocks of 256 rows
+ * plus columns (about 6.7 million rows and columns together); larger LPs are
+ * refused with MPX_ERR_INVALID before any device work.  Same arguments and
+ * outputs, bit-identical results.  Calls that overlap on one GPU share the
+ * array and race: serialise them (one stream, or an event between calls).
+ */
+int mpx_kkt_check_device_noalloc(const mpx_lp* lp, const int32_t* col_starts,
+                                 const int32_t* row_indices, const double* col_values,
+                                 const double* x, const double* y, double* out, void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -18,7 +18,13 @@
 // order; each block leaves five partial fields in a scratch buffer, and one
 // block combines the partials in block order: the result is deterministic.
 //
-// The partial sums live in a stream-ordered allocation of the call.
+// Two entry points: mpx_kkt_check_device takes its scratch from a
+// stream-ordered allocation, mpx_kkt_check_device_noalloc from a fixed
+// device array (for callers that cannot allocate).  Every partial is
+// written before it is read in the same call, so the fixed array needs no
+// clearing; two calls that overlap on one device race on it.
+#include <mutex>
+#include <string>
 
 #include "../../include/mipx_pdlp.h"
 #include "device_common.h"
@@ -163,6 +169,36 @@ int kktArgs(const mpx_lp* lp, const int32_t* cs, const int32_t* ri, const double
     return MPX_OK;
 }
 
+// ---- allocation-free variant ---------------------------------------------
+// Fixed scratch of the allocation-free entry point: 1 MiB of partial sums per
+// device, held until the process ends; enough for LPs whose row and column
+// blocks of 256 number at most kKktNoAllocDoubles / 5 = 26,214 together
+// (about 6.7 million rows plus columns).  Calls on one device share it.
+constexpr size_t kKktNoAllocDoubles = size_t(1) << 17;
+__device__ __align__(256) double scratch[kKktNoAllocDoubles];
+
+// The device address of `scratch` on each device, looked up once per device
+// with that device current (the name itself is only a host-side stand-in).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+double* noAllocScratch() {
+    int dev = 0;
+    MPX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        setError("mpx_kkt_check_device_noalloc: device ordinal out of range");
+        throw DeviceError();
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        MPX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
 void launchKkt(const KktArgs& a, double* scratch, double* out, hipStream_t st) {
     const int rb = ceilDiv(a.m, kBlock), cb = ceilDiv(a.n, kBlock);
     double* rpart = scratch;
@@ -196,3 +232,25 @@ extern "C" int mpx_kkt_check_device(const mpx_lp* lp, const int32_t* col_starts,
     return MPX_OK;
 }
 
+extern "C" int mpx_kkt_check_device_noalloc(const mpx_lp* lp, const int32_t* col_starts,
+                                            const int32_t* row_indices, const double* col_values,
+                                            const double* x, const double* y, double* out,
+                                            void* stream) {
+    using namespace mpx;
+    KktArgs a;
+    if (const int rc = kktArgs(lp, col_starts, row_indices, col_values, x, y, out, a)) return rc;
+    if (kktScratchDoubles(a.m, a.n) > kKktNoAllocDoubles) {
+        setError("mpx_kkt_check_device_noalloc: the LP needs " +
+                 std::to_string(kktScratchDoubles(a.m, a.n)) +
+                 " doubles of partial sums, more than the fixed scratch of " +
+                 std::to_string(kKktNoAllocDoubles) +
+                 " (at most 26,214 row plus column blocks of 256); use mpx_kkt_check_device");
+        return MPX_ERR_INVALID;
+    }
+    try {
+        launchKkt(a, noAllocScratch(), out, static_cast<hipStream_t>(stream));
+    } catch (const DeviceError&) {
+        return MPX_ERR_DEVICE;
+    }
+    return MPX_OK;
+}

@@ -23,7 +23,7 @@ EXPORTED = [
     "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
     "mpx_pdhg_iterate", "mpx_pdlp_scale_problem_batched", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
     "mpx_batch_results", "mpx_batch_get_stats", "mpx_batch_profile", "mpx_batch_destroy",
-    "mpx_kkt_check_device",
+    "mpx_kkt_check_device", "mpx_kkt_check_device_noalloc",
 ]
 
 
@@ -83,6 +83,7 @@ def lib():
     L.mpx_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.POINTER(MpxBatchStats)]
     L.mpx_batch_destroy.argtypes = [C.c_void_p]
     L.mpx_kkt_check_device.argtypes = [C.POINTER(MpxLp)] + [C.c_void_p] * 7
+    L.mpx_kkt_check_device_noalloc.argtypes = [C.POINTER(MpxLp)] + [C.c_void_p] * 7
     _lib = L
     return L
 
