@@ -68,6 +68,8 @@ def build_host():
     _run(["g++"] + flags + inc + ["-shared", "-o", os.path.join(PKG, "libmipx_host.so"),
                                   os.path.join(src, "pdlp.cpp"), os.path.join(src, "sparse_matrix.cpp"),
                                   os.path.join(src, "mps_io.cpp"),
+                                  os.path.join(src, "presolve.cpp"),
+                                  os.path.join(src, "presolve_c.cpp"),
                                   "-L" + PKG, "-lmipx_pdlp", "-lz", "-Wl,-rpath,$ORIGIN"])
     bind = os.path.join(PKG, "bin")
     os.makedirs(bind, exist_ok=True)

@@ -14,6 +14,7 @@
 //              [--precision fp64|mixed] [--tol T] [--quiet] [--print-backend]
 //              [--write-mps OUT.mps] [--max-iter N]
 //              [--gpu|--no-gpu] [--gpu-min-rows N] [--gpu-min-nnz N]
+//              [--no-presolve]
 //
 // The gate flags keep the reference CLI's defaults (512 rows, 10,000
 // nonzeros); an LP below them reports "PDLP backend: CPU" and status Error,
@@ -29,6 +30,7 @@
 
 #include "mipx/io.h"
 #include "mipx/pdlp.h"
+#include "mipx/presolve.h"
 
 extern "C" {
 struct mpxgen_params {
@@ -120,6 +122,7 @@ int main(int argc, char** argv) {
     int max_iter = 0;
     // The reference CLI's GPU gate flags and defaults (main.cpp:75-80, 284-293).
     bool use_gpu = true;
+    bool presolve = true;  // the reference CLI presolves unless --no-presolve
     int gpu_min_rows = 512, gpu_min_nnz = 10000;
     mipx::PdlpOptimalityNorm norm = mipx::PdlpOptimalityNorm::L2;
     for (int i = 1; i < argc; ++i) {
@@ -150,6 +153,8 @@ int main(int argc, char** argv) {
         else if (a == "--print-backend") print_backend = true;
         else if (a == "--write-mps") write_mps = next();
         else if (a == "--max-iter") max_iter = std::atoi(next().c_str());
+        else if (a == "--no-presolve") presolve = false;
+        else if (a == "--presolve") presolve = true;
         else if (a == "--gpu") use_gpu = true;
         else if (a == "--no-gpu") use_gpu = false;
         else if (a == "--gpu-min-rows") gpu_min_rows = std::max(0, std::atoi(next().c_str()));
@@ -172,6 +177,28 @@ int main(int argc, char** argv) {
         std::printf("LP %s: %d rows, %d cols, %d nonzeros\n", lp.name.c_str(), lp.num_rows,
                     lp.num_cols, lp.matrix.numNonzeros());
     const auto t0 = std::chrono::steady_clock::now();
+    // Presolve, as the reference CLI does before an LP solve
+    // (src/cli/main.cpp:455-467); the reduced LP is used only when something
+    // was removed.
+    mipx::Presolver pre;
+    mipx::LpProblem working;
+    bool did_presolve = false;
+    if (presolve) {
+        working = pre.presolve(lp);
+        if (pre.isInfeasible()) {
+            std::printf("Presolve: infeasible\n");
+            std::printf("PDLP backend: CPU\n");
+            std::printf("Status: Infeasible\nObjective: %.10e\nIterations: 0\nWork units: 0.00\n",
+                        0.0);
+            std::printf("Time: %.2fs\n", std::chrono::duration<double>(
+                                             std::chrono::steady_clock::now() - t0).count());
+            return 0;
+        }
+        did_presolve = pre.stats().vars_removed > 0 || pre.stats().rows_removed > 0;
+        if (verbose || print_backend)
+            std::printf("Presolve: removed %d rows, %d cols\n", pre.stats().rows_removed,
+                        pre.stats().vars_removed);
+    }
     mipx::PdlpSolver solver;
     mipx::PdlpOptions o;
     o.verbose = verbose;
@@ -184,8 +211,15 @@ int main(int argc, char** argv) {
     if (max_iter > 0) o.max_iter = max_iter;
     o.precision = precision == "mixed" ? mipx::PdlpPrecision::Mixed : mipx::PdlpPrecision::FP64;
     solver.setOptions(o);
-    solver.load(lp);
-    const mipx::LpResult r = solver.solve();
+    solver.load(did_presolve ? working : lp);
+    mipx::LpResult r = solver.solve();
+    // Postsolve and the objective on the original columns (main.cpp:560-568).
+    if (did_presolve && r.status == mipx::Status::Optimal) {
+        const std::vector<double> x = pre.postsolve(solver.getPrimalValues());
+        double obj = lp.obj_offset;
+        for (int j = 0; j < lp.num_cols; ++j) obj += lp.obj[j] * x[j];
+        r.objective = obj;
+    }
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::fflush(stdout);

@@ -132,3 +132,99 @@ def farkas_bounds(lp, lo, hi, row):
             lo[j], hi[j] = -np.inf, 0.0
     assert amin > lp.row_lower[row]
     return lo, hi
+
+
+def planted_presolve_lp(seed=0x9E5, m=300, n=750):
+    """A C0-family LP with structure presolve removes: 20 fixed columns
+    (l = u at the generator's feasible point), singleton rows that bind at
+    the optimum (x_j <= 9 or 2 x_j >= 1 on columns the LP pushes to a bound,
+    an equality 3 x_j = 3 x0_j), two empty rows (one ranged around 0, one an
+    equality 0 = 0) and three empty columns (cost +, -, 0).  Returns the LP,
+    the generator's point x0, and the expected counts."""
+    import numpy as np
+    import scipy.sparse as sp
+    from mipx_amd import lpgen
+    from mipx_amd.lp import LpProblem
+    from oracle import binding as ob
+    g = lpgen.c0_plain(seed=seed, m=m, n=n)
+    lp, x0 = g.lp, g.x0
+    r, x, _ = ob.solve(lp, primal_tol=1e-6, dual_tol=1e-6, optimality_tol=1e-6, verbose=0)
+    rng = np.random.default_rng(seed)
+    lo, hi = lp.col_lower.copy(), lp.col_upper.copy()
+    fixed = rng.choice(n, 20, replace=False)
+    lo[fixed] = hi[fixed] = x0[fixed]
+    rest = np.setdiff1d(np.arange(n), fixed)
+    at_hi = [j for j in rest if x[j] > 9.9][:8]
+    at_lo = [j for j in rest if x[j] < 0.1][:8]
+    mid = [j for j in rest if 1.0 < x[j] < 9.0][:1]
+    rows = []   # (col, coeff, rl, ru)
+    rows += [(j, 1.0, -np.inf, 9.0) for j in at_hi]
+    rows += [(j, 2.0, 1.0, np.inf) for j in at_lo]
+    rows += [(j, 3.0, 3.0 * x0[j], 3.0 * x0[j]) for j in mid]
+    a = sp.csr_matrix((lp.values, lp.col_indices, lp.row_starts), shape=(m, n))
+    extra = sp.csr_matrix(([c for _, c, _, _ in rows], ([k for k in range(len(rows))],
+                                                        [j for j, _, _, _ in rows])),
+                          shape=(len(rows), n))
+    empty_rows = sp.csr_matrix((2, n))
+    a2 = sp.vstack([a, extra, empty_rows]).tocsr()
+    a2 = sp.hstack([a2, sp.csr_matrix((a2.shape[0], 3))]).tocsr()
+    a2.sort_indices()
+    rl = np.concatenate([lp.row_lower, [r_[2] for r_ in rows], [-1.0, 0.0]])
+    ru = np.concatenate([lp.row_upper, [r_[3] for r_ in rows], [2.0, 0.0]])
+    obj = np.concatenate([lp.obj, [1.5, -2.0, 0.0]])
+    lo = np.concatenate([lo, [-1.0, 0.0, -3.0]])
+    hi = np.concatenate([hi, [4.0, 5.0, 3.0]])
+    out = LpProblem(a2.shape[0], a2.shape[1], a2.indptr, a2.indices, a2.data, obj, lo, hi, rl, ru,
+                    0, 0.0, name="planted")
+    counts = dict(fixed=20, singleton_rows=len(rows), empty_rows=2, empty_cols=3)
+    return out, np.concatenate([x0, [0.0, 0.0, 0.0]]), counts
+
+
+def write_free_mps(lp, path):
+    """Free-format MPS of an LpProblem (N row COST, then E / L / G / ranged
+    rows, RHS, RANGES, BOUNDS), enough for the CLI's reader."""
+    import numpy as np
+    lines = [f"NAME {lp.name or 'LP'}", "ROWS", " N COST"]
+    kinds = []
+    for i in range(lp.num_rows):
+        l, u = lp.row_lower[i], lp.row_upper[i]
+        k = "E" if l == u else ("L" if np.isinf(l) else ("G" if np.isinf(u) else "R"))
+        kinds.append(k)
+        lines.append(f" {'L' if k == 'R' else k} R{i}")
+    lines.append("COLUMNS")
+    cols = [[] for _ in range(lp.num_cols)]
+    for i in range(lp.num_rows):
+        for q in range(lp.row_starts[i], lp.row_starts[i + 1]):
+            cols[lp.col_indices[q]].append((i, lp.values[q]))
+    for j in range(lp.num_cols):
+        lines.append(f" C{j} COST {float(lp.obj[j])!r}")
+        for i, v in cols[j]:
+            lines.append(f" C{j} R{i} {float(v)!r}")
+    lines.append("RHS")
+    for i, k in enumerate(kinds):
+        rhs = lp.row_lower[i] if k == "G" else lp.row_upper[i]
+        if rhs != 0.0:
+            lines.append(f" RHS R{i} {float(rhs)!r}")
+    if "R" in kinds:
+        lines.append("RANGES")
+        for i, k in enumerate(kinds):
+            if k == "R":
+                lines.append(f" RNG R{i} {float(lp.row_upper[i] - lp.row_lower[i])!r}")
+    lines.append("BOUNDS")
+    for j in range(lp.num_cols):
+        l, u = lp.col_lower[j], lp.col_upper[j]
+        if l == u:
+            lines.append(f" FX BND C{j} {float(l)!r}")
+            continue
+        if np.isinf(l) and np.isinf(u):
+            lines.append(f" FR BND C{j}")
+            continue
+        if np.isinf(l):
+            lines.append(f" MI BND C{j}")
+        elif l != 0.0:
+            lines.append(f" LO BND C{j} {float(l)!r}")
+        if not np.isinf(u):
+            lines.append(f" UP BND C{j} {float(u)!r}")
+    lines.append("ENDATA")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

@@ -65,3 +65,16 @@ def test_server_rejects_invalid_arguments_without_gpu():
     h = C.c_void_p()
     assert lib.mpx_server_create(C.byref(o), 0.0, 0, C.byref(h)) == 4
     assert not h.value
+
+
+def test_presolve_header_exported_by_host_library():
+    """Every function include/mipx_presolve.h declares is exported by
+    libmipx_host.so (the C++ host library)."""
+    from mipx_amd import presolve
+    text = open(os.path.join(ROOT, "include", "mipx_presolve.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(mpx_presolve_[a-z0-9_]+)\s*\(", text)))
+    assert len(names) == 5
+    lib = presolve.lib()
+    for nm in names:
+        assert hasattr(lib, nm), nm

@@ -83,3 +83,55 @@ def test_cli_reads_reference_tiny_mps():
     names = {0: "Optimal", 1: "Infeasible", 2: "Unbounded", 3: "Iteration limit"}
     assert lines["Status"] == names[ro["status"]] == "Iteration limit"
     assert int(lines["Iterations"]) == ro["iterations"] == 20000
+
+
+def test_cli_presolve_default_vs_no_presolve(tmp_path):
+    """The reference CLI presolves by default (src/cli/main.cpp:455-467) and
+    reports the objective of the postsolved columns (560-568).  On an LP with
+    fixed columns, singleton and empty rows and empty columns: with presolve
+    the GPU solve is Optimal and within 1e-6 of the exact optimum once solved
+    to 1e-8; without it the fixed columns stall the reference's dual-residual
+    test (pdlp.cpp:1127-1142) and the oracle stops at the iteration limit
+    just as the device path does."""
+    from lp_cases import highs_objective, planted_presolve_lp, write_free_mps
+    lp, _, _ = planted_presolve_lp()
+    path = str(tmp_path / "planted.mps")
+    write_free_mps(lp, path)
+    base = [os.path.join(BIN, "mipx_solve"), path, "--pdlp", "--print-backend", "--quiet",
+            "--max-iter", "40000"] + OPEN_GATE
+    r1 = subprocess.run(base + ["--tol", "1e-8", "--max-iter", "2000000"], capture_output=True,
+                        text=True, timeout=300)
+    r2 = subprocess.run(base + ["--no-presolve"], capture_output=True, text=True, timeout=300)
+    print(r1.stdout, r2.stdout)
+    a, b = _stable_lines(r1.stdout), _stable_lines(r2.stdout)
+    assert "Presolve: removed 19 rows, 24 cols" in r1.stdout
+    assert a["Status"] == "Optimal" and a["PDLP backend"] == "GPU"
+    opt = highs_objective(lp)
+    assert abs(float(a["Objective"]) - opt) <= 1e-6 * (1 + abs(opt))
+    ro, _, _ = ob.solve(lp, max_iter=40000, primal_tol=1e-6, dual_tol=1e-6, optimality_tol=1e-6,
+                        verbose=0)
+    assert b["Status"] == "Iteration limit" and ro["status"] == 3
+
+
+def test_presolved_gpu_solve_postsolved_and_certified():
+    """The reduced LP solved on the GPU (FP64 and mixed precision), primal
+    and dual postsolved: certified on the original LP, objective within 1e-6
+    of the oracle's solve of the same reduced LP (FP64)."""
+    from conftest import TIGHT
+    from lp_cases import planted_presolve_lp
+    from mipx_amd import native
+    from mipx_amd.presolve import Presolved
+    lp, _, _ = planted_presolve_lp()
+    pre = Presolved(lp)
+    ro, _, _ = ob.solve(pre.reduced, **TIGHT)
+    for prec in (0, 1):
+        r, xr, yr = native.solve(pre.reduced, precision=prec, **TIGHT)
+        assert r["status"] == 0
+        x, y = pre.postsolve(xr, yr)
+        k = ob.kkt_check(lp, x, y)
+        print(prec, r["objective"], ro["objective"], k)
+        assert k["rel_primal"] <= 1.01e-6 and k["rel_dual"] <= 1.01e-6 and k["rel_gap"] <= 1.01e-6
+        assert k["dual_sign_viol"] == 0.0 and k["bound_viol"] == 0.0
+        assert abs(lp.obj @ x - r["objective"]) <= 1e-9 * (1 + abs(r["objective"]))
+        bar = 1e-6 if prec == 0 else 5e-6
+        assert abs(r["objective"] - ro["objective"]) <= bar * (1 + abs(ro["objective"]))

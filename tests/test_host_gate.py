@@ -35,3 +35,23 @@ def test_cli_below_gate_reports_cpu_backend_and_error():
         assert lines["PDLP backend"] == "CPU" and lines["Status"] == "Error"
         assert "GPU gate" in lines["Error detail"]
         assert float(lines["Objective"]) == 0.0 and int(lines["Iterations"]) == 0
+
+
+def test_cli_presolve_proves_infeasibility_without_device(tmp_path):
+    """An LP that presolve proves infeasible (a singleton row against a
+    column bound) is reported Infeasible before any device work."""
+    import numpy as np
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from lp_cases import write_free_mps
+    from mipx_amd.lp import from_dense
+    lp = from_dense(np.array([[1.0, 0.0], [1.0, 1.0]]), [1.0, 1.0], [0.0, 0.0], [5.0, 5.0],
+                    [6.0, -np.inf], [np.inf, 4.0])
+    path = str(tmp_path / "inf.mps")
+    write_free_mps(lp, path)
+    r = subprocess.run([os.path.join(BIN, "mipx_solve"), path, "--pdlp", "--quiet"],
+                       capture_output=True, text=True, timeout=60)
+    print(r.stdout, r.stderr)
+    lines = _lines(r.stdout)
+    assert r.returncode == 0 and lines["Status"] == "Infeasible"
+    assert "Presolve: infeasible" in r.stdout
