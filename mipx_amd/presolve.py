@@ -25,7 +25,7 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        L = load_library("libmipx_host.so")
+        L = load_library("libmipx_host.so", local=True)
         L.mpx_presolve_run.argtypes = [C.POINTER(MpxLp), C.POINTER(C.c_void_p)]
         L.mpx_presolve_reduced.argtypes = [C.c_void_p, C.POINTER(MpxLp)]
         L.mpx_presolve_get_stats.argtypes = [C.c_void_p, C.POINTER(MpxPresolveStats)]
