@@ -146,6 +146,14 @@ typedef struct mpx_pdlp_options {
                                          chunk); 0 = automatic (chunks of
                                          2^19 columns when n > 2^20), < 0 =
                                          never */
+    int32_t packed_preset;            /* batched / served LPs in the packed
+                                         kernels: lanes per column and per row
+                                         (preset 0..7: (1,1) (2,2) (2,4) (4,4)
+                                         (4,8) (8,8) (8,16) (16,16)); -1 = the
+                                         autotuned table for the LP's shape
+                                         (mean entries per column / row), the
+                                         nearest-shape heuristic where the
+                                         table has no entry */
 } mpx_pdlp_options;
 
 /* Mirrors `mipx::LpResult` (include/mipx/lp_solver.h:11-16) plus the final

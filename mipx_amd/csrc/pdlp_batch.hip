@@ -45,13 +45,14 @@
 #include "batch_interleaved.h"
 #include "batch_kernels.h"
 #include "batch_setup.h"
+#include "packed_presets.h"
 #include "device_common.h"
 
 namespace mpx {
 
 
 // Heuristic preset from mean entries per column / row (about two entries per
-// lane); the measured autotuner (mpx_batch_autotune) can override it.
+// lane), for LP shapes the autotuned table (packed_presets.h) has no entry for.
 int heuristicPreset(long long nnz, int m, int n) {
     const double ac = n ? static_cast<double>(nnz) / n : 1.0;
     const double ar = m ? static_cast<double>(nnz) / m : 1.0;
@@ -1017,10 +1018,17 @@ OwnerHost ownerHostData(const mpx_lp& lp) {
     return h;
 }
 
-// Lane-group preset of a matrix: the heuristic, or the one with
-// `spmv_group` lanes per column when the caller forces a group size.
+// Lane-group preset of a matrix: the caller's `packed_preset`, the one with
+// `spmv_group` lanes per column when the caller forces a group size, the
+// autotuned table entry for the matrix's shape, or the heuristic.  The
+// choice depends on the matrix alone (not on timing or on the rest of a
+// batch), so an instance's arithmetic is the same whatever it runs with.
 int ownerPreset(const mpx_pdlp_options& o, int nnz, int m, int n) {
-    if (o.spmv_group <= 0) return heuristicPreset(nnz, m, n);
+    if (o.packed_preset >= 0 && o.packed_preset < kNumPresets) return o.packed_preset;
+    if (o.spmv_group <= 0) {
+        const int tuned = tunedPreset(nnz, m, n);
+        return tuned >= 0 ? tuned : heuristicPreset(nnz, m, n);
+    }
     if (o.spmv_group == 1) return 0;
     int best = 0;
     for (int p = 0; p < kNumPresets; ++p)

@@ -1572,6 +1572,7 @@ void mpx_pdlp_default_options(mpx_pdlp_options* o) {
     o->device = 0;
     o->mixed_handover = 3.0;
     o->spmv_chunk_cols = 0;
+    o->packed_preset = -1;
 }
 
 const char* mpx_last_error(void) { return mpx::lastError(); }

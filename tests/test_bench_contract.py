@@ -71,3 +71,22 @@ def test_cpu_baseline_runs_the_restatement_only():
     src = open(os.path.join(ROOT, "bench.py")).read()
     assert "ref_solve" not in src and "ref_available" not in src
     assert '"reference"' not in src
+
+
+def test_autotuner_training_shapes_and_table():
+    """The autotuner's training LPs fall in the bucket they tune, and the
+    compiled table holds its measured choices (the seeded heterogeneous
+    family's bucket (3, 4) included)."""
+    import re
+    from mipx_amd import autotune
+    import numpy as np
+    for bc, br in ((2, 4), (3, 4), (3, 5)):
+        lps = autotune.training_batch(bc, br, 3)
+        for lp in lps:
+            assert round(np.log2(lp.nnz / lp.num_cols)) == bc
+            assert round(np.log2(lp.nnz / lp.num_rows)) == br
+    text = open(autotune.TABLE).read()
+    rows = re.findall(r"\{([-\d, ]+)\},", text)
+    table = [[int(v) for v in r.split(",")] for r in rows]
+    assert len(table) == autotune.COL_BUCKETS and all(len(r) == autotune.ROW_BUCKETS for r in table)
+    assert table[3][4] in range(8) and table[2][4] in range(8)

@@ -243,3 +243,26 @@ def test_mixed_refined_to_1e8_matches_oracle():
         assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"])), k
         assert abs(r["objective"] - opt) <= 1e-6 * (1 + abs(opt)), k
         certify(lp, r, x, y, tol=1e-8, tag=f"case{k}")
+
+
+def test_packed_preset_from_autotuned_table():
+    """Packed LPs take their lanes per column / row from the autotuned table
+    (mipx_amd/csrc/packed_presets.h) by shape alone: the default solve is
+    bit-identical to forcing that table entry, alone and in a mix, and
+    differs from another preset in the last bits only (same status, the
+    objective to 1e-6)."""
+    from mipx_amd.autotune import training_batch
+    lp34 = lpgen.c3_instance(5, 300).lp         # bucket (3, 4): preset 4
+    lp35 = training_batch(3, 5, 1)[0]            # bucket (3, 5): preset 6
+    for lp, tuned in ((lp34, 4), (lp35, 6)):
+        d, xd, yd = native.solve_batch([lp], precision=1, max_iter=200000, **TIGHT)
+        f, xf, yf = native.solve_batch([lp], precision=1, max_iter=200000, packed_preset=tuned,
+                                       **TIGHT)
+        o, _, _ = native.solve_batch([lp], precision=1, max_iter=200000, packed_preset=0, **TIGHT)
+        assert d[0]["status"] == f[0]["status"] == o[0]["status"] == 0
+        assert (d[0]["iterations"], d[0]["objective"]) == (f[0]["iterations"], f[0]["objective"])
+        assert np.array_equal(xd[0], xf[0]) and np.array_equal(yd[0], yf[0])
+        assert abs(o[0]["objective"] - d[0]["objective"]) <= 1e-5 * (1 + abs(d[0]["objective"]))
+    mix, xm, _ = native.solve_batch([lp35, lp34], precision=1, max_iter=200000, **TIGHT)
+    alone, xa, _ = native.solve_batch([lp34], precision=1, max_iter=200000, **TIGHT)
+    assert mix[1]["objective"] == alone[0]["objective"] and np.array_equal(xm[1], xa[0])
