@@ -440,6 +440,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--handover", type=float, default=0.0,
                     help="mixed precision: FP32 -> FP64 handover KKT ratio (0: library default)")
+    ap.add_argument("--refine", type=int, default=1, choices=[0, 1],
+                    help="mixed precision: FP32 refinement around an FP64 base after the plain "
+                         "FP32 phase (1, default) or FP64 iterations (0)")
     ap.add_argument("--spmv-group", type=int, default=0,
                     help="lanes per column / row of the packed kernels (0: per-LP choice)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -476,6 +479,7 @@ def main():
         cap["mixed_handover"] = a.handover
     if a.spmv_group > 0:
         cap["spmv_group"] = a.spmv_group
+    cap["mixed_refine"] = a.refine
     if a.workload == "c3":
         lps = c3_workload(a.count)
         lp = lps[0]

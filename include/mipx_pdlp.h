@@ -125,8 +125,8 @@ typedef struct mpx_pdlp_options {
     /* ---- MI355X extensions ---- */
     int32_t precision;                /* MPX_PREC_FP64 */
     double refine_tol;                /* mixed precision: after the handover
-                                         (see mixed_handover) FP64 iterations
-                                         refine until primal, dual and gap are
+                                         (see mixed_handover) the refinement
+                                         runs until primal, dual and gap are
                                          below refine_tol (<= 0: the solve's
                                          own tolerances) */
     int32_t spmv_group;               /* lanes per row/column in the SpMV
@@ -154,6 +154,14 @@ typedef struct mpx_pdlp_options {
                                          (mean entries per column / row), the
                                          nearest-shape heuristic where the
                                          table has no entry */
+    int32_t mixed_refine;             /* mixed precision: after the plain FP32
+                                         phase (see mixed_handover), FP32
+                                         iterations continue on the
+                                         correction around an FP64 base point
+                                         that is re-centred at every restart
+                                         (FP64 iterative refinement) until
+                                         refine_tol; 0 = FP64 iterations
+                                         instead (default 1) */
 } mpx_pdlp_options;
 
 /* Mirrors `mipx::LpResult` (include/mipx/lp_solver.h:11-16) plus the final

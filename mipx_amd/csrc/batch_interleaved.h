@@ -22,6 +22,7 @@
 #pragma once
 
 #include "batch_kernels.h"
+#include "pdlp_refine.h"
 
 namespace mpx {
 
@@ -39,65 +40,57 @@ struct IlBufs {
       *y0 = nullptr;
 };
 
-// Slot lists: slots[p][s] = instance id for precision p (0 = FP32, 1 = FP64),
-// counts[p] = number of slots, lanes[p][s] = its scalars.  Only instances
-// whose `owner` equals `owner` take part (all of them when owner < 0).  One
-// block, in instance order (deterministic).
+// Slot lists: slots[p][s] = instance id for list p (0 = plain FP32, 1 =
+// FP64, 2 = FP32 refinement), counts[p] = number of slots, lanes[p][s] =
+// its scalars.  Only instances whose `owner` equals `owner` take part (all
+// of them when owner < 0).  One block, in instance order (deterministic).
 __global__ __launch_bounds__(1024) void kSlotLists(const BInst* __restrict__ insts, int K,
                                                    int owner, int* __restrict__ slots32,
                                                    int* __restrict__ slots64,
+                                                   int* __restrict__ slotsR,
                                                    ILane* __restrict__ lanes32,
                                                    ILane* __restrict__ lanes64,
+                                                   ILane* __restrict__ lanesR,
                                                    int* __restrict__ counts) {
-    __shared__ int part[2][1024];
-    __shared__ int carry[2];
-    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __shared__ int part[3][1024];
+    __shared__ int carry[3];
+    if (threadIdx.x == 0) carry[0] = carry[1] = carry[2] = 0;
     __syncthreads();
     for (int base = 0; base < K; base += 1024) {
         const int k = base + threadIdx.x;
-        int p32 = 0, p64 = 0;
+        int pl[3] = {0, 0, 0};
         if (k < K) {
             const BCtrl& c = *insts[k].ctrl;
             const bool mine = owner < 0 || insts[k].owner == owner;
             const bool act = mine && c.phase != kPhaseDone && c.round_iters > 0;
-            p32 = act && c.phase == kPhaseFp32;
-            p64 = act && c.phase != kPhaseFp32;
+            pl[0] = act && c.phase == kPhaseFp32;
+            pl[2] = act && c.phase == kPhaseRefine;
+            pl[1] = act && !iteratesFp32(c.phase);
         }
-        part[0][threadIdx.x] = p32;
-        part[1][threadIdx.x] = p64;
+        for (int p = 0; p < 3; ++p) part[p][threadIdx.x] = pl[p];
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
-            const int a0 = threadIdx.x >= off ? part[0][threadIdx.x - off] : 0;
-            const int a1 = threadIdx.x >= off ? part[1][threadIdx.x - off] : 0;
+            int a[3];
+            for (int p = 0; p < 3; ++p) a[p] = threadIdx.x >= off ? part[p][threadIdx.x - off] : 0;
             __syncthreads();
-            part[0][threadIdx.x] += a0;
-            part[1][threadIdx.x] += a1;
+            for (int p = 0; p < 3; ++p) part[p][threadIdx.x] += a[p];
             __syncthreads();
         }
-        if (k < K && (p32 || p64)) {
+        if (k < K && (pl[0] || pl[1] || pl[2])) {
             const BCtrl& c = *insts[k].ctrl;
             const ILane ln{c.step, c.pw, c.inner, c.round_iters};
-            if (p32) {
-                const int s = carry[0] + part[0][threadIdx.x] - 1;
-                slots32[s] = k;
-                lanes32[s] = ln;
-            } else {
-                const int s = carry[1] + part[1][threadIdx.x] - 1;
-                slots64[s] = k;
-                lanes64[s] = ln;
-            }
+            const int p = pl[0] ? 0 : (pl[1] ? 1 : 2);
+            const int s = carry[p] + part[p][threadIdx.x] - 1;
+            (p == 0 ? slots32 : p == 1 ? slots64 : slotsR)[s] = k;
+            (p == 0 ? lanes32 : p == 1 ? lanes64 : lanesR)[s] = ln;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            carry[0] += part[0][1023];
-            carry[1] += part[1][1023];
-        }
+        if (threadIdx.x == 0)
+            for (int p = 0; p < 3; ++p) carry[p] += part[p][1023];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        counts[0] = carry[0];
-        counts[1] = carry[1];
-    }
+    if (threadIdx.x == 0)
+        for (int p = 0; p < 3; ++p) counts[p] = carry[p];
 }
 
 // Work items of a group pass: (group, block of 4 columns or rows).  Fixed
@@ -149,7 +142,8 @@ __global__ __launch_bounds__(kBlock) void kInterleave(
     __shared__ unsigned long long own;
     const int nslots = *count;
     const int per_group = (len + kLanes - 1) / kLanes;
-    const int nsrc = which == 0 ? 4 : 2;
+    // which: 0 x, x0, l, u; 1 y, y0; 2 the refinement's c'; 3 its rl', ru', q.
+    const int nsrc = which == 0 ? 4 : which == 1 ? 2 : which == 2 ? 1 : 3;
     const bool mark = which == 0 && flags != nullptr;
     forGroupItems(nslots, per_group, [&](int g, int tb) {
         const int e0 = tb * kLanes;
@@ -162,9 +156,15 @@ __global__ __launch_bounds__(kBlock) void kInterleave(
                 const int e = e0 + static_cast<int>(threadIdx.x % kLanes);
                 T v = T(0);
                 if (s < nslots && e < len) {
-                    const BVecs<T>& vv = vecsOf<T>(insts[slots[s]]);
-                    const T* src = which == 0 ? (a == 0 ? vv.x : a == 1 ? vv.x0 : a == 2 ? vv.l : vv.u)
-                                              : (a == 0 ? vv.y : vv.y0);
+                    const BInst& in = insts[slots[s]];
+                    const BVecs<T>& vv = vecsOf<T>(in);
+                    const T* src = nullptr;
+                    if (which == 0) src = a == 0 ? vv.x : a == 1 ? vv.x0 : a == 2 ? vv.l : vv.u;
+                    else if (which == 1) src = a == 0 ? vv.y : vv.y0;
+                    if constexpr (sizeof(T) == 4) {
+                        if (which == 2) src = in.rc;
+                        else if (which == 3) src = a == 0 ? in.rrl : a == 1 ? in.rru : in.rq;
+                    }
                     v = src[e];
                 }
                 tile[r][threadIdx.x % kLanes] = v;
@@ -356,7 +356,9 @@ __device__ inline void laneScalars(const ILane& ln, int kiter, int& lim, T& lam,
 // Primal half step: each wave handles kCW consecutive columns for the 64
 // instances of its group (4 waves per block).  Shared data: CSC structure and
 // values, c, tau_base.
-template <typename T, int kCW>
+// REFINE: the refinement's correction phase, whose objective c' is per lane
+// (interleaved IC) instead of the matrix's shared c.
+template <typename T, int kCW, bool REFINE = false>
 __global__ __launch_bounds__(kBlock) void kILPrimal(
     const BInst* __restrict__ insts, const int* __restrict__ slots, const int* __restrict__ count,
     const ILane* __restrict__ lanes, int n, int m, const int* __restrict__ cs,
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
     const T* __restrict__ tau_base, const T* __restrict__ IY, T* __restrict__ IX,
     const T* __restrict__ IX0, T* __restrict__ IXR, const T* __restrict__ IL,
     const T* __restrict__ IU, const T* __restrict__ BL, const T* __restrict__ BU,
-    const unsigned long long* __restrict__ bflags, int kiter) {
+    const unsigned long long* __restrict__ bflags, int kiter, const T* __restrict__ IC = nullptr) {
     const int nslots = *count;
     const int per_group = (n + 4 * kCW - 1) / (4 * kCW);
     const int ntiles = (n + kLanes - 1) / kLanes;
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
         const __amdgpu_buffer_rsrc_t ry = groupRsrc(IY + static_cast<size_t>(g) * m * kLanes, m);
         const int voff = lane * static_cast<int>(sizeof(T));
         const size_t base = (static_cast<size_t>(g) * n + j0) * kLanes + lane;
-        T xv[kCW], x0v[kCW], lv[kCW], uv[kCW], acc[kCW];
+        T xv[kCW], x0v[kCW], lv[kCW], uv[kCW], acc[kCW], cv[kCW];
         int qb[kCW], qe[kCW];
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
@@ -401,6 +403,7 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
             const size_t o = base + static_cast<size_t>(cc) * kLanes;
             xv[cc] = in ? ldNT(IX + o) : T(0);
             x0v[cc] = in ? ldNT(IX0 + o) : T(0);
+            if constexpr (REFINE) cv[cc] = in ? ldNT(IC + o) : T(0);
         }
         // FP64: the kCW columns lie in one 64-column tile (kCW divides 64); a
         // column whose bounds match the owner's in every lane reads the
@@ -435,7 +438,10 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
             if (j >= n) break;
             const size_t o = base + static_cast<size_t>(cc) * kLanes;
             const T tau = step * tau_base[j] / pw;
-            const T t = xv[cc] - tau * (c[j] + acc[cc]);
+            T cj;
+            if constexpr (REFINE) cj = cv[cc];
+            else cj = c[j];
+            const T t = xv[cc] - tau * (cj + acc[cc]);
             const T h = clampRef(t, lv[cc], uv[cc]);
             const T r = T(2) * h - xv[cc];
             IXR[o] = r;  // gathered by the dual step: keep it cacheable
@@ -445,13 +451,17 @@ __global__ __launch_bounds__(kBlock) void kILPrimal(
     });
 }
 
-template <typename T, int kCW>
+// REFINE: per-lane row data of the refinement (interleaved IRL, IRU, IQ)
+// and the shifted prox of pdlp_refine.h.
+template <typename T, int kCW, bool REFINE = false>
 __global__ __launch_bounds__(kBlock) void kILDual(
     const BInst* __restrict__ insts, const int* __restrict__ slots, const int* __restrict__ count,
     const ILane* __restrict__ lanes, int n, int m, const int* __restrict__ rs,
     const int* __restrict__ ci, const T* __restrict__ val, const T* __restrict__ sigma_base,
     const T* __restrict__ rl, const T* __restrict__ ru, const T* __restrict__ IXR,
-    T* __restrict__ IY, const T* __restrict__ IY0, int kiter) {
+    T* __restrict__ IY, const T* __restrict__ IY0, int kiter,
+    const T* __restrict__ IRL = nullptr, const T* __restrict__ IRU = nullptr,
+    const T* __restrict__ IQ = nullptr) {
     const int nslots = *count;
     const int per_group = (m + 4 * kCW - 1) / (4 * kCW);
     int curg = -1, lim = 0;
@@ -472,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void kILDual(
         const __amdgpu_buffer_rsrc_t rx = groupRsrc(IXR + static_cast<size_t>(g) * n * kLanes, n);
         const int voff = lane * static_cast<int>(sizeof(T));
         const size_t base = (static_cast<size_t>(g) * m + i0) * kLanes + lane;
-        T yv[kCW], y0v[kCW], acc[kCW];
+        T yv[kCW], y0v[kCW], acc[kCW], rlv[kCW], ruv[kCW], qv[kCW];
         int qb[kCW], qe[kCW];
 #pragma unroll
         for (int cc = 0; cc < kCW; ++cc) {
@@ -480,6 +490,11 @@ __global__ __launch_bounds__(kBlock) void kILDual(
             const size_t o = base + static_cast<size_t>(cc) * kLanes;
             yv[cc] = in ? IY[o] : T(0);  // y is gathered by the next primal step
             y0v[cc] = in ? ldNT(IY0 + o) : T(0);
+            if constexpr (REFINE) {
+                rlv[cc] = in ? ldNT(IRL + o) : T(0);
+                ruv[cc] = in ? ldNT(IRU + o) : T(0);
+                qv[cc] = in ? ldNT(IQ + o) : T(0);
+            }
             qb[cc] = in ? rs[i0 + cc] : 0;
             qe[cc] = in ? rs[i0 + cc + 1] : 0;
             acc[cc] = T(0);
@@ -494,7 +509,9 @@ __global__ __launch_bounds__(kBlock) void kILDual(
             const size_t o = base + static_cast<size_t>(cc) * kLanes;
             const T sig = step * sigma_base[i] * pw;
             const T v = yv[cc] + sig * acc[cc];
-            const T h = dualProx(v, sig, rl[i], ru[i]);
+            T h;
+            if constexpr (REFINE) h = dualProxShifted(v, sig, rlv[cc], ruv[cc], qv[cc]);
+            else h = dualProx(v, sig, rl[i], ru[i]);
             const T r = T(2) * h - yv[cc];
             IY[o] = lam * r + (T(1) - lam) * y0v[cc];
             if (last) yh[i] = h;

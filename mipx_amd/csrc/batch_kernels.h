@@ -46,18 +46,26 @@ struct BCtrl {
     int32_t inner, total, pcount, phase, status, restarts, fp32_iters, stall;
     int32_t power_iters, g_row, g_col, use_lds;
     int32_t round_iters, skip_eval;
+    int32_t refined, pad2;        // 1 once the instance entered kPhaseRefine
     unsigned long long t_finish;  // device wall clock when the instance finished
 };
 
-enum BPhase { kPhaseFp32 = 0, kPhaseFp64 = 1, kPhasePolish = 2, kPhaseDone = 3 };
+// kPhaseRefine: FP32 iterations on the correction around an FP64 base point
+// (iterative refinement, pdlp_refine.h), evaluated in FP64.
+enum BPhase { kPhaseFp32 = 0, kPhaseFp64 = 1, kPhasePolish = 2, kPhaseDone = 3, kPhaseRefine = 4 };
+
+// Phases whose iterations run in the FP32 kernels.
+__host__ __device__ inline bool iteratesFp32(int phase) {
+    return phase == kPhaseFp32 || phase == kPhaseRefine;
+}
 
 struct BOptions {
     int32_t max_iter, N, polish_max_iter, norm_inf, update_pw, enable_polish, precond_refresh,
         do_pc;
     double primal_tol, dual_tol, opt_tol, polish_p_tol, polish_d_tol;
     double refine_p_tol, refine_d_tol, refine_g_tol;  // FP64 phase of mixed precision
-    int32_t mixed, pad;
-    double handover;  // FP32 -> FP64 at this KKT ratio
+    int32_t mixed, refine;  // refine: the FP32 phase hands over to kPhaseRefine
+    double handover;  // FP32 -> refinement (or FP64) at this KKT ratio
     double pid_kp, pid_ki, pid_kd, pid_i_smooth;
     double rs_suff, rs_nec, rs_art, precond_ratio;
 };
@@ -66,6 +74,7 @@ template <typename T>
 struct BVecs {
     T *x, *x0, *xr, *xh, *y, *y0, *yh;
     const T *val, *valt, *c, *tau, *l, *u, *sig, *rl, *ru;
+    const T* q;  // refinement row shift (FP32 correction phase), else null
 };
 
 struct BInst {
@@ -80,6 +89,10 @@ struct BInst {
     BVecs<float> f;
     BVecs<double> d;
     BCtrl* ctrl;
+    // Correction data of the refinement phase (mixed precision): c', rl',
+    // ru', q; l' and u' overwrite the instance's FP32 bounds.  At the first
+    // re-centring f.c / f.rl / f.ru / f.q are pointed at them.
+    float *rc, *rrl, *rru, *rq;
 };
 
 // A distinct constraint matrix of a batch (an "owner"): its original data,
@@ -390,6 +403,62 @@ __device__ inline int bCertificates(const BInst& in, const BVecs<T>& v, const BC
     const double sg = sense_max ? -1.0 : 1.0;
     if (sbad == 0.0 && sg * scr < -tol) return 2;
     return 0;
+}
+
+// Re-centring of an instance's refinement (one workgroup): the FP64 base
+// (x_b in d.x, y_b in d.y) is already in place; forms c' = c + A'y_b,
+// l' = l - x_b, u' = u - x_b, rl' = rl - A x_b, ru' = ru - A x_b and q = y_b
+// in FP64 (sequential sums per column / row) rounded to FP32, points the
+// FP32 iteration vectors at them and zeroes the correction iterates.
+__device__ inline void refineRecenter(const BInst& in) {
+    BInst& w = const_cast<BInst&>(in);
+    const int tid = threadIdx.x;
+    float* lf = const_cast<float*>(in.f.l);
+    float* uf = const_cast<float*>(in.f.u);
+    for (int j = tid; j < in.n; j += kBlock) {
+        double g = 0.0;
+        for (int k = in.cs[j]; k < in.cs[j + 1]; ++k) g += in.valt[k] * in.d.y[in.ri[k]];
+        const double xb = in.d.x[j];
+        in.rc[j] = static_cast<float>(in.c[j] + g);
+        lf[j] = static_cast<float>(in.l[j] - xb);
+        uf[j] = static_cast<float>(in.u[j] - xb);
+        in.f.x[j] = in.f.x0[j] = in.f.xh[j] = 0.0f;
+    }
+    for (int i = tid; i < in.m; i += kBlock) {
+        double r = 0.0;
+        for (int k = in.rs[i]; k < in.rs[i + 1]; ++k) r += in.val[k] * in.d.x[in.ci[k]];
+        in.rrl[i] = static_cast<float>(in.rl[i] - r);
+        in.rru[i] = static_cast<float>(in.ru[i] - r);
+        in.rq[i] = static_cast<float>(in.d.y[i]);
+        in.f.y[i] = in.f.y0[i] = in.f.yh[i] = 0.0f;
+    }
+    if (tid == 0) {
+        w.f.c = in.rc;
+        w.f.rl = in.rrl;
+        w.f.ru = in.rru;
+        w.f.q = in.rq;
+    }
+    __syncthreads();
+}
+
+// The FP64 evaluation point of a refining instance (one workgroup), as
+// kRefineMaterialize of the single-LP path: d.xh / x0 / yh / y0 from the
+// base and the FP32 correction, snapped to the FP64 bounds and to exact
+// zero duals where the iteration put them.
+__device__ inline void refineMaterialize(const BInst& in) {
+    const int tid = threadIdx.x;
+    for (int j = tid; j < in.n; j += kBlock) {
+        const float d = in.f.xh[j];
+        const double xb = in.d.x[j];
+        in.d.xh[j] = d == in.f.l[j] ? in.l[j] : (d == in.f.u[j] ? in.u[j] : xb + static_cast<double>(d));
+        in.d.x0[j] = xb + static_cast<double>(in.f.x0[j]);
+    }
+    for (int i = tid; i < in.m; i += kBlock) {
+        const float d = in.f.yh[i];
+        const bool inactive = in.rrl[i] != in.rru[i] && d == -in.rq[i];
+        in.d.yh[i] = inactive ? 0.0 : in.d.y[i] + static_cast<double>(d);
+        in.d.y0[i] = in.d.y[i] + static_cast<double>(in.f.y0[i]);
+    }
 }
 
 }  // namespace mpx

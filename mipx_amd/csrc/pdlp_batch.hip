@@ -104,13 +104,17 @@ __device__ inline void updatePrimalWeight(BCtrl& c, const BEval& e, const BOptio
 
 // Decision codes written by thread 0 for the workgroup.
 enum BAction { kActNone = 0, kActRestart = 1, kActToFp64 = 2, kActConverged = 3, kActFinish = 4,
-               kActCert = 5 };
+               kActCert = 5, kActToRefine = 6, kActRefineToFp64 = 7 };
 
 // Mixed precision: FP32 hands over to FP64 once the FP64-judged KKT ratio
 // (max of residual / tolerance) is within this factor, or after this many
 // evaluations without halving it.
 constexpr double kFp32ExitRatio = 3.0;
 constexpr int kFp32Stall = 20;
+// A refining instance that goes this many evaluations without halving its
+// KKT ratio continues in FP64 (a precision stall of the FP32 matrix; the
+// seeded families reach 1e-8 in the correction phase without one).
+constexpr int kRefineStall = 10 * kFp32Stall;
 
 template <typename T>
 __device__ inline const BVecs<T>& pickVecs(const BInst& in);
@@ -167,6 +171,7 @@ struct PRec {
     const int *cs, *ri, *rs, *ci;
     const T *valt, *val, *c, *tau, *l, *u, *x0, *sig, *rl, *ru, *y0;
     T *x, *xr, *xh, *y, *yh;
+    const T* q;  // refinement row shift, or null
     T step, pw;
     int32_t inner, lim, n, m;
 };
@@ -236,13 +241,13 @@ __device__ inline void dualTile(const PRec<T>& rec, int first, int kiter) {
     struct {
         decltype(asGlobal(rec.rs)) rs;
         decltype(asGlobal(rec.ci)) ci;
-        decltype(asGlobal(rec.val)) val, sig, rl, ru, y0;
+        decltype(asGlobal(rec.val)) val, sig, rl, ru, y0, q;
         decltype(asGlobal(rec.xr)) xr, y, yh;
         T step, pw;
         int32_t inner, lim, m;
     } r{asGlobal(rec.rs), asGlobal(rec.ci), asGlobal(rec.val), asGlobal(rec.sig), asGlobal(rec.rl),
-        asGlobal(rec.ru), asGlobal(rec.y0), asGlobal(rec.xr), asGlobal(rec.y), asGlobal(rec.yh),
-        rec.step, rec.pw, rec.inner, rec.lim, rec.m};
+        asGlobal(rec.ru), asGlobal(rec.y0), asGlobal(rec.q), asGlobal(rec.xr), asGlobal(rec.y),
+        asGlobal(rec.yh), rec.step, rec.pw, rec.inner, rec.lim, rec.m};
     constexpr int per = kBlock / G;
     const int lane = static_cast<int>(threadIdx.x) % G, grp = static_cast<int>(threadIdx.x) / G;
     int i[kPackCW], b[kPackCW], cnt[kPackCW];
@@ -276,7 +281,8 @@ __device__ inline void dualTile(const PRec<T>& rec, int first, int kiter) {
         const T sig = r.step * r.sig[ii] * r.pw;
         const T yi = r.y[ii];
         const T vv = yi + sig * axr;
-        const T h = dualProx(vv, sig, r.rl[ii], r.ru[ii]);
+        const T h = r.q ? dualProxShifted(vv, sig, r.rl[ii], r.ru[ii], r.q[ii])
+                        : dualProx(vv, sig, r.rl[ii], r.ru[ii]);
         const T rr = T(2) * h - yi;
         if (kiter == r.lim) r.yh[ii] = h;
         r.y[ii] = lam * rr + (T(1) - lam) * r.y0[ii];
@@ -334,7 +340,7 @@ __global__ void kTileCounts(const BInst* __restrict__ insts, int K, int* __restr
     const int cpt = tileCols(in.preset), rpt = tileRows(in.preset);
     const BCtrl& c = *in.ctrl;
     const bool act = !in.il && c.phase != kPhaseDone && c.round_iters > 0;
-    const bool p32 = act && c.phase == kPhaseFp32, p64 = act && c.phase != kPhaseFp32;
+    const bool p32 = act && iteratesFp32(c.phase), p64 = act && !iteratesFp32(c.phase);
     const int ct = (in.n + cpt - 1) / cpt, rt = (in.m + rpt - 1) / rpt;
     cnt[0 * K + k] = p32 ? ct : 0;
     cnt[1 * K + k] = p32 ? rt : 0;
@@ -407,6 +413,7 @@ __device__ inline void fillRec(const BInst& in, const BCtrl& c, PRec<T>& r) {
     r.xh = v.xh;
     r.y = v.y;
     r.yh = v.yh;
+    r.q = v.q;
     r.step = static_cast<T>(c.step);
     r.pw = static_cast<T>(c.pw);
     r.inner = c.inner;
@@ -427,7 +434,7 @@ __global__ void kTileFill(const BInst* __restrict__ insts, int K, const int* __r
     const BCtrl& c = *in.ctrl;
     const bool act = !in.il && c.phase != kPhaseDone && c.round_iters > 0;
     if (!act) return;
-    const bool p32 = c.phase == kPhaseFp32;
+    const bool p32 = iteratesFp32(c.phase);
     if (p32) fillRec<float>(in, c, rec32[k]);
     else fillRec<double>(in, c, rec64[k]);
     Tile* cl = p32 ? c32 : c64;
@@ -468,6 +475,7 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         if (s_skip) return;
     }
     const bool polish = s_phase == kPhasePolish;
+    const bool refining = s_phase == kPhaseRefine;
     const int ran = s_ran;
     // The sums of the instance's evaluation parts (kEvalParts, same round),
     // combined in part order.
@@ -490,7 +498,7 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         c.inner += ran;
         c.total += ran;
         if (polish) c.pcount += ran;
-        if (kFp32) c.fp32_iters += ran;
+        if (kFp32 || refining) c.fp32_iters += ran;
         if (c.fpe0 < 0.0) c.fpe0 = e.fpe;
         c.rel_p = e.rel_p;
         c.rel_d = e.rel_d;
@@ -532,11 +540,23 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
             } else if (++c.stall >= kFp32Stall) {
                 handover = true;
             }
-            if (handover) act = kActToFp64;
+            if (handover) {
+                act = o.refine ? kActToRefine : kActToFp64;
+                c.best_kkt = k;
+                c.stall = 0;
+            }
         } else if (tp <= (o.mixed ? o.refine_p_tol : o.primal_tol) &&
                    td <= (o.mixed ? o.refine_d_tol : o.dual_tol) &&
                    e.gap <= (o.mixed ? o.refine_g_tol : o.opt_tol)) {
             act = kActConverged;
+        } else if (refining) {
+            const double k = kktRatio(e, o);
+            if (k < 0.5 * c.best_kkt) {
+                c.best_kkt = k;
+                c.stall = 0;
+            } else if (++c.stall >= kRefineStall) {
+                act = kActRefineToFp64;
+            }
         }
         if (act == -1 && c.total >= 10 * o.N && c.total % (10 * o.N) == 0) act = kActCert;
         s_act = act;
@@ -632,6 +652,43 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
             c.fpe0 = -1.0;
             c.last_fpe = INFINITY;
             ++c.restarts;
+        }
+    } else if (act == kActToRefine) {
+        // Refinement from the FP32 point: it becomes the FP64 base and the
+        // correction starts at zero.
+        for (int j = tid; j < in.n; j += kBlock) in.d.x[j] = static_cast<double>(in.f.xh[j]);
+        for (int i = tid; i < in.m; i += kBlock) in.d.y[i] = static_cast<double>(in.f.yh[i]);
+        __syncthreads();
+        refineRecenter(in);
+        if (tid == 0) {
+            c.phase = kPhaseRefine;
+            c.refined = 1;
+            c.skip_eval = 1;
+            c.inner = 0;
+            c.fpe0 = -1.0;
+            c.last_fpe = INFINITY;
+            ++c.restarts;
+        }
+    } else if (act == kActRefineToFp64) {
+        for (int j = tid; j < in.n; j += kBlock) in.d.x[j] = in.d.x0[j] = in.d.xh[j];
+        for (int i = tid; i < in.m; i += kBlock) in.d.y[i] = in.d.y0[i] = in.d.yh[i];
+        if (tid == 0) {
+            c.phase = kPhaseFp64;
+            c.inner = 0;
+            c.fpe0 = -1.0;
+            c.last_fpe = INFINITY;
+            ++c.restarts;
+        }
+    } else if (act == kActRestart && refining) {
+        // The base moves to the evaluated point; the correction restarts.
+        for (int j = tid; j < in.n; j += kBlock) in.d.x[j] = in.d.xh[j];
+        for (int i = tid; i < in.m; i += kBlock) in.d.y[i] = in.d.yh[i];
+        __syncthreads();
+        refineRecenter(in);
+        if (tid == 0) {
+            c.inner = 0;
+            c.fpe0 = -1.0;
+            c.last_fpe = INFINITY;
         }
     } else if (act == kActRestart) {
         for (int j = tid; j < in.n; j += kBlock) {
@@ -768,11 +825,25 @@ __global__ __launch_bounds__(kBlock) void kEvalControl(const BInst* __restrict__
     }
 }
 
+// FP64 evaluation points of the refining instances evaluated this round.
+__global__ __launch_bounds__(kBlock) void kRefineMaterialize(const BInst* __restrict__ insts,
+                                                             int K) {
+    if (static_cast<int>(blockIdx.x) >= K) return;
+    const BInst& in = insts[blockIdx.x];
+    const BCtrl& c = *in.ctrl;
+    if (c.phase != kPhaseRefine || !evaluatedNow<double>(c)) return;
+    refineMaterialize(in);
+}
+
 // The evaluation and control of one round for the instances in precision T:
-// part offsets, the parts, then the per-instance control.
+// part offsets, the parts, then the per-instance control.  The FP64
+// evaluation also covers refining instances, at their materialized points.
 template <typename T>
 void enqueueEval(hipStream_t st, int grid, const BInst* di, int K, const BOptions& bo, int* off,
                  double* parts) {
+    if constexpr (std::is_same_v<T, double>) {
+        if (bo.refine) kRefineMaterialize<<<K, kBlock, 0, st>>>(di, K);
+    }
     kEvalScan<T><<<1, 1024, 0, st>>>(di, K, off);
     kEvalParts<T><<<grid, kBlock, 0, st>>>(di, K, off, parts);
     kEvalControl<T><<<K, kBlock, 0, st>>>(di, K, bo, parts);
@@ -804,6 +875,9 @@ struct BInitArgs {
     double* u;
     float* lf;
     float* uf;
+    // The matrix's FP32 objective and row bounds: a refined instance of an
+    // earlier run had its FP32 vectors pointed at its correction data.
+    const float *cf, *rlf, *ruf;
     const double* col_scale;
     const double* scal;  // owner scal
     int n, m, own_bounds, mixed;
@@ -849,7 +923,13 @@ __global__ __launch_bounds__(kBlock) void kInstInit(const BInst* __restrict__ in
         in.f.yh[i] = 0.0f;
     }
     if (tid == 0) {
+        BInst& w = const_cast<BInst&>(insts[blockIdx.x]);
+        w.f.c = a.cf;
+        w.f.rl = a.rlf;
+        w.f.ru = a.ruf;
+        w.f.q = nullptr;
         BCtrl& c = *in.ctrl;
+        c.refined = 0;
         c.step = 0.998 / sqrt(maxRef(a.scal[4], 1e-12));
         c.pw = primal_weight;
         c.fpe0 = -1.0;
@@ -893,7 +973,8 @@ __global__ void kInstRecover(const BInst* __restrict__ insts, double* const* __r
     const BCtrl& c = *in.ctrl;
     const int tid = threadIdx.x;
     const double inv_cs = 1.0 / c.cs, inv_os = 1.0 / c.os;
-    const bool f32 = c.fp32_iters == c.total && c.total > 0;  // never left FP32
+    // Never left the plain FP32 phase: the answer is the FP32 iterate.
+    const bool f32 = !c.refined && c.fp32_iters == c.total && c.total > 0;
     const bool clamp = c.status != MPX_STATUS_UNBOUNDED;
     double* x = xo[blockIdx.x];
     double* y = yo[blockIdx.x];
@@ -989,6 +1070,7 @@ size_t instBytes(int m, int n, bool own_bounds) {
     const size_t M = m, N = n;
     size_t b = 4 * pad(N, 8) + 3 * pad(M, 8) + 4 * pad(N, 4) + 3 * pad(M, 4);  // iterates
     b += 2 * pad(N, 4);                                                     // fp32 bounds
+    b += pad(N, 4) + 3 * pad(M, 4);                                         // refinement data
     if (own_bounds) b += 4 * pad(N, 8);                                     // lo_o hi_o l u
     b += pad(1, sizeof(BCtrl)) + 1024;
     return b;
@@ -1202,6 +1284,15 @@ void carveInstance(const BOwner& w, const mpx_lp& lp, int owner_id, int preset, 
     float* uf = cv.take<float>(n);
     a.lf = lf;
     a.uf = uf;
+    a.cf = w.cf;
+    a.rlf = w.rlf;
+    a.ruf = w.ruf;
+    if (mixed) {
+        in.rc = cv.take<float>(n);
+        in.rrl = cv.take<float>(m);
+        in.rru = cv.take<float>(m);
+        in.rq = cv.take<float>(m);
+    }
     auto fillVecs = [&](auto& v, auto* tag) {
         using T = std::remove_pointer_t<decltype(tag)>;
         v.x = cv.take<T>(n);
@@ -1245,8 +1336,12 @@ void carveInstance(const BOwner& w, const mpx_lp& lp, int owner_id, int preset, 
 struct IlSet {
     IlBufs<float> f;
     IlBufs<double> d;
-    int *slots32 = nullptr, *slots64 = nullptr, *counts = nullptr;
-    ILane *lanes32 = nullptr, *lanes64 = nullptr;
+    // Mixed precision: the FP32 refinement set (iterates and bounds in r,
+    // per-lane c' in rc and row data rrl, rru, rq).
+    IlBufs<float> r;
+    float *rc = nullptr, *rrl = nullptr, *rru = nullptr, *rq = nullptr;
+    int *slots32 = nullptr, *slots64 = nullptr, *slotsR = nullptr, *counts = nullptr;
+    ILane *lanes32 = nullptr, *lanes64 = nullptr, *lanesR = nullptr;
     // FP64 bound flags per (group, tile of 64 columns): bit j % 64 set when
     // some lane's l or u at column j differs from the owner's; the other
     // columns read the owner's bound (one cached value) instead of 64 lanes'
@@ -1260,9 +1355,9 @@ struct IlSet {
         const size_t en = ng * kLanes * static_cast<size_t>(n);
         const size_t em = ng * kLanes * static_cast<size_t>(m);
         size_t b = 5 * pad(en, 8) + 2 * pad(em, 8);
-        if (mixed) b += 5 * pad(en, 4) + 2 * pad(em, 4);
+        if (mixed) b += 2 * (5 * pad(en, 4) + 2 * pad(em, 4)) + pad(en, 4) + 3 * pad(em, 4);
         const size_t c = static_cast<size_t>(cap);
-        return b + 2 * pad(c, 4) + 2 * pad(c, sizeof(ILane)) + pad(4, 4) +
+        return b + 3 * pad(c, 4) + 3 * pad(c, sizeof(ILane)) + pad(4, 4) +
                pad(ng * boundTiles(n), 8);
     }
     void carve(Carve& c, int cap, int n, int m, bool mixed) {
@@ -1279,11 +1374,20 @@ struct IlSet {
             b.y0 = c.take<T>(ng * kLanes * M);
         };
         take(d, static_cast<double*>(nullptr));
-        if (mixed) take(f, static_cast<float*>(nullptr));
+        if (mixed) {
+            take(f, static_cast<float*>(nullptr));
+            take(r, static_cast<float*>(nullptr));
+            rc = c.take<float>(ng * kLanes * N);
+            rrl = c.take<float>(ng * kLanes * M);
+            rru = c.take<float>(ng * kLanes * M);
+            rq = c.take<float>(ng * kLanes * M);
+        }
         slots32 = c.take<int>(static_cast<size_t>(cap));
         slots64 = c.take<int>(static_cast<size_t>(cap));
+        slotsR = c.take<int>(static_cast<size_t>(cap));
         lanes32 = c.take<ILane>(static_cast<size_t>(cap));
         lanes64 = c.take<ILane>(static_cast<size_t>(cap));
+        lanesR = c.take<ILane>(static_cast<size_t>(cap));
         counts = c.take<int>(4);
         bf64 = c.take<unsigned long long>(ng * boundTiles(n));
     }
@@ -1309,13 +1413,24 @@ inline int ilGrid(int grid) { return 2 * grid; }
 void ilGather(hipStream_t st, int grid, const BInst* di, int K, const BOwner& w, IlSet& s,
               bool f32) {
     const int n = w.n, m = w.m;
-    kSlotLists<<<1, 1024, 0, st>>>(di, K, s.owner, s.slots32, s.slots64, s.lanes32, s.lanes64,
-                                   s.counts);
+    kSlotLists<<<1, 1024, 0, st>>>(di, K, s.owner, s.slots32, s.slots64, s.slotsR, s.lanes32,
+                                   s.lanes64, s.lanesR, s.counts);
     if (f32) {
         kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, n, 0, s.f.x, s.f.x0,
                                                     s.f.l, s.f.u, nullptr, nullptr, nullptr);
         kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, m, 1, s.f.y, s.f.y0,
                                                     nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (s.rc) {
+            const int* cr = s.counts + 2;
+            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, n, 0, s.r.x, s.r.x0,
+                                                        s.r.l, s.r.u, nullptr, nullptr, nullptr);
+            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, m, 1, s.r.y, s.r.y0,
+                                                        nullptr, nullptr, nullptr, nullptr, nullptr);
+            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, n, 2, s.rc, nullptr,
+                                                        nullptr, nullptr, nullptr, nullptr, nullptr);
+            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, m, 3, s.rrl, s.rru,
+                                                        s.rq, nullptr, nullptr, nullptr, nullptr);
+        }
     }
     kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x, s.d.x0,
                                                  s.d.l, s.d.u, w.l, w.u, s.bf64);
@@ -1367,6 +1482,15 @@ void ilIterate(hipStream_t st, int grid_base, const BInst* di, const BOwner& w, 
     if (f32) {
         primal(true);
         dual(true);
+        if (s.rc) {
+            // The refinement's correction phase: one column / row per wave.
+            kILPrimal<float, 1, true><<<grid, kBlock, 0, st>>>(
+                di, s.slotsR, s.counts + 2, s.lanesR, n, m, w.cs, w.ri, w.valtf, nullptr, w.tauf,
+                s.r.y, s.r.x, s.r.x0, s.r.xr, s.r.l, s.r.u, nullptr, nullptr, nullptr, k, s.rc);
+            kILDual<float, 1, true><<<grid, kBlock, 0, st>>>(
+                di, s.slotsR, s.counts + 2, s.lanesR, n, m, w.rs, w.ci, w.valf, w.sigf, nullptr,
+                nullptr, s.r.xr, s.r.y, s.r.y0, k, s.rrl, s.rru, s.rq);
+        }
     }
     primal(false);
     dual(false);
@@ -1378,6 +1502,10 @@ void ilScatter(hipStream_t st, int grid, const BInst* di, const BOwner& w, IlSet
     if (f32) {
         kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, n, 0, s.f.x);
         kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, m, 1, s.f.y);
+        if (s.rc) {
+            kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, s.counts + 2, n, 0, s.r.x);
+            kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, s.counts + 2, m, 1, s.r.y);
+        }
     }
     kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x);
     kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y);
@@ -1414,8 +1542,11 @@ BOptions makeBOptions(const mpx_pdlp_options& o) {
     b.rs_art = o.restart_artificial_fraction;
     b.precond_ratio = o.preconditioner_refresh_ratio;
     b.mixed = o.precision == MPX_PREC_MIXED;
+    // FP32 refinement phase after the plain FP32 phase (else FP64 iterations).
+    b.refine = b.mixed && o.mixed_refine ? 1 : 0;
     b.handover = o.mixed_handover > 0.0 ? o.mixed_handover : kFp32ExitRatio;
-    // FP64 phase of mixed precision: refine_tol, or the solve's own tolerances.
+    // Tolerance of the refinement (or FP64) phase of mixed precision:
+    // refine_tol, or the solve's own tolerances.
     b.refine_p_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.primal_tol) : o.primal_tol;
     b.refine_d_tol = o.refine_tol > 0.0 ? std::min(o.refine_tol, o.dual_tol) : o.dual_tol;
     b.refine_g_tol =
@@ -1955,10 +2086,11 @@ int BatchEngine::step(int rounds) {
     if (mixed_ && !fp32_gone_) {
         // FP32 instances at the start of the last round (0 stays 0).
         const int* src = interleaved_ ? il_.counts : totals_;
-        int c2[2] = {1, 1};
-        MPX_HIP(hipMemcpyAsync(c2, src, sizeof(c2), hipMemcpyDeviceToHost, st_));
+        int c3[3] = {1, 1, 1};
+        MPX_HIP(hipMemcpyAsync(c3, src, sizeof(c3), hipMemcpyDeviceToHost, st_));
         MPX_HIP(hipStreamSynchronize(st_));
-        n32 = interleaved_ ? c2[0] : c2[0] + c2[1];
+        // Instances in the FP32 kernels (plain or refinement).
+        n32 = interleaved_ ? c3[0] + c3[2] : c3[0] + c3[1];
         if (rounds_ > 0 && n32 == 0) fp32_gone_ = true;
     }
     MPX_HIP(hipStreamSynchronize(st_));
@@ -2784,7 +2916,7 @@ private:
             // is in its FP32 phase (activation of new ones resets this).
             bool any = false;
             for (int s = 0; s < S_ && !any; ++s)
-                any = slot_req_[s] && h_ctrl_.p[s].phase == kPhaseFp32;
+                any = slot_req_[s] && iteratesFp32(h_ctrl_.p[s].phase);
             fp32_gone_ = !any;
         }
         iterate_ms_ += 1000.0 * std::chrono::duration<double>(std::chrono::steady_clock::now() -

@@ -24,6 +24,7 @@
 #pragma once
 
 #include "device_common.h"
+#include "pdlp_refine.h"
 
 namespace mpx {
 
@@ -138,18 +139,21 @@ __device__ inline T dualProx(T v, T sig, T rl, T ru) {
     return T(0);
 }
 
+// q (may be null): the refinement's base multipliers; with q the prox is
+// the shifted one of pdlp_refine.h.
 template <typename T, bool STORE_HAT>
 __device__ inline void dualEpilogue(int i, T axr, T* __restrict__ y, const T* __restrict__ y0,
                                     T* __restrict__ yh, const T* __restrict__ sigma_base,
                                     const T* __restrict__ rl, const T* __restrict__ ru,
-                                    const IterScalars* __restrict__ sc, int k) {
+                                    const T* __restrict__ q, const IterScalars* __restrict__ sc,
+                                    int k) {
     const int kk = sc->inner + k;
     const T lam = static_cast<T>(static_cast<double>(kk) / static_cast<double>(kk + 1));
     const T step = static_cast<T>(sc->step), pw = static_cast<T>(sc->primal_weight);
     const T sig = step * sigma_base[i] * pw;
     const T yi = y[i];
     const T v = yi + sig * axr;
-    const T h = dualProx(v, sig, rl[i], ru[i]);
+    const T h = q ? dualProxShifted(v, sig, rl[i], ru[i], q[i]) : dualProx(v, sig, rl[i], ru[i]);
     const T r = T(2) * h - yi;
     if (STORE_HAT) yh[i] = h;
     y[i] = lam * r + (T(1) - lam) * y0[i];
@@ -161,8 +165,8 @@ __global__ __launch_bounds__(kBlock) void kDualStep(
     int m, const int* __restrict__ rstarts, const int* __restrict__ cidx,
     const T* __restrict__ val, const T* __restrict__ xr, T* __restrict__ y,
     const T* __restrict__ y0, T* __restrict__ yh, const T* __restrict__ sigma_base,
-    const T* __restrict__ rl, const T* __restrict__ ru, const IterScalars* __restrict__ sc,
-    int k, int skip_above) {
+    const T* __restrict__ rl, const T* __restrict__ ru, const T* __restrict__ q,
+    const IterScalars* __restrict__ sc, int k, int skip_above) {
     const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
     const int i = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
     if (i >= m) return;
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void kDualStep(
     if (e - b > skip_above) return;
     const T axr = groupDotBE<G>(b, e, cidx, val, xr, lane);
     if (lane != 0) return;
-    dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, sc, k);
+    dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, q, sc, k);
 }
 
 template <typename T, bool STORE_HAT>
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void kDualStepLong(
     const int* __restrict__ cidx, const T* __restrict__ val, const T* __restrict__ xr,
     T* __restrict__ y, const T* __restrict__ y0, T* __restrict__ yh,
     const T* __restrict__ sigma_base, const T* __restrict__ rl, const T* __restrict__ ru,
-    const IterScalars* __restrict__ sc, int k) {
+    const T* __restrict__ q, const IterScalars* __restrict__ sc, int k) {
     const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(gid / kWave));
     const int lane = static_cast<int>(gid % kWave);
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void kDualStepLong(
     const int i = list[w];
     const T axr = groupDotBE<kWave>(rstarts[i], rstarts[i + 1], cidx, val, xr, lane);
     if (lane != 0) return;
-    dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, sc, k);
+    dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, q, sc, k);
 }
 
 // Dual half step with A x_r in column chunks: chunk c of C holds, row by
@@ -202,8 +206,8 @@ __global__ __launch_bounds__(kBlock) void kDualChunk(
     int m, const int* __restrict__ rs_c, const int* __restrict__ ci_c, const T* __restrict__ val_c,
     const T* __restrict__ xr, T* __restrict__ part, int c, int C, T* __restrict__ y,
     const T* __restrict__ y0, T* __restrict__ yh, const T* __restrict__ sigma_base,
-    const T* __restrict__ rl, const T* __restrict__ ru, const IterScalars* __restrict__ sc,
-    int k) {
+    const T* __restrict__ rl, const T* __restrict__ ru, const T* __restrict__ q,
+    const IterScalars* __restrict__ sc, int k) {
     const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
     const int i = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
     if (i >= m) return;
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void kDualChunk(
         part[i] = part[i] + acc;
     } else {
         const T axr = C > 1 ? part[i] + acc : acc;
-        dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, sc, k);
+        dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, q, sc, k);
     }
 }
 

@@ -39,6 +39,7 @@
 #include "lp_filter.h"
 #include "pdlp_accel.h"
 #include "pdlp_kernels.h"
+#include "pdlp_refine.h"
 #include "pdlp_setup.h"
 
 namespace mpx {
@@ -359,6 +360,14 @@ private:
     int aa_w_ = 0, aa_count_ = 0, nbaa_ = 1;
     double *aa_z_ = nullptr, *aa_r_ = nullptr, *aa_alpha_ = nullptr;
     std::unique_ptr<PinnedHost> acc_pinned_;
+    // FP64 iterative refinement of mixed precision (pdlp_refine.h): while
+    // refine_ is set the FP32 iterates are a correction around the FP64 base
+    // (x_b in v64_.x, y_b in v64_.y), v32_.c / l / u / rl / ru hold the
+    // correction's data and q32_ the row shift.
+    bool refine_ = false;
+    float* q32_ = nullptr;
+    void recenter();
+    void materialize();
 };
 
 template <>
@@ -428,7 +437,7 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     if (adaptive_ || aa_w_) bytes += padBytes(accParts, 8) + padBytes(std::max<size_t>(pairs, 2), 8);
     if (mixed_) {
         bytes += 2 * padBytes(nnz, 4);
-        bytes += 8 * padBytes(n, 4) + 6 * padBytes(m, 4);
+        bytes += 8 * padBytes(n, 4) + 7 * padBytes(m, 4);
     }
     bytes += 4096;
     arena_.reserve(bytes);
@@ -511,6 +520,7 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
         v32_.sig = arena_.take<float>(m);
         v32_.rl = arena_.take<float>(m);
         v32_.ru = arena_.take<float>(m);
+        q32_ = arena_.take<float>(m);
     }
     pinned_ = std::make_unique<PinnedHost>(sizeof(IterScalars) + 32 * sizeof(double));
     sc_host_ = static_cast<IterScalars*>(pinned_->p);
@@ -782,6 +792,9 @@ template <typename T>
 void SingleSolver::launchIteration(int k, bool storeHat) {
     IterVecs<T>& v = vecs<T>();
     hipStream_t st = stream_.s;
+    // The refinement's row shift (FP32 correction phase only).
+    const T* q = nullptr;
+    if constexpr (std::is_same_v<T, float>) q = refine_ ? q32_ : nullptr;
     const int gridC = std::max(1, ceilDiv(static_cast<long long>(n_) * gc_, kBlock));
     const int gridR = std::max(1, ceilDiv(static_cast<long long>(m_) * gr_, kBlock));
     withGroup(gc_, [&](auto GI) {
@@ -825,11 +838,11 @@ void SingleSolver::launchIteration(int k, bool storeHat) {
                 if (storeHat)
                     kDualChunk<G, T, true><<<gridCh, kBlock, 0, st>>>(
                         m_, rsc, cich_, valc, v.xr, part, c, nch_, v.y, v.y0, v.yh, v.sig, v.rl,
-                        v.ru, sc_dev_, k);
+                        v.ru, q, sc_dev_, k);
                 else
                     kDualChunk<G, T, false><<<gridCh, kBlock, 0, st>>>(
                         m_, rsc, cich_, valc, v.xr, part, c, nch_, v.y, v.y0, v.yh, v.sig, v.rl,
-                        v.ru, sc_dev_, k);
+                        v.ru, q, sc_dev_, k);
             }
         });
         return;
@@ -838,23 +851,23 @@ void SingleSolver::launchIteration(int k, bool storeHat) {
         constexpr int G = decltype(GI)::value;
         if (storeHat)
             kDualStep<G, T, true><<<gridR, kBlock, 0, st>>>(m_, rs_, ci_, v.val, v.xr, v.y, v.y0,
-                                                           v.yh, v.sig, v.rl, v.ru, sc_dev_, k,
+                                                           v.yh, v.sig, v.rl, v.ru, q, sc_dev_, k,
                                                            skip_r_);
         else
             kDualStep<G, T, false><<<gridR, kBlock, 0, st>>>(m_, rs_, ci_, v.val, v.xr, v.y,
-                                                            v.y0, v.yh, v.sig, v.rl, v.ru,
+                                                            v.y0, v.yh, v.sig, v.rl, v.ru, q,
                                                             sc_dev_, k, skip_r_);
     });
     if (n_long_r_) {
         const int g = ceilDiv(static_cast<long long>(n_long_r_) * kWave, kBlock);
         if (storeHat)
             kDualStepLong<T, true><<<g, kBlock, 0, st>>>(n_long_r_, long_r_, rs_, ci_, v.val, v.xr,
-                                                         v.y, v.y0, v.yh, v.sig, v.rl, v.ru,
+                                                         v.y, v.y0, v.yh, v.sig, v.rl, v.ru, q,
                                                          sc_dev_, k);
         else
             kDualStepLong<T, false><<<g, kBlock, 0, st>>>(n_long_r_, long_r_, rs_, ci_, v.val,
                                                           v.xr, v.y, v.y0, v.yh, v.sig, v.rl,
-                                                          v.ru, sc_dev_, k);
+                                                          v.ru, q, sc_dev_, k);
     }
 }
 
@@ -980,6 +993,34 @@ void SingleSolver::recover(double* x_out, double* y_out, double* objective) {
     *objective = objv;
     if (x_out) std::memcpy(x_out, hx.data(), hx.size() * 8);
     if (y_out) std::memcpy(y_out, hy.data(), hy.size() * 8);
+}
+
+// Moves the refinement's base to the evaluated point (v64_.xh / yh, already
+// materialized in FP64) and rebuilds the correction's data around it.
+void SingleSolver::recenter() {
+    hipStream_t st = stream_.s;
+    MPX_HIP(hipMemcpyAsync(v64_.x, v64_.xh, static_cast<size_t>(n_) * 8, hipMemcpyDeviceToDevice, st));
+    if (m_)
+        MPX_HIP(hipMemcpyAsync(v64_.y, v64_.yh, static_cast<size_t>(m_) * 8, hipMemcpyDeviceToDevice,
+                               st));
+    kRefineCols<float><<<std::max(1, ceilDiv(n_, kBlock)), kBlock, 0, st>>>(
+        n_, cs_, ri_, v64_.valt, v64_.c, v64_.l, v64_.u, v64_.x, v64_.y, v32_.c, v32_.l, v32_.u,
+        v32_.x, v32_.x0, v32_.xh);
+    if (m_)
+        kRefineRows<float><<<ceilDiv(m_, kBlock), kBlock, 0, st>>>(
+            m_, rs_, ci_, v64_.val, v64_.rl, v64_.ru, v64_.x, v64_.y, v32_.rl, v32_.ru, q32_,
+            v32_.y, v32_.y0, v32_.yh);
+    MPX_HIP(hipGetLastError());
+}
+
+// FP64 evaluation point of the correction phase: x^ = x_b + dx^ (snapped to
+// the FP64 bounds), y^ = y_b + dy^ and the anchors, into v64_.xh / x0 / yh / y0.
+void SingleSolver::materialize() {
+    hipStream_t st = stream_.s;
+    kRefineMaterialize<float><<<std::max(1, ceilDiv(std::max(n_, m_), kBlock)), kBlock, 0, st>>>(
+        n_, m_, v64_.x, v64_.y, v32_.xh, v32_.x0, v32_.yh, v32_.y0, v32_.l, v32_.u, v64_.l,
+        v64_.u, q32_, v32_.rl, v32_.ru, v64_.xh, v64_.x0, v64_.yh, v64_.y0);
+    MPX_HIP(hipGetLastError());
 }
 
 void SingleSolver::convertIterates64to32() {
@@ -1171,6 +1212,13 @@ bool SingleSolver::andersonRestart() {
     return true;
 }
 
+// Evaluations without halving the KKT ratio after which the plain FP32 phase
+// hands over (kRefineStall) and the correction phase falls back to FP64 (ten
+// times that: on the seeded families the correction phase reaches 1e-8 in
+// FP32 with the iteration count of FP64 PDHG, so only a real precision stall
+// of the FP32 matrix should trigger it).
+constexpr int kRefineStall = 20;
+
 void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double* x_out,
                          double* y_out, mpx_pdlp_result* res) {
     const mpx_pdlp_options& o = o_;
@@ -1197,12 +1245,18 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
     inner_ = 0;
     int status = MPX_STATUS_ITER_LIMIT;
     bool done = false;
-    Graph g64, g32;
+    Graph g64, g32, g32r;
     // Mixed precision: FP32 blocks until the FP64-evaluated KKT reaches the
-    // target or stops improving, then FP64 blocks from the FP32 iterate.
+    // handover ratio or stops improving; then FP32 blocks on the correction
+    // around an FP64 base that moves at every restart (iterative refinement,
+    // pdlp_refine.h), or -- with mixed_refine off, adaptive steps or Anderson
+    // acceleration -- FP64 blocks from the FP32 iterate.  A correction phase
+    // that stops improving hands over to FP64 as well.
     bool fp32_phase = mixed_;
-    int stall = 0;
-    double best_kkt = std::numeric_limits<double>::infinity();
+    refine_ = false;
+    const bool can_refine = mixed_ && o.mixed_refine != 0 && !adaptive_ && !aa_w_;
+    int stall = 0, rstall = 0;
+    double best_kkt = std::numeric_limits<double>::infinity(), rbest = best_kkt;
     if (fp32_phase) {
         convertProblem<float>(stream_.s, v32_, v64_, m_, n_, nnz_);
         convertChunks();
@@ -1227,7 +1281,8 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
         if (accelerate && aa_w_)
             extrapolated = fp32_phase ? andersonRestart<float>() : andersonRestart<double>();
         if (!extrapolated) {
-            if (fp32_phase) restartCopy<float>();
+            if (refine_) recenter();
+            else if (fp32_phase) restartCopy<float>();
             else restartCopy<double>();
         }
         have_prev_ = false;
@@ -1272,19 +1327,26 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
             break;
         }
         const int batch = std::min(N, o.max_iter - total);
-        if (fp32_phase) runBlock<float>(batch, g32, true);
+        if (refine_) runBlock<float>(batch, g32r, true);
+        else if (fp32_phase) runBlock<float>(batch, g32, true);
         else runBlock<double>(batch, g64, true);
         inner_ += batch;
         total += batch;
-        if (fp32_phase) fp32_iters += batch;
-        e = fp32_phase ? evaluate<float>() : evaluate<double>();
+        if (fp32_phase || refine_) fp32_iters += batch;
+        if (refine_) {
+            materialize();
+            e = evaluate<double>();
+        } else {
+            e = fp32_phase ? evaluate<float>() : evaluate<double>();
+        }
         if (fpe0 < 0.0) fpe0 = e.fpe;
         const double tp = o.optimality_norm == 1 ? e.rel_p_inf : e.rel_p;
         const double td = o.optimality_norm == 1 ? e.rel_d_inf : e.rel_d;
         if (o.verbose && (total <= N || total % (10 * N) == 0))
             std::printf("PDLP %7d  pobj=% .8e  pinf=% .2e  dinf=% .2e  gap=% .2e  fpe=% .2e  "
                         "pw=% .2e  step=% .2e  [MI355X%s]\n",
-                        total, e.pobj, tp, td, e.gap, e.fpe, pw_, step_, fp32_phase ? " fp32" : "");
+                        total, e.pobj, tp, td, e.gap, e.fpe, pw_, step_,
+                        refine_ ? " fp32 refine" : (fp32_phase ? " fp32" : ""));
         if (fp32_phase) {
             // FP32 iterations judged in FP64: hand over to FP64 (restarted at
             // the FP32 point) once the KKT ratio is within mixed_handover, or when FP32
@@ -1294,13 +1356,16 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
             if (k < 0.5 * best_kkt) {
                 best_kkt = k;
                 stall = 0;
-            } else if (++stall >= 20) {
+            } else if (++stall >= kRefineStall) {
                 handover = true;
             }
             if (handover) {
                 convertIterates32to64();
                 fp32_phase = false;
+                refine_ = can_refine;  // the base is the FP32 point (v64_.xh / yh)
                 restartNow(false);
+                rbest = kktOf(e);
+                rstall = 0;
                 ++restarts;
                 continue;
             }
@@ -1308,6 +1373,19 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
                    e.gap <= (mixed_ ? rtol_g : o.optimality_tol)) {
             converged = true;
             break;
+        } else if (refine_) {
+            // A correction phase that stops improving continues in FP64
+            // from its evaluated point.
+            const double k = kktOf(e);
+            if (k < 0.5 * rbest) {
+                rbest = k;
+                rstall = 0;
+            } else if (++rstall >= 10 * kRefineStall) {
+                refine_ = false;
+                restartNow(false);
+                ++restarts;
+                continue;
+            }
         }
         if (adaptive_) {
             if (fp32_phase) adaptStep<float>(step_lb, step_ub);
@@ -1366,6 +1444,8 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
         }
     }
 
+    // The polish phase and recovery run in FP64 on the evaluated point.
+    refine_ = false;
     if (!done && converged && o.enable_polishing && !timeUp()) {
         const double rp = o.optimality_norm == 1 ? e.rel_p_inf : e.rel_p;
         const double rd = o.optimality_norm == 1 ? e.rel_d_inf : e.rel_d;
@@ -1573,6 +1653,7 @@ void mpx_pdlp_default_options(mpx_pdlp_options* o) {
     o->mixed_handover = 3.0;
     o->spmv_chunk_cols = 0;
     o->packed_preset = -1;
+    o->mixed_refine = 1;
 }
 
 const char* mpx_last_error(void) { return mpx::lastError(); }

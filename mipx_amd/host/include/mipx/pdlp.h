@@ -74,6 +74,8 @@ struct PdlpOptions {
     Int device = 0;
     Real mixed_handover = 3.0;  // FP32 -> FP64 at this KKT ratio (see mipx_pdlp.h)
     Int spmv_chunk_cols = 0;    // column chunks of the dual step's SpMV (see mipx_pdlp.h)
+    Real refine_tol = -1.0;     // mixed precision: tolerance of the refinement (see mipx_pdlp.h)
+    bool mixed_refine = true;   // mixed precision: FP32 refinement around an FP64 base
 };
 
 class PdlpSolver : public LpSolver {

@@ -75,6 +75,8 @@ mpx_pdlp_options toC(const PdlpOptions& p) {
     o.device = p.device;
     o.mixed_handover = p.mixed_handover;
     o.spmv_chunk_cols = p.spmv_chunk_cols;
+    o.refine_tol = p.refine_tol;
+    o.mixed_refine = p.mixed_refine ? 1 : 0;
     return o;
 }
 
