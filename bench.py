@@ -81,6 +81,15 @@ def primal_kernel_bytes(lp, slots, sv):
     return slots * (6 * n + m) * sv + 2 * n * sv + nnz * (sv + 4) + (n + 1) * 4
 
 
+def refine_primal_bytes(lp, slots):
+    """Algorithmic bytes of one launch of the refinement's primal kernel
+    (FP32) for `slots` instances sharing A: per instance x, x0, l', u', c'
+    read, x and x_r written (7n values) and y read once (m values); shared
+    once per launch: tau (n values) and A' in CSR."""
+    m, n, nnz = lp.num_rows, lp.num_cols, lp.nnz
+    return slots * (7 * n + m) * 4 + n * 4 + nnz * (4 + 4) + (n + 1) * 4
+
+
 def read_traffic(workload, precision, kernel=None):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
     (FETCH_SIZE, WRITE_SIZE), committed under profiles/ when available."""
@@ -203,14 +212,18 @@ def iter_bytes(m, n, nnz, sv):
 
 def whole_solve_bytes_shared(lp, res, rounds, N):
     """Algorithmic bytes of a whole shared-A batch solve: per instance and
-    iteration (7 n + 4 m) s_v at the precision it ran in (FP32 iterations at
-    4 bytes, the rest at 8), plus per batched iteration the matrix in both
-    orientations and the shared vectors (c, tau, sigma, rl, ru) once, counted
-    at FP32 width (a lower bound)."""
+    iteration (7 n + 4 m) s_v at the precision it ran in (plain FP32
+    iterations at 4 bytes, FP64 at 8), the refinement's FP32 iterations
+    (8 n + 7 m) 4 (its objective and row data are per instance: c', rl', ru',
+    q), plus per batched iteration the matrix in both orientations and the
+    shared vectors (c, tau, sigma, rl, ru) once, counted at FP32 width (a
+    lower bound)."""
     m, n, nnz = lp.num_rows, lp.num_cols, lp.nnz
     per = lambda sv: (7 * n + 4 * m) * sv
-    inst = sum(r["iterations_fp32"] * per(4) + (r["iterations"] - r["iterations_fp32"]) * per(8)
-               for r in res)
+    ref = (8 * n + 7 * m) * 4
+    inst = sum((r["iterations_fp32"] - r.get("iterations_refine", 0)) * per(4)
+               + r.get("iterations_refine", 0) * ref
+               + (r["iterations"] - r["iterations_fp32"]) * per(8) for r in res)
     shared = rounds * N * (2 * nnz * 8 + (m + n + 2) * 4 + (2 * n + 3 * m) * 4)
     return inst + shared
 
@@ -453,6 +466,9 @@ def main():
                     help="upload the workload and time --launches launches of the iteration "
                          "kernels one by one (for rocprofv3), then a 1 GiB device copy")
     ap.add_argument("--launches", type=int, default=20)
+    ap.add_argument("--profile-refine", action="store_true",
+                    help="with --profile-only (mixed precision): time the refinement's "
+                         "correction kernels instead, every instance moved into that phase")
     a = ap.parse_args()
     if a.count is None:
         a.count = 1024 if a.workload == "c3" else 4096
@@ -496,9 +512,10 @@ def main():
         # bench).  Packed batches have nothing to tune.
         if lps is None:
             batch.run()
-        prof = batch.profile(launches=a.launches)
+        prof = batch.profile(launches=a.launches, refine=a.profile_refine)
         sv = 4 if prec == 1 else 8
         bl = (packed_primal_bytes(lps, sv) if lps is not None
+              else refine_primal_bytes(lp, prof["profiled_slots"]) if a.profile_refine
               else primal_kernel_bytes(lp, prof["profiled_slots"], sv))
         print(json.dumps({"columns_per_wave": prof["columns_per_wave"],
                           "rows_per_wave": prof["rows_per_wave"],
@@ -554,6 +571,26 @@ def main():
     kernels = {primal_kernel or "kPackedPrimal": {
         "us": round(prof["primal_kernel_us"], 2), "GBs": round(achieved, 1),
         "frac": round(achieved / PEAK_GBS, 4)}}
+    mean_plain = float(np.mean([r["iterations_fp32"] - r["iterations_refine"] for r in res]))
+    mean_ref = float(np.mean([r["iterations_refine"] for r in res]))
+    mean_64 = float(np.mean([r["iterations"] - r["iterations_fp32"] for r in res]))
+    # Kernel time of a whole solve, kernel by kernel: mean iterations in the
+    # kernel's phase x its launch time at full occupancy (an upper bound).
+    # (packed batches run the refinement in the same FP32 packed kernel)
+    first = (mean_64 if prec == 0 else mean_plain + (mean_ref if lps is not None else 0.0))
+    shares = {primal_kernel or "kPackedPrimal": first * prof["primal_kernel_us"]}
+    if prec == 1 and lps is None and a.refine:
+        b2 = native.ResidentBatch(lp, lo, hi, precision=1, **TOL)
+        pr = b2.profile(launches=50, refine=True)
+        b2.close()
+        byr = refine_primal_bytes(lp, pr["profiled_slots"])
+        gr = byr / (pr["primal_kernel_us"] * 1e-6) / 1e9
+        kernels["kILPrimal<float,1,true> (refinement)"] = {
+            "us": round(pr["primal_kernel_us"], 2), "GBs": round(gr, 1),
+            "frac": round(gr / PEAK_GBS, 4), "dual_us": round(pr["dual_kernel_us"], 2),
+            "algorithmic_bytes_per_launch": int(byr), "instances_in_launch": pr["profiled_slots"]}
+        shares["kILPrimal<float,1,true> (refinement)"] = mean_ref * pr["primal_kernel_us"]
+        log(f"refinement primal kernel {pr['primal_kernel_us']:.1f} us -> {gr:.0f} GB/s")
     if prec == 1 and lps is None:
         # The FP64 primal kernel of the same instances, timed the same way
         # (an FP64 batch, every instance active).
@@ -566,7 +603,24 @@ def main():
         kernels[f"kILPrimal<double,{p64['columns_per_wave']}>"] = {
             "us": round(p64["primal_kernel_us"], 2), "GBs": round(g64, 1),
             "frac": round(g64 / PEAK_GBS, 4), "dual_us": round(p64["dual_kernel_us"], 2)}
+        shares[f"kILPrimal<double,{p64['columns_per_wave']}>"] = mean_64 * p64["primal_kernel_us"]
         log(f"fp64 primal kernel {p64['primal_kernel_us']:.1f} us -> {g64:.0f} GB/s")
+    # The roofline line's kernel: the primal kernel with the largest share of
+    # a solve's kernel time.
+    dom_key = max(shares, key=shares.get)
+    if dom_key in kernels and "algorithmic_bytes_per_launch" in kernels[dom_key]:
+        kd = kernels[dom_key]
+        dom_gbs, dom_us = kd["GBs"], kd["us"]
+        dom_bytes, dom_slots = kd["algorithmic_bytes_per_launch"], kd["instances_in_launch"]
+    elif dom_key.startswith("kILPrimal<double"):
+        kd = kernels[dom_key]
+        dom_gbs, dom_us = kd["GBs"], kd["us"]
+        dom_bytes, dom_slots = primal_kernel_bytes(lp, a.count, 8), a.count
+    else:
+        dom_key = primal_kernel or "kPackedPrimal"
+        dom_gbs, dom_us, dom_bytes, dom_slots = (achieved, prof["primal_kernel_us"], bytes_launch,
+                                                 prof["profiled_slots"])
+    log(f"dominant primal kernel: {dom_key} ({shares})")
     ceiling = copy_ceiling()
     log(f"streaming copy ceiling {ceiling} GB/s")
 
@@ -627,14 +681,16 @@ def main():
                                 "nnz=48000, tol 1e-6 rel KKT"),
                    "instances": a.count, "precision": a.precision,
                    "kernels": "lane-per-instance" if st["interleaved"] else "packed"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
-                     "traffic": read_traffic(a.workload, a.precision, primal_kernel),
-                     "kernel": ("kPackedPrimal (primal half step, packed tiles)" if lps is not None
-                                else f"{primal_kernel} (primal half step, lane-per-instance)"),
-                     "kernel_us": round(prof["primal_kernel_us"], 2),
-                     "algorithmic_bytes_per_launch": int(bytes_launch),
-                     "instances_in_launch": prof["profiled_slots"],
+        "roofline": {"bound": "hbm", "achieved": round(dom_gbs, 1), "peak": PEAK_GBS,
+                     "unit": "GB/s", "frac": round(dom_gbs / PEAK_GBS, 4),
+                     "traffic": read_traffic(a.workload, a.precision, dom_key.split(" ")[0]),
+                     "kernel": (("kPackedPrimal (primal half step, packed tiles)" if lps is not None
+                                 else f"{dom_key} (primal half step, lane-per-instance)")
+                                + ": the primal kernel with the largest share of a solve's "
+                                  "kernel time"),
+                     "kernel_us": round(dom_us, 2),
+                     "algorithmic_bytes_per_launch": int(dom_bytes),
+                     "instances_in_launch": dom_slots,
                      "kernels": kernels,
                      "whole_solve": {"achieved": round(ws_gbs, 1), "frac": round(ws_gbs / PEAK_GBS, 4),
                                      "algorithmic_bytes": int(ws_bytes),
@@ -652,6 +708,9 @@ def main():
         "status_counts": status,
         "mean_iterations": float(iters.mean()),
         "mean_fp32_iterations": float(np.mean([r["iterations_fp32"] for r in res])),
+        "mean_iterations_by_phase": {"fp32": round(mean_plain, 1), "fp32_refinement": round(mean_ref, 1),
+                                     "fp64": round(mean_64, 1)},
+        "primal_kernel_time_per_solve_us": {k: round(v, 1) for k, v in shares.items()},
         "plain_mean_iterations": float(np.mean(plain_iters)),
         "max_rel_kkt": kkt,
         "setup_ms": round(st["setup_ms"], 2),

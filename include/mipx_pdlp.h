@@ -179,9 +179,13 @@ typedef struct mpx_pdlp_result {
     double step;             /* final step: 0.998 / ||A_scaled||_2, moved by adaptive_step */
     double primal_weight;    /* final primal weight */
     int32_t restarts;
-    int32_t iterations_fp32; /* MPX_PREC_MIXED: iterations run in FP32 */
+    int32_t iterations_fp32; /* MPX_PREC_MIXED: iterations run in FP32 (plain
+                                FP32 phase and refinement) */
     double seconds;          /* wall time inside the library */
     double setup_seconds;    /* upload + scaling + power iteration */
+    int32_t iterations_refine; /* MPX_PREC_MIXED: of iterations_fp32, those on
+                                  the correction around the FP64 base */
+    int32_t pad;
 } mpx_pdlp_result;
 
 void mpx_pdlp_default_options(mpx_pdlp_options* opts);
@@ -305,6 +309,10 @@ int mpx_batch_get_stats(mpx_batch* b, mpx_batch_stats* stats);
 /* Re-initialises the batch and times `launches` launches of the two
  * iteration kernels one by one (fills the primal/dual kernel fields). */
 int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats);
+/* Mixed precision: the same for the refinement's correction kernels, with
+ * every instance first moved into the refinement phase around its initial
+ * point (the correction kernels at full occupancy). */
+int mpx_batch_profile_refine(mpx_batch* b, int32_t launches, mpx_batch_stats* stats);
 int mpx_batch_destroy(mpx_batch* b);
 
 /* ---- serving loop (continuous batching) ---------------------------------

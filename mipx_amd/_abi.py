@@ -116,6 +116,8 @@ class MpxPdlpResult(C.Structure):
         ("iterations_fp32", C.c_int32),
         ("seconds", C.c_double),
         ("setup_seconds", C.c_double),
+        ("iterations_refine", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
     def as_dict(self):

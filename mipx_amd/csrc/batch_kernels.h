@@ -46,7 +46,7 @@ struct BCtrl {
     int32_t inner, total, pcount, phase, status, restarts, fp32_iters, stall;
     int32_t power_iters, g_row, g_col, use_lds;
     int32_t round_iters, skip_eval;
-    int32_t refined, pad2;        // 1 once the instance entered kPhaseRefine
+    int32_t refined, refine_iters;  // 1 once the instance entered kPhaseRefine; its iterations
     unsigned long long t_finish;  // device wall clock when the instance finished
 };
 

@@ -499,6 +499,7 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         c.total += ran;
         if (polish) c.pcount += ran;
         if (kFp32 || refining) c.fp32_iters += ran;
+        if (refining) c.refine_iters += ran;
         if (c.fpe0 < 0.0) c.fpe0 = e.fpe;
         c.rel_p = e.rel_p;
         c.rel_d = e.rel_d;
@@ -930,6 +931,7 @@ __global__ __launch_bounds__(kBlock) void kInstInit(const BInst* __restrict__ in
         w.f.q = nullptr;
         BCtrl& c = *in.ctrl;
         c.refined = 0;
+        c.refine_iters = 0;
         c.step = 0.998 / sqrt(maxRef(a.scal[4], 1e-12));
         c.pw = primal_weight;
         c.fpe0 = -1.0;
@@ -1596,7 +1598,10 @@ public:
     void prepare();
     // One instrumented round: every launch of the iteration kernels timed
     // with HIP events on the engine's stream.  Returns averages in us.
-    void profileRound(int launches, double* primal_us, double* dual_us, int* slots);
+    // refine (mixed precision): every instance is first moved into the
+    // refinement phase and the correction kernels are timed instead.
+    void profileRound(int launches, double* primal_us, double* dual_us, int* slots,
+                      bool refine = false);
     // Per-instance completion times (ms since the end of prepare()).
     std::vector<double> finishTimesMs();
     int rounds() const { return rounds_; }
@@ -1916,7 +1921,28 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     enqueueEval<double>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
 }
 
-void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us, int* slots) {
+// Every active instance moved into the refinement phase around its current
+// FP32 point (used to time the correction kernels at full occupancy).
+__global__ __launch_bounds__(kBlock) void kForceRefine(const BInst* __restrict__ insts, int K) {
+    if (static_cast<int>(blockIdx.x) >= K) return;
+    const BInst& in = insts[blockIdx.x];
+    BCtrl& c = *in.ctrl;
+    __shared__ int go;
+    if (threadIdx.x == 0) go = c.phase == kPhaseFp32 && c.round_iters > 0;
+    __syncthreads();
+    if (!go) return;
+    for (int j = threadIdx.x; j < in.n; j += kBlock) in.d.x[j] = static_cast<double>(in.f.xh[j]);
+    for (int i = threadIdx.x; i < in.m; i += kBlock) in.d.y[i] = static_cast<double>(in.f.yh[i]);
+    __syncthreads();
+    refineRecenter(in);
+    if (threadIdx.x == 0) {
+        c.phase = kPhaseRefine;
+        c.refined = 1;
+    }
+}
+
+void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us, int* slots,
+                               bool refine) {
     const int K = count();
     const BInst* di = static_cast<const BInst*>(d_insts_buf_->p);
     launches = std::max(1, std::min(launches, bo_.N));
@@ -1925,7 +1951,33 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
     double tp = 0.0, td = 0.0;
     int active = 0;
     const bool f32 = mixed_;  // the FP32 kernels carry the work in mixed mode
-    if (interleaved_) {
+    refine = refine && mixed_;
+    if (refine) kForceRefine<<<K, kBlock, 0, st_>>>(di, K);
+    if (interleaved_ && refine) {
+        const BOwner& w = owners_[0];
+        ilGather(st_, grid_, di, K, w, il_, true);
+        for (int k = 1; k <= launches; ++k) {
+            MPX_HIP(hipEventRecord(e[0], st_));
+            kILPrimal<float, 1, true><<<ilGrid(grid_), kBlock, 0, st_>>>(
+                di, il_.slotsR, il_.counts + 2, il_.lanesR, w.n, w.m, w.cs, w.ri, w.valtf, nullptr,
+                w.tauf, il_.r.y, il_.r.x, il_.r.x0, il_.r.xr, il_.r.l, il_.r.u, nullptr, nullptr,
+                nullptr, k, il_.rc);
+            MPX_HIP(hipEventRecord(e[1], st_));
+            kILDual<float, 1, true><<<ilGrid(grid_), kBlock, 0, st_>>>(
+                di, il_.slotsR, il_.counts + 2, il_.lanesR, w.n, w.m, w.rs, w.ci, w.valf, w.sigf,
+                nullptr, nullptr, il_.r.xr, il_.r.y, il_.r.y0, k, il_.rrl, il_.rru, il_.rq);
+            MPX_HIP(hipEventRecord(e[2], st_));
+            MPX_HIP(hipEventSynchronize(e[2]));
+            float a = 0.0f, b = 0.0f;
+            MPX_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+            MPX_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+            tp += a;
+            td += b;
+        }
+        int c3[3];
+        MPX_HIP(hipMemcpy(c3, il_.counts, sizeof(c3), hipMemcpyDeviceToHost));
+        active = c3[2];
+    } else if (interleaved_) {
         const BOwner& w = owners_[0];
         // Only the precision that carries the work is gathered and timed.
         ilGather(st_, grid_, di, K, w, il_, f32);
@@ -2168,6 +2220,7 @@ void BatchEngine::results(double* const* x_out, double* const* y_out, mpx_pdlp_r
         r.primal_weight = c.pw;
         r.restarts = c.restarts;
         r.iterations_fp32 = c.fp32_iters;
+        r.iterations_refine = c.refine_iters;
         r.setup_seconds = setup_seconds_;
         const double* hx = host.data() + xo[i];
         const double* hy = host.data() + yo[i];
@@ -2977,6 +3030,7 @@ private:
             r.primal_weight = c.pw;
             r.restarts = c.restarts;
             r.iterations_fp32 = c.fp32_iters;
+            r.iterations_refine = c.refine_iters;
             r.seconds = std::chrono::duration<double>(now - q->t_submit).count();
             r.setup_seconds = std::chrono::duration<double>(q->t_active - q->t_submit).count();
             // Same conventions as the batched results: Optimal keeps both,
@@ -3316,13 +3370,17 @@ int mpx_batch_get_stats(mpx_batch* b, mpx_batch_stats* stats) {
     return MPX_OK;
 }
 
-int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats) {
+static int profileBatch(mpx_batch* b, int32_t launches, mpx_batch_stats* stats, bool refine) {
     if (!b || launches <= 0) return MPX_ERR_INVALID;
+    if (refine && b->opts.precision != MPX_PREC_MIXED) {
+        mpx::setError("mpx_batch_profile_refine: the batch is not mixed precision");
+        return MPX_ERR_INVALID;
+    }
     try {
-        if (b->eng->rounds() > 0) b->eng->prepare();  // fresh after create otherwise
+        if (b->eng->rounds() > 0 || refine) b->eng->prepare();  // fresh after create otherwise
         double p = 0.0, d = 0.0;
         int slots = 0;
-        b->eng->profileRound(launches, &p, &d, &slots);
+        b->eng->profileRound(launches, &p, &d, &slots, refine);
         b->stats.primal_kernel_us = p;
         b->stats.dual_kernel_us = d;
         b->stats.profiled_slots = slots;
@@ -3335,6 +3393,14 @@ int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats) {
     }
     if (stats) *stats = b->stats;
     return MPX_OK;
+}
+
+int mpx_batch_profile(mpx_batch* b, int32_t launches, mpx_batch_stats* stats) {
+    return profileBatch(b, launches, stats, false);
+}
+
+int mpx_batch_profile_refine(mpx_batch* b, int32_t launches, mpx_batch_stats* stats) {
+    return profileBatch(b, launches, stats, true);
 }
 
 int mpx_batch_destroy(mpx_batch* b) {

@@ -1235,7 +1235,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
     have_prev_ = false;
     aa_count_ = 0;
 
-    int total = 0, restarts = 0, fp32_iters = 0;
+    int total = 0, restarts = 0, fp32_iters = 0, refine_iters = 0;
     double fpe0 = -1.0, last_fpe = std::numeric_limits<double>::infinity();
     double pw_err_sum = 0.0, pw_last_err = 0.0, best_pw = pw_, best_score =
                                                                  std::numeric_limits<double>::infinity();
@@ -1333,6 +1333,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
         inner_ += batch;
         total += batch;
         if (fp32_phase || refine_) fp32_iters += batch;
+        if (refine_) refine_iters += batch;
         if (refine_) {
             materialize();
             e = evaluate<double>();
@@ -1508,6 +1509,7 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
     res->step = step_;
     res->restarts = restarts;
     res->iterations_fp32 = fp32_iters;
+    res->iterations_refine = refine_iters;
 }
 
 void SingleSolver::iterateFrom(const mpx_scaled_problem* sp, int iters, int inner, double step,

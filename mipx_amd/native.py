@@ -23,6 +23,7 @@ EXPORTED = [
     "mpx_server_destroy", "mpx_spmv_csr_f64", "mpx_spmv_csr_f32", "mpx_pdlp_scale_problem",
     "mpx_pdhg_iterate", "mpx_pdlp_scale_problem_batched", "mpx_batch_create_shared", "mpx_batch_create", "mpx_batch_run",
     "mpx_batch_results", "mpx_batch_get_stats", "mpx_batch_profile", "mpx_batch_destroy",
+    "mpx_batch_profile_refine",
     "mpx_kkt_check_device", "mpx_kkt_check_device_noalloc",
 ]
 
@@ -81,6 +82,7 @@ def lib():
     L.mpx_batch_results.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MpxPdlpResult)]
     L.mpx_batch_get_stats.argtypes = [C.c_void_p, C.POINTER(MpxBatchStats)]
     L.mpx_batch_profile.argtypes = [C.c_void_p, C.c_int32, C.POINTER(MpxBatchStats)]
+    L.mpx_batch_profile_refine.argtypes = [C.c_void_p, C.c_int32, C.POINTER(MpxBatchStats)]
     L.mpx_batch_destroy.argtypes = [C.c_void_p]
     L.mpx_kkt_check_device.argtypes = [C.POINTER(MpxLp)] + [C.c_void_p] * 7
     L.mpx_kkt_check_device_noalloc.argtypes = [C.POINTER(MpxLp)] + [C.c_void_p] * 7
@@ -253,9 +255,12 @@ class ResidentBatch:
         _check(lib().mpx_batch_get_stats(self._h, C.byref(st)), "mpx_batch_get_stats")
         return st.as_dict()
 
-    def profile(self, launches=50):
+    def profile(self, launches=50, refine=False):
+        """Times `launches` launches of the iteration kernels (refine: the
+        refinement's correction kernels, mixed precision)."""
         st = MpxBatchStats()
-        _check(lib().mpx_batch_profile(self._h, launches, C.byref(st)), "mpx_batch_profile")
+        f = lib().mpx_batch_profile_refine if refine else lib().mpx_batch_profile
+        _check(f(self._h, launches, C.byref(st)), "mpx_batch_profile")
         return st.as_dict()
 
     def close(self):

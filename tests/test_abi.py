@@ -34,7 +34,7 @@ def test_default_options_match_reference_defaults():
 def test_struct_sizes():
     # Field order mirrors the header; sizes follow the x86-64 C ABI.
     assert C.sizeof(_abi.MpxLp) == 4 * 4 + 8 * 8 + 8
-    assert C.sizeof(_abi.MpxPdlpResult) == 2 * 4 + 9 * 8 + 2 * 4 + 2 * 8
+    assert C.sizeof(_abi.MpxPdlpResult) == 2 * 4 + 9 * 8 + 2 * 4 + 2 * 8 + 2 * 4
 
 
 def test_invalid_arguments_rejected_without_gpu():
