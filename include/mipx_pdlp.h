@@ -217,7 +217,10 @@ int mpx_copy_bench(int64_t bytes, double* ms);
  * (pdlp.cpp:2271-2307), anderson_acceleration extrapolates at main-loop
  * restarts (293-437, 2418-2466; window at most 64), matrix_zero_tol drops
  * |a| < tol before scaling (499-518; certificates use the LP as given).
- * Returns MPX_OK unless the arguments are invalid; a device failure is
+ * Returns MPX_ERR_INVALID for invalid arguments and MPX_ERR_OUT_OF_MEMORY
+ * (res->status MPX_STATUS_ERROR) when device memory cannot be allocated, so
+ * that a caller can fall back to a CPU solve as the reference does
+ * (pdlp.cpp:1767-1771); otherwise MPX_OK, with any other device failure
  * reported as MPX_STATUS_ERROR in `res`.
  */
 int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts,
@@ -336,8 +339,10 @@ int mpx_batch_destroy(mpx_batch* b);
  * variant's to mpx_pdlp_solve_batch_shared() of its base with those column
  * bounds alone (same options), whatever else is in flight.
  *
- * Per request, max_solve_seconds (if >= 0) counts from its activation;
- * stop_fn is not polled.  Calls on one server must not overlap.
+ * Per request, max_solve_seconds (if >= 0) counts from its activation.
+ * stop_fn is polled once per mpx_server_step(): while it returns nonzero,
+ * every instance running at that step finishes with MPX_STATUS_ITER_LIMIT.
+ * Calls on one server must not overlap.
  */
 typedef struct mpx_server mpx_server;
 

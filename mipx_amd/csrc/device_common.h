@@ -29,11 +29,15 @@ const char* lastError();
         hipError_t e_ = (expr);                                                            \
         if (e_ != hipSuccess) {                                                            \
             ::mpx::setError(std::string(#expr) + ": " + hipGetErrorString(e_));            \
-            throw ::mpx::DeviceError();                                                    \
+            throw ::mpx::DeviceError{e_};                                                  \
         }                                                                                  \
     } while (0)
 
-struct DeviceError {};
+// A failed HIP call (code: its error) or an internal consistency failure
+// (code hipSuccess).
+struct DeviceError {
+    hipError_t code = hipSuccess;
+};
 
 inline int ceilDiv(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 

@@ -3058,15 +3058,23 @@ private:
         }
     }
 
+    // Per-request time limits, and the stop callback: while it returns
+    // nonzero every running instance ends with ITER_LIMIT, as a stopped
+    // single solve does (reference pdlp.cpp:2136-2140).
     void enforceTimeLimit() {
-        if (o_.max_solve_seconds < 0.0 || running_ == 0) return;
+        if (running_ == 0) return;
+        const bool stop = o_.stop_fn != nullptr && o_.stop_fn(o_.stop_user) != 0;
+        if (o_.max_solve_seconds < 0.0 && !stop) return;
         const auto now = std::chrono::steady_clock::now();
         BCtrl* dc = static_cast<BCtrl*>(ctrl_->p);
         for (int s = 0; s < S_; ++s) {
             const ServeReq* q = slot_req_[s].get();
             if (!q || h_ctrl_.p[s].phase == kPhaseDone) continue;
-            if (std::chrono::duration<double>(now - q->t_active).count() >= o_.max_solve_seconds)
+            if (o_.max_solve_seconds >= 0.0 &&
+                std::chrono::duration<double>(now - q->t_active).count() >= o_.max_solve_seconds)
                 kAbortSlot<<<1, 1, 0, st_>>>(dc + s, MPX_STATUS_TIME_LIMIT);
+            else if (stop)
+                kAbortSlot<<<1, 1, 0, st_>>>(dc + s, MPX_STATUS_ITER_LIMIT);
         }
         MPX_HIP(hipGetLastError());
     }

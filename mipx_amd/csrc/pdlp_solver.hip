@@ -1793,9 +1793,13 @@ int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
         res->setup_seconds =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         s.solve(t0, x_out, y_out, res);
-    } catch (const DeviceError&) {
+    } catch (const DeviceError& e) {
         res->status = MPX_STATUS_ERROR;
         res->objective = 0.0;
+        if (e.code == hipErrorOutOfMemory) {
+            finish();
+            return MPX_ERR_OUT_OF_MEMORY;
+        }
     } catch (const std::bad_alloc&) {
         setError("host out of memory");
         res->status = MPX_STATUS_ERROR;

@@ -167,6 +167,31 @@ def test_server_statuses_and_time_limit():
     assert r["status"] == 4 and 0 < r["iterations"] <= 2 * 200, r
 
 
+def test_server_stop_callback():
+    """The stop callback (reference pdlp.cpp:2136-2140) is polled once per
+    step: while it returns nonzero the running requests end as IterLimit;
+    once it returns zero again new requests solve as usual."""
+    from mipx_amd._abi import STOP_FN
+    flag = {"stop": False, "polls": 0}
+
+    def stop(_user):
+        flag["polls"] += 1
+        return 1 if flag["stop"] and flag["polls"] >= 3 else 0
+
+    cb = STOP_FN(stop)
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    srv = native.Server(max_slots=4, stop_fn=cb, **TIGHT)
+    flag["stop"] = True
+    t = srv.submit(g.lp)
+    r = _drain(srv, [t])[t][0]
+    assert r["status"] == 3 and 0 < r["iterations"] <= 4 * 200, r
+    flag["stop"] = False
+    t = srv.submit(g.lp)
+    r2 = _drain(srv, [t])[t][0]
+    srv.close()
+    assert r2["status"] == 0 and r2["iterations"] > r["iterations"], r2
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_server_warm_variants_identical_to_solo_batch(precision):
     """Warm-started bounds variants (children of a solved root) in a mix with
