@@ -436,10 +436,11 @@ def run_c1(a):
 
 def run_c1_solve(a, prec):
     """C1 solved to 1e-6 relative KKT in one precision (mixed: FP32
-    iterations, FP64 checks, then refinement to 1e-8), under a wall-clock
-    limit; the library's progress lines (every 2,000 iterations) go to
-    stdout.  The returned point is certified on the device
-    (mpx_kkt_check_device) in original space."""
+    iterations, FP64 checks, then FP32 refinement around an FP64 base, to
+    --c1-refine-tol when given), under a wall-clock limit; the library's
+    progress lines (every 2,000 iterations) go to stdout.  The returned
+    point is certified on the device (mpx_kkt_check_device) in original
+    space; like the reference, a solve stopped by a limit returns no point."""
     from mipx_amd import lpgen, native
     g = lpgen.c1_large()
     lp = g.lp
@@ -447,8 +448,8 @@ def run_c1_solve(a, prec):
     log(f"c1 solve: m={m} n={n} nnz={nnz} precision={'mixed' if prec else 'fp64'} "
         f"limit {a.c1_solve:.0f} s")
     opts = dict(TOL, precision=prec, max_iter=4000000, max_solve_seconds=a.c1_solve, verbose=1)
-    if prec == 1:
-        opts["refine_tol"] = 1e-8
+    if prec == 1 and a.c1_refine_tol > 0:
+        opts["refine_tol"] = a.c1_refine_tol
     t = time.perf_counter()
     r, x, y = native.solve(lp, **opts)
     el = time.perf_counter() - t
@@ -472,7 +473,9 @@ def run_c1_solve(a, prec):
         "vs_baseline": None, "dtype": "f32+f64" if prec else "f64",
         "data": "synthetic (seeded generator: mipx_amd/csrc/lpgen.cpp, seed 0xC1)",
         "config": {"workload": f"c1: m={m} n={n} nnz={nnz}, power-law columns 1..200 mean 6, "
-                               f"solved to 1e-6 rel KKT" + (", refined to 1e-8" if prec else "")},
+                               f"solved to 1e-6 rel KKT" +
+                               (f", refined to {a.c1_refine_tol:g}" if prec and a.c1_refine_tol > 0
+                                else "")},
         "solve": out,
     }
 
@@ -487,6 +490,8 @@ def main():
     ap.add_argument("--c1-solve", type=float, default=0.0,
                     help="c1: solve to tolerance in --precision under this wall-clock limit "
                          "(seconds) instead of timing --c1-iters iterations")
+    ap.add_argument("--c1-refine-tol", type=float, default=0.0,
+                    help="c1 solve, mixed precision: refine to this tolerance (0: the solve's)")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--slots", type=int, default=2048)

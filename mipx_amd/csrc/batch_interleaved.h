@@ -93,6 +93,30 @@ __global__ __launch_bounds__(1024) void kSlotLists(const BInst* __restrict__ ins
         for (int p = 0; p < 3; ++p) counts[p] = carry[p];
 }
 
+// The counts kSlotLists will find at the start of the next round (plain
+// FP32, FP64 or polish, refinement), taken after this round's evaluation
+// with the same tests, so that the host can leave the kernels of an empty
+// set out of the next round.
+__global__ __launch_bounds__(1024) void kNextCounts(const BInst* __restrict__ insts, int K,
+                                                    int owner, int* __restrict__ out) {
+    int c[3] = {0, 0, 0};
+    for (int base = 0; base < K; base += 1024) {
+        const int k = base + threadIdx.x;
+        int pl[3] = {0, 0, 0};
+        if (k < K) {
+            const BCtrl& ct = *insts[k].ctrl;
+            const bool mine = owner < 0 || insts[k].owner == owner;
+            const bool act = mine && ct.phase != kPhaseDone && ct.round_iters > 0;
+            pl[0] = act && ct.phase == kPhaseFp32;
+            pl[2] = act && ct.phase == kPhaseRefine;
+            pl[1] = act && !iteratesFp32(ct.phase);
+        }
+        for (int p = 0; p < 3; ++p) c[p] += __syncthreads_count(pl[p]);
+    }
+    if (threadIdx.x == 0)
+        for (int p = 0; p < 3; ++p) out[p] = c[p];
+}
+
 // Work items of a group pass: (group, block of 4 columns or rows).  Fixed
 // grid, count from device memory; groups are dealt XCD by XCD (group g's
 // items run on XCD g % 8 when there are at least 8 groups) so that a group's

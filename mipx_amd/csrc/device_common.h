@@ -19,6 +19,8 @@ namespace mpx {
 
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kWave = 64;
+// XCDs of an MI355X; workgroups of a launch are dealt to them round robin.
+constexpr int kNumXcd = 8;
 
 // Thread-local last error text of the C-ABI.
 void setError(const std::string& msg);

@@ -111,9 +111,9 @@ enum BAction { kActNone = 0, kActRestart = 1, kActToFp64 = 2, kActConverged = 3,
 // evaluations without halving it.
 constexpr double kFp32ExitRatio = 3.0;
 constexpr int kFp32Stall = 20;
-// A refining instance that goes this many evaluations without halving its
-// KKT ratio continues in FP64 (a precision stall of the FP32 matrix; the
-// seeded families reach 1e-8 in the correction phase without one).
+// A refining instance that goes this many evaluations without improving its
+// KKT ratio by 1% continues in FP64 (a precision stall of the FP32 matrix;
+// the seeded families reach 1e-8 in the correction phase without one).
 constexpr int kRefineStall = 10 * kFp32Stall;
 
 template <typename T>
@@ -145,7 +145,6 @@ struct Tile {
     int32_t inst;
     int32_t first;
 };
-constexpr int kNumXcd = 8;
 
 template <typename F>
 __device__ inline void forTiles(int cnt, F&& f) {
@@ -551,8 +550,10 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
                    e.gap <= (o.mixed ? o.refine_g_tol : o.opt_tol)) {
             act = kActConverged;
         } else if (refining) {
+            // Only a ratio that stops improving (not 1% below its best in
+            // kRefineStall evaluations) ends the correction phase.
             const double k = kktRatio(e, o);
-            if (k < 0.5 * c.best_kkt) {
+            if (k < 0.99 * c.best_kkt) {
                 c.best_kkt = k;
                 c.stall = 0;
             } else if (++c.stall >= kRefineStall) {
@@ -1343,6 +1344,7 @@ struct IlSet {
     IlBufs<float> r;
     float *rc = nullptr, *rrl = nullptr, *rru = nullptr, *rq = nullptr;
     int *slots32 = nullptr, *slots64 = nullptr, *slotsR = nullptr, *counts = nullptr;
+    int* next = nullptr;  // kNextCounts: the next round's counts, same order as counts
     ILane *lanes32 = nullptr, *lanes64 = nullptr, *lanesR = nullptr;
     // FP64 bound flags per (group, tile of 64 columns): bit j % 64 set when
     // some lane's l or u at column j differs from the owner's; the other
@@ -1359,7 +1361,7 @@ struct IlSet {
         size_t b = 5 * pad(en, 8) + 2 * pad(em, 8);
         if (mixed) b += 2 * (5 * pad(en, 4) + 2 * pad(em, 4)) + pad(en, 4) + 3 * pad(em, 4);
         const size_t c = static_cast<size_t>(cap);
-        return b + 3 * pad(c, 4) + 3 * pad(c, sizeof(ILane)) + pad(4, 4) +
+        return b + 3 * pad(c, 4) + 3 * pad(c, sizeof(ILane)) + 2 * pad(4, 4) +
                pad(ng * boundTiles(n), 8);
     }
     void carve(Carve& c, int cap, int n, int m, bool mixed) {
@@ -1391,6 +1393,7 @@ struct IlSet {
         lanes64 = c.take<ILane>(static_cast<size_t>(cap));
         lanesR = c.take<ILane>(static_cast<size_t>(cap));
         counts = c.take<int>(4);
+        next = c.take<int>(4);
         bf64 = c.take<unsigned long long>(ng * boundTiles(n));
     }
 };
@@ -1411,33 +1414,42 @@ void withCW(int cw, F&& f) {
 // 228 -> 211 us, FP64 dual 241 -> 219 us against 8 per CU.
 inline int ilGrid(int grid) { return 2 * grid; }
 
+// Kernel sets of a lane-per-instance round: plain FP32 phase, refinement
+// (FP32 correction kernels), FP64 (FP64 phase and polish).  A round may
+// leave out the set of a phase that no instance of the round is in.
+constexpr int kSetPlain = 1, kSetRefine = 2, kSetFp64 = 4, kSetAll = 7;
+inline int setsFor(bool f32) { return (f32 ? kSetPlain | kSetRefine : 0) | kSetFp64; }
+
 // Slot lists of the coming round and per-instance -> interleaved copies.
 void ilGather(hipStream_t st, int grid, const BInst* di, int K, const BOwner& w, IlSet& s,
-              bool f32) {
+              int sets) {
     const int n = w.n, m = w.m;
     kSlotLists<<<1, 1024, 0, st>>>(di, K, s.owner, s.slots32, s.slots64, s.slotsR, s.lanes32,
                                    s.lanes64, s.lanesR, s.counts);
-    if (f32) {
+    if (sets & kSetPlain) {
         kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, n, 0, s.f.x, s.f.x0,
                                                     s.f.l, s.f.u, nullptr, nullptr, nullptr);
         kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, m, 1, s.f.y, s.f.y0,
                                                     nullptr, nullptr, nullptr, nullptr, nullptr);
-        if (s.rc) {
-            const int* cr = s.counts + 2;
-            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, n, 0, s.r.x, s.r.x0,
-                                                        s.r.l, s.r.u, nullptr, nullptr, nullptr);
-            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, m, 1, s.r.y, s.r.y0,
-                                                        nullptr, nullptr, nullptr, nullptr, nullptr);
-            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, n, 2, s.rc, nullptr,
-                                                        nullptr, nullptr, nullptr, nullptr, nullptr);
-            kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, m, 3, s.rrl, s.rru,
-                                                        s.rq, nullptr, nullptr, nullptr, nullptr);
-        }
     }
-    kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x, s.d.x0,
-                                                 s.d.l, s.d.u, w.l, w.u, s.bf64);
-    kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y, s.d.y0,
-                                                 nullptr, nullptr, nullptr, nullptr, nullptr);
+    if ((sets & kSetRefine) && s.rc) {
+        const int* cr = s.counts + 2;
+        kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, n, 0, s.r.x, s.r.x0, s.r.l,
+                                                    s.r.u, nullptr, nullptr, nullptr);
+        kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, m, 1, s.r.y, s.r.y0,
+                                                    nullptr, nullptr, nullptr, nullptr, nullptr);
+        kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, n, 2, s.rc, nullptr,
+                                                    nullptr, nullptr, nullptr, nullptr, nullptr);
+        kInterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, cr, m, 3, s.rrl, s.rru, s.rq,
+                                                    nullptr, nullptr, nullptr, nullptr);
+    }
+    if (sets & kSetFp64) {
+        kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x,
+                                                     s.d.x0, s.d.l, s.d.u, w.l, w.u, s.bf64);
+        kInterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y,
+                                                     s.d.y0, nullptr, nullptr, nullptr, nullptr,
+                                                     nullptr);
+    }
 }
 
 // Columns (rows) per wave of the FP64 kernels of a mixed-precision batch.
@@ -1450,9 +1462,9 @@ constexpr int kIlFp64PerWave = 1;
 
 // Iteration k (primal then dual; FP32 instances first) with `cwp` columns
 // per wave in the primal kernels and `cwd` rows per wave in the dual ones
-// (`cwp64` / `cwd64` for the FP64 kernels).
+// (`cwp64` / `cwd64` for the FP64 kernels), for the kernel sets in `sets`.
 void ilIterate(hipStream_t st, int grid_base, const BInst* di, const BOwner& w, IlSet& s,
-               bool f32, int cwp, int cwd, int cwp64, int cwd64, int k) {
+               int sets, int cwp, int cwd, int cwp64, int cwd64, int k) {
     const int n = w.n, m = w.m;
     const int grid = ilGrid(grid_base);
     auto primal = [&](bool fp32) {
@@ -1481,36 +1493,40 @@ void ilIterate(hipStream_t st, int grid_base, const BInst* di, const BOwner& w, 
                     w.ru, s.d.xr, s.d.y, s.d.y0, k);
         });
     };
-    if (f32) {
+    if (sets & kSetPlain) {
         primal(true);
         dual(true);
-        if (s.rc) {
-            // The refinement's correction phase: one column / row per wave.
-            kILPrimal<float, 1, true><<<grid, kBlock, 0, st>>>(
-                di, s.slotsR, s.counts + 2, s.lanesR, n, m, w.cs, w.ri, w.valtf, nullptr, w.tauf,
-                s.r.y, s.r.x, s.r.x0, s.r.xr, s.r.l, s.r.u, nullptr, nullptr, nullptr, k, s.rc);
-            kILDual<float, 1, true><<<grid, kBlock, 0, st>>>(
-                di, s.slotsR, s.counts + 2, s.lanesR, n, m, w.rs, w.ci, w.valf, w.sigf, nullptr,
-                nullptr, s.r.xr, s.r.y, s.r.y0, k, s.rrl, s.rru, s.rq);
-        }
     }
-    primal(false);
-    dual(false);
+    if ((sets & kSetRefine) && s.rc) {
+        // The refinement's correction phase: one column / row per wave.
+        kILPrimal<float, 1, true><<<grid, kBlock, 0, st>>>(
+            di, s.slotsR, s.counts + 2, s.lanesR, n, m, w.cs, w.ri, w.valtf, nullptr, w.tauf,
+            s.r.y, s.r.x, s.r.x0, s.r.xr, s.r.l, s.r.u, nullptr, nullptr, nullptr, k, s.rc);
+        kILDual<float, 1, true><<<grid, kBlock, 0, st>>>(
+            di, s.slotsR, s.counts + 2, s.lanesR, n, m, w.rs, w.ci, w.valf, w.sigf, nullptr,
+            nullptr, s.r.xr, s.r.y, s.r.y0, k, s.rrl, s.rru, s.rq);
+    }
+    if (sets & kSetFp64) {
+        primal(false);
+        dual(false);
+    }
 }
 
 // Interleaved -> per-instance x and y after the round's last iteration.
-void ilScatter(hipStream_t st, int grid, const BInst* di, const BOwner& w, IlSet& s, bool f32) {
+void ilScatter(hipStream_t st, int grid, const BInst* di, const BOwner& w, IlSet& s, int sets) {
     const int n = w.n, m = w.m;
-    if (f32) {
+    if (sets & kSetPlain) {
         kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, n, 0, s.f.x);
         kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slots32, s.counts, m, 1, s.f.y);
-        if (s.rc) {
-            kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, s.counts + 2, n, 0, s.r.x);
-            kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, s.counts + 2, m, 1, s.r.y);
-        }
     }
-    kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x);
-    kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y);
+    if ((sets & kSetRefine) && s.rc) {
+        kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, s.counts + 2, n, 0, s.r.x);
+        kDeinterleave<float><<<grid, kBlock, 0, st>>>(di, s.slotsR, s.counts + 2, m, 1, s.r.y);
+    }
+    if (sets & kSetFp64) {
+        kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, n, 0, s.d.x);
+        kDeinterleave<double><<<grid, kBlock, 0, st>>>(di, s.slots64, s.counts + 1, m, 1, s.d.y);
+    }
 }
 
 // Iteration presets the online autotuner tries: (lanes per column, lanes per
@@ -1686,7 +1702,13 @@ private:
     std::unique_ptr<DevBuf> d_il_buf_;
     EvalBufs ev_;
     IlSet il_;
+    // Lane-per-instance, mixed precision: the per-phase instance counts of
+    // the coming round (plain FP32, FP64, refinement), known after a round.
+    bool next_known_ = false;
+    int next_[3] = {1, 1, 1};
     void enqueueInterleavedRound(bool timed);
+    int roundSets() const;
+    void readNextCounts();
 };
 
 void BatchEngine::setup() {
@@ -1799,6 +1821,7 @@ void BatchEngine::prepare() {
     graphs_.clear();
     rounds_ = 0;
     fp32_gone_ = false;
+    next_known_ = false;
     if (no) cohort_->run(st_, false);
     if (ni)
         kInstInit<<<ni, kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
@@ -1909,16 +1932,31 @@ void BatchEngine::enqueueInterleavedRound(bool timed) {
     // FP32 kernels only while some instance is still in its FP32 phase
     // (instances never return to FP32, so this is decided on the host).
     const bool f32 = mixed_ && !fp32_gone_;
-    ilGather(st_, grid_, di, K, w, il_, f32);
+    const int sets = roundSets();
+    ilGather(st_, grid_, di, K, w, il_, sets);
     if (timed) MPX_HIP(hipEventRecord(ev0_, st_));
     // FP64 kernels: the tuned pair when the batch is FP64 only.
     const int cwp64 = mixed_ ? kIlFp64PerWave : cwp_, cwd64 = mixed_ ? kIlFp64PerWave : cwd_;
     for (int k = 1; k <= bo_.N; ++k)
-        ilIterate(st_, grid_, di, w, il_, f32, cwp_, cwd_, cwp64, cwd64, k);
+        ilIterate(st_, grid_, di, w, il_, sets, cwp_, cwd_, cwp64, cwd64, k);
     if (timed) MPX_HIP(hipEventRecord(ev1_, st_));
-    ilScatter(st_, grid_, di, w, il_, f32);
+    ilScatter(st_, grid_, di, w, il_, sets);
     if (f32) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
     enqueueEval<double>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
+    if (mixed_) kNextCounts<<<1, 1024, 0, st_>>>(di, K, il_.owner, il_.next);
+}
+
+// The kernel sets of the coming lane-per-instance round: every set until
+// the device has counted the phases once (kNextCounts, read back after each
+// round of a mixed-precision batch), then only those with instances in them.
+int BatchEngine::roundSets() const {
+    if (!mixed_) return kSetFp64;
+    int sets = kSetAll;
+    if (next_known_)
+        sets = (next_[0] > 0 ? kSetPlain : 0) | (next_[2] > 0 ? kSetRefine : 0) |
+               (next_[1] > 0 ? kSetFp64 : 0);
+    if (fp32_gone_) sets &= kSetFp64;
+    return sets;
 }
 
 // Every active instance moved into the refinement phase around its current
@@ -1955,7 +1993,7 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
     if (refine) kForceRefine<<<K, kBlock, 0, st_>>>(di, K);
     if (interleaved_ && refine) {
         const BOwner& w = owners_[0];
-        ilGather(st_, grid_, di, K, w, il_, true);
+        ilGather(st_, grid_, di, K, w, il_, kSetAll);
         for (int k = 1; k <= launches; ++k) {
             MPX_HIP(hipEventRecord(e[0], st_));
             kILPrimal<float, 1, true><<<ilGrid(grid_), kBlock, 0, st_>>>(
@@ -1980,7 +2018,7 @@ void BatchEngine::profileRound(int launches, double* primal_us, double* dual_us,
     } else if (interleaved_) {
         const BOwner& w = owners_[0];
         // Only the precision that carries the work is gathered and timed.
-        ilGather(st_, grid_, di, K, w, il_, f32);
+        ilGather(st_, grid_, di, K, w, il_, setsFor(f32));
         for (int k = 1; k <= launches; ++k) {
             MPX_HIP(hipEventRecord(e[0], st_));
             withCW(cwp_, [&](auto CWI) {
@@ -2077,7 +2115,7 @@ void BatchEngine::launchRound(const IterPreset& p, bool timed) {
         MPX_HIP(hipGetLastError());
         return;
     }
-    G& g = graphs_[{p.gc, p.gr * 2 + (fp32_gone_ ? 1 : 0)}];
+    G& g = graphs_[{p.gc, p.gr * 16 + (interleaved_ ? roundSets() : 0) * 2 + (fp32_gone_ ? 1 : 0)}];
     if (g.e == nullptr) {
         bool ok = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
@@ -2097,6 +2135,25 @@ void BatchEngine::launchRound(const IterPreset& p, bool timed) {
         }
     }
     MPX_HIP(hipGraphLaunch(g.e, st_));
+}
+
+// After a lane-per-instance round of a mixed-precision batch: the phase
+// counts of the next round, and a check that the round just run left out
+// no kernel set that had instances in it.
+void BatchEngine::readNextCounts() {
+    if (!interleaved_ || !mixed_) return;
+    const int ran = roundSets();
+    int h[7];
+    MPX_HIP(hipMemcpyAsync(h, il_.counts, 3 * sizeof(int), hipMemcpyDeviceToHost, st_));
+    MPX_HIP(hipMemcpyAsync(h + 3, il_.next, 3 * sizeof(int), hipMemcpyDeviceToHost, st_));
+    MPX_HIP(hipStreamSynchronize(st_));
+    if ((h[0] > 0 && !(ran & kSetPlain)) || (h[2] > 0 && !(ran & kSetRefine)) ||
+        (h[1] > 0 && !(ran & kSetFp64))) {
+        setError("internal: a lane-per-instance round left out a kernel set with instances");
+        throw DeviceError();
+    }
+    for (int p = 0; p < 3; ++p) next_[p] = h[3 + p];
+    next_known_ = true;
 }
 
 int BatchEngine::step(int rounds) {
@@ -2123,10 +2180,12 @@ int BatchEngine::step(int rounds) {
                 cur_ = tune_res_.front().second;
                 if (interleaved_) tuned_ = cur_;
             }
+            readNextCounts();
             continue;
         }
         launchRound(cur_, false);
         ++rounds_;
+        readNextCounts();
     }
     kZeroInt<<<1, 1, 0, st_>>>(dcount);
     kCountActive<<<ceilDiv(K, kBlock), kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
@@ -2910,7 +2969,7 @@ private:
         kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
         kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_, rec32_, rec64_);
         const bool f32 = mixed_ && !fp32_gone_;
-        for (auto& b : bases_) ilGather(st_, grid_, di, K, b->owner, b->il, f32);
+        for (auto& b : bases_) ilGather(st_, grid_, di, K, b->owner, b->il, setsFor(f32));
         for (int k = 1; k <= bo_.N; ++k) {
             if (f32) {
                 kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tc32_, totals_ + 0, k);
@@ -2922,10 +2981,10 @@ private:
             // batched engine's autotuner settles on for C2-sized bases (FP32
             // primal 174.5 us against 187 us at four columns per wave).
             for (auto& b : bases_)
-                ilIterate(st_, grid_, di, b->owner, b->il, f32, 1, 1, kIlFp64PerWave,
+                ilIterate(st_, grid_, di, b->owner, b->il, setsFor(f32), 1, 1, kIlFp64PerWave,
                           kIlFp64PerWave, k);
         }
-        for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, f32);
+        for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, setsFor(f32));
         if (mixed_) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
         enqueueEval<double>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
     }
