@@ -162,13 +162,6 @@ typedef struct mpx_pdlp_options {
                                          (FP64 iterative refinement) until
                                          refine_tol; 0 = FP64 iterations
                                          instead (default 1) */
-    int32_t spmv_row_parts;           /* single LP: the primal step's A'y in
-                                         this many row parts (2, 4 or 8), part
-                                         p summed by its own XCDs so that each
-                                         gathers from a slice of y that stays
-                                         in its L2, the parts then added in
-                                         order; 0 = automatic (one part, at
-                                         present), < 0 or 1 = never */
 } mpx_pdlp_options;
 
 /* Mirrors `mipx::LpResult` (include/mipx/lp_solver.h:11-16) plus the final

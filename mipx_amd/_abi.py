@@ -96,7 +96,6 @@ class MpxPdlpOptions(C.Structure):
         ("spmv_chunk_cols", C.c_int32),
         ("packed_preset", C.c_int32),
         ("mixed_refine", C.c_int32),
-        ("spmv_row_parts", C.c_int32),
     ]
 
 
@@ -165,7 +164,7 @@ DEFAULT_OPTIONS = dict(
     anderson_acceleration=0, anderson_window_size=5, anderson_regularization=1e-10,
     use_gpu=1, gpu_min_rows=512, gpu_min_nnz=10000, verbose=1,
     precision=PREC_FP64, refine_tol=-1.0, spmv_group=0, device=0, mixed_handover=3.0,
-    spmv_chunk_cols=0, packed_preset=-1, mixed_refine=1, spmv_row_parts=0,
+    spmv_chunk_cols=0, packed_preset=-1, mixed_refine=1,
 )
 
 

@@ -127,49 +127,6 @@ __global__ __launch_bounds__(kBlock) void kPrimalStepLong(
     primalEpilogue<T, STORE_HAT>(j, aty, x, x0, xr, xh, c, tau_base, lo, hi, sc, k);
 }
 
-// ---- primal half step with A'y in row parts -------------------------------
-// For an LP whose y is larger than one XCD's L2, A'y is summed in P row parts
-// (P = 2, 4 or 8): part p of every column is summed by the workgroups of
-// XCDs p * 8/P .. (p + 1) * 8/P - 1 (workgroups are dealt to the eight XCDs
-// round robin), so that each XCD gathers from one slice of y, which its L2
-// holds.  The matrix is kept part-major: per part its own column starts, then
-// that part's entries column by column (rows in order; since a column's rows
-// are sorted, a part is a contiguous piece of the column).  kPrimalPart
-// leaves the sums in part[p * n + j]; kPrimalCombine adds them in part order
-// and runs the primal epilogue.
-template <int G, typename T>
-__global__ __launch_bounds__(kBlock) void kPrimalPart(int n, int P, int nbPer,
-                                                      const int* __restrict__ csp,
-                                                      const int* __restrict__ rip,
-                                                      const T* __restrict__ valp,
-                                                      const T* __restrict__ y,
-                                                      T* __restrict__ part) {
-    const int per = kNumXcd / P;  // XCDs per part
-    const int xcd = static_cast<int>(blockIdx.x) % kNumXcd;
-    const int p = xcd / per;
-    const int pb = static_cast<int>(blockIdx.x) / kNumXcd * per + xcd % per;
-    if (pb >= nbPer) return;
-    const long long gid = static_cast<long long>(pb) * kBlock + threadIdx.x;
-    const int j = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
-    if (j >= n) return;
-    const int* cs = csp + static_cast<size_t>(p) * (static_cast<size_t>(n) + 1);
-    const T acc = groupDotBE<G>(cs[j], cs[j + 1], rip, valp, y, lane);
-    if (lane == 0) part[static_cast<size_t>(p) * n + j] = acc;
-}
-
-template <typename T, bool STORE_HAT>
-__global__ __launch_bounds__(kBlock) void kPrimalCombine(
-    int n, int P, const T* __restrict__ part, T* __restrict__ x, const T* __restrict__ x0,
-    T* __restrict__ xr, T* __restrict__ xh, const T* __restrict__ c,
-    const T* __restrict__ tau_base, const T* __restrict__ lo, const T* __restrict__ hi,
-    const IterScalars* __restrict__ sc, int k) {
-    const int j = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
-    if (j >= n) return;
-    T aty = part[j];
-    for (int p = 1; p < P; ++p) aty += part[static_cast<size_t>(p) * n + j];
-    primalEpilogue<T, STORE_HAT>(j, aty, x, x0, xr, xh, c, tau_base, lo, hi, sc, k);
-}
-
 // Row-type prox of the dual step (src/lp/pdlp.cpp:1043-1067): equality,
 // upper-only, lower-only, ranged; the row type follows from the bounds.
 template <typename T>

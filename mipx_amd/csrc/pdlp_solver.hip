@@ -304,9 +304,6 @@ private:
         if (nch_ && valch32_ && nnz_ > 0)
             kConvert<float, double><<<ceilDiv(nnz_, kBlock), kBlock, 0, stream_.s>>>(nnz_, valch64_,
                                                                                      valch32_);
-        if (nrp_ && valp32_ && nnz_ > 0)
-            kConvert<float, double><<<ceilDiv(nnz_, kBlock), kBlock, 0, stream_.s>>>(nnz_, valp64_,
-                                                                                     valp32_);
     }
     template <typename T>
     void adaptStep(double lb, double ub);
@@ -358,13 +355,6 @@ private:
     int *rsch_ = nullptr, *cich_ = nullptr, *permch_ = nullptr;
     double *valch64_ = nullptr, *part64_ = nullptr;
     float *valch32_ = nullptr, *part32_ = nullptr;
-    // Row parts of the primal step's A'y (0: kPrimalStep): part-major CSC
-    // (column starts per part, then the entries of part 0, 1, ...), its
-    // position map into the scaled CSR values, and the partial sums.
-    int nrp_ = 0, grp_ = 1;
-    int *csp_ = nullptr, *rip_ = nullptr, *permp_ = nullptr;
-    double *valp64_ = nullptr, *ppart64_ = nullptr;
-    float *valp32_ = nullptr, *ppart32_ = nullptr;
     // Anderson acceleration (pdlp.cpp:293-437): window of aa_w_ restart
     // candidates and residuals in HBM, aa_count_ stored so far.
     int aa_w_ = 0, aa_count_ = 0, nbaa_ = 1;
@@ -442,20 +432,6 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
             if (mixed_) bytes += padBytes(nnz, 4) + padBytes(m, 4);
         }
     }
-    // Row parts of the primal step: only when forced by spmv_row_parts (2,
-    // 4, 8); automatic mode keeps one part (on C1 the parts measured slower).
-    {
-        int P = o_.spmv_row_parts;
-        if (P == 0) P = 1;
-        if ((P == 2 || P == 4 || P == 8) && m_ >= P && n_ > 0 && nnz_ > 0) {
-            nrp_ = P;
-            grp_ = pickGroup(nnz_ / P, n_, 0);
-            const size_t PP = static_cast<size_t>(P);
-            bytes += padBytes(PP * (n + 1), 4) + 2 * padBytes(nnz, 4) + padBytes(nnz, 8) +
-                     padBytes(PP * n, 8);
-            if (mixed_) bytes += padBytes(nnz, 4) + padBytes(PP * n, 4);
-        }
-    }
     if (adaptive_) bytes += padBytes(m, 8) + padBytes(n, 8);
     if (aa_w_) bytes += 2 * padBytes(static_cast<size_t>(aa_w_) * dim, 8) + padBytes(aa_w_, 8);
     if (adaptive_ || aa_w_) bytes += padBytes(accParts, 8) + padBytes(std::max<size_t>(pairs, 2), 8);
@@ -515,18 +491,6 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
         if (mixed_) {
             valch32_ = arena_.take<float>(nnz);
             part32_ = arena_.take<float>(m);
-        }
-    }
-    if (nrp_) {
-        const size_t PP = static_cast<size_t>(nrp_);
-        csp_ = arena_.take<int>(PP * (n + 1));
-        rip_ = arena_.take<int>(nnz);
-        permp_ = arena_.take<int>(nnz);
-        valp64_ = arena_.take<double>(nnz);
-        ppart64_ = arena_.take<double>(PP * n);
-        if (mixed_) {
-            valp32_ = arena_.take<float>(nnz);
-            ppart32_ = arena_.take<float>(PP * n);
         }
     }
     if (aa_w_) {
@@ -634,35 +598,6 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
             MPX_HIP(hipMemcpyAsync(permch_, permch.data(), nnz * 4, hipMemcpyHostToDevice, st));
         }
     }
-    // Part-major copy of the CSC structure for the row-part primal step: the
-    // rows of part p are [p mq, (p + 1) mq); a column's entries of one part
-    // are contiguous (rows sorted), kept in order.  Values follow through
-    // permp_ (CSR positions) after scaling.
-    std::vector<int> csp, rip, permp;
-    if (nrp_) {
-        const long long mq = (static_cast<long long>(m_) + nrp_ - 1) / nrp_;
-        csp.resize(static_cast<size_t>(nrp_) * (n + 1));
-        rip.resize(nnz);
-        permp.resize(nnz);
-        size_t at = 0;
-        for (int p = 0; p < nrp_; ++p) {
-            const long long r0 = p * mq, r1 = std::min<long long>(m_, (p + 1) * mq);
-            int* cp = csp.data() + static_cast<size_t>(p) * (n + 1);
-            for (int j = 0; j < n_; ++j) {
-                cp[j] = static_cast<int>(at);
-                for (int k = cs[j]; k < cs[j + 1]; ++k)
-                    if (ri[k] >= r0 && ri[k] < r1) {
-                        rip[at] = ri[k];
-                        permp[at] = perm[k];
-                        ++at;
-                    }
-            }
-            cp[n_] = static_cast<int>(at);
-        }
-        MPX_HIP(hipMemcpyAsync(csp_, csp.data(), csp.size() * 4, hipMemcpyHostToDevice, st));
-        MPX_HIP(hipMemcpyAsync(rip_, rip.data(), nnz * 4, hipMemcpyHostToDevice, st));
-        MPX_HIP(hipMemcpyAsync(permp_, permp.data(), nnz * 4, hipMemcpyHostToDevice, st));
-    }
     MPX_HIP(hipMemcpyAsync(obj_o_, lp_.obj, n * 8, hipMemcpyHostToDevice, st));
     MPX_HIP(hipMemcpyAsync(l_o_, lp_.col_lower, n * 8, hipMemcpyHostToDevice, st));
     MPX_HIP(hipMemcpyAsync(u_o_, lp_.col_upper, n * 8, hipMemcpyHostToDevice, st));
@@ -698,7 +633,6 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
     if (m_) kScaleRows<<<gm, kBlock, 0, st>>>(m_, row_scale_, rl_o_, ru_o_, v64_.rl, v64_.ru);
     if (nnz_) kGather<<<gz, kBlock, 0, st>>>(nnz_, perm_, v64_.val, v64_.valt);
     if (nnz_ && nch_) kGather<<<gz, kBlock, 0, st>>>(nnz_, permch_, v64_.val, valch64_);
-    if (nnz_ && nrp_) kGather<<<gz, kBlock, 0, st>>>(nnz_, permp_, v64_.val, valp64_);
     // Pock-Chambolle (pdlp.cpp:581-604).
     if (o_.do_pock_chambolle_scaling && m_ > 0 && n_ > 0) {
         kPockChambolle<<<gm, kBlock, 0, st>>>(m_, rs_, v64_.val, v64_.sig);
@@ -863,32 +797,6 @@ void SingleSolver::launchIteration(int k, bool storeHat) {
     if constexpr (std::is_same_v<T, float>) q = refine_ ? q32_ : nullptr;
     const int gridC = std::max(1, ceilDiv(static_cast<long long>(n_) * gc_, kBlock));
     const int gridR = std::max(1, ceilDiv(static_cast<long long>(m_) * gr_, kBlock));
-    if (nrp_) {
-        const T* valp;
-        T* part;
-        if constexpr (std::is_same_v<T, double>) {
-            valp = valp64_;
-            part = ppart64_;
-        } else {
-            valp = valp32_;
-            part = ppart32_;
-        }
-        const int nbPer = std::max(1, ceilDiv(static_cast<long long>(n_) * grp_, kBlock));
-        const int per = kNumXcd / nrp_;
-        const int gridP = kNumXcd * ceilDiv(nbPer, per);
-        withGroup(grp_, [&](auto GI) {
-            constexpr int G = decltype(GI)::value;
-            kPrimalPart<G, T><<<gridP, kBlock, 0, st>>>(n_, nrp_, nbPer, csp_, rip_, valp, v.y,
-                                                        part);
-        });
-        const int gn = std::max(1, ceilDiv(n_, kBlock));
-        if (storeHat)
-            kPrimalCombine<T, true><<<gn, kBlock, 0, st>>>(n_, nrp_, part, v.x, v.x0, v.xr, v.xh,
-                                                           v.c, v.tau, v.l, v.u, sc_dev_, k);
-        else
-            kPrimalCombine<T, false><<<gn, kBlock, 0, st>>>(n_, nrp_, part, v.x, v.x0, v.xr, v.xh,
-                                                            v.c, v.tau, v.l, v.u, sc_dev_, k);
-    } else {
     withGroup(gc_, [&](auto GI) {
         constexpr int G = decltype(GI)::value;
         if (storeHat)
@@ -910,7 +818,6 @@ void SingleSolver::launchIteration(int k, bool storeHat) {
             kPrimalStepLong<T, false><<<g, kBlock, 0, st>>>(n_long_c_, long_c_, cs_, ri_, v.valt,
                                                             v.y, v.x, v.x0, v.xr, v.xh, v.c,
                                                             v.tau, v.l, v.u, sc_dev_, k);
-    }
     }
     if (nch_) {
         // A x_r chunk by chunk (no long rows in this mode).
@@ -1637,8 +1544,6 @@ void SingleSolver::iterateFrom(const mpx_scaled_problem* sp, int iters, int inne
     // caller's values too (setup gathered it from its own scaling).
     if (nnz_ && nch_)
         kGather<<<ceilDiv(nnz_, kBlock), kBlock, 0, st>>>(nnz_, permch_, v64_.val, valch64_);
-    if (nnz_ && nrp_)
-        kGather<<<ceilDiv(nnz_, kBlock), kBlock, 0, st>>>(nnz_, permp_, v64_.val, valp64_);
     step_ = step;
     pw_ = pw;
     inner_ = inner;
@@ -1757,7 +1662,6 @@ void mpx_pdlp_default_options(mpx_pdlp_options* o) {
     o->spmv_chunk_cols = 0;
     o->packed_preset = -1;
     o->mixed_refine = 1;
-    o->spmv_row_parts = 0;
 }
 
 const char* mpx_last_error(void) { return mpx::lastError(); }
@@ -1840,11 +1744,6 @@ int mpx_pdlp_solve(const mpx_lp* lp, const mpx_pdlp_options* opts, const double*
     }
     if (opts->anderson_acceleration && opts->anderson_window_size > kMaxAndersonWindow) {
         setError("mpx_pdlp_solve: anderson_window_size above 64");
-        return MPX_ERR_INVALID;
-    }
-    if (opts->spmv_row_parts > 1 && opts->spmv_row_parts != 2 && opts->spmv_row_parts != 4 &&
-        opts->spmv_row_parts != 8) {
-        setError("mpx_pdlp_solve: spmv_row_parts must be 2, 4 or 8 (or <= 1, 0 = automatic)");
         return MPX_ERR_INVALID;
     }
     std::memset(res, 0, sizeof(*res));

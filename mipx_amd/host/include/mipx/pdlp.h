@@ -76,7 +76,6 @@ struct PdlpOptions {
     Int spmv_chunk_cols = 0;    // column chunks of the dual step's SpMV (see mipx_pdlp.h)
     Real refine_tol = -1.0;     // mixed precision: tolerance of the refinement (see mipx_pdlp.h)
     bool mixed_refine = true;   // mixed precision: FP32 refinement around an FP64 base
-    Int spmv_row_parts = 0;     // row parts of the primal step's A'y (see mipx_pdlp.h)
 };
 
 class PdlpSolver : public LpSolver {

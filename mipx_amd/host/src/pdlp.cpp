@@ -77,7 +77,6 @@ mpx_pdlp_options toC(const PdlpOptions& p) {
     o.spmv_chunk_cols = p.spmv_chunk_cols;
     o.refine_tol = p.refine_tol;
     o.mixed_refine = p.mixed_refine ? 1 : 0;
-    o.spmv_row_parts = p.spmv_row_parts;
     return o;
 }
 

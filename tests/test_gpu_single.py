@@ -429,28 +429,3 @@ def test_reduced_costs_at_device_optimum():
     assert rc[np.isinf(hi)].min() >= -scale     # lower-only and free columns
     assert rc[np.isinf(lo)].max() <= scale      # upper-only and free columns
 
-
-@pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("parts", [2, 4, 8])
-def test_row_parts_primal_vs_oracle(parts, precision):
-    """The primal step with A'y summed in row parts (spmv_row_parts; each
-    part on its own XCDs, the parts added in order) solves to the oracle's
-    answer: same status, objective within 1e-6 (FP64; mixed within the 5e-6
-    of the mixed-precision envelope), the point certified; the rest of the
-    solve is the same as without parts."""
-    lp = lpgen.c0_plain(seed=0xC0, m=2000, n=5000).lp
-    r, x, y = native.solve(lp, spmv_row_parts=parts, precision=precision, **TIGHT)
-    ro, _, _ = ob.solve(lp, **TIGHT)
-    print(f"parts={parts} gpu iters={r['iterations']} obj={r['objective']!r}  "
-          f"oracle iters={ro['iterations']} obj={ro['objective']!r}")
-    assert r["status"] == ro["status"] == 0
-    tol = 1e-6 if precision == 0 else 5e-6
-    assert abs(r["objective"] - ro["objective"]) <= tol * (1 + abs(ro["objective"]))
-    certify(lp, r, x, y)
-    assert r["work_units"] == r["iterations"] * (4.0 * lp.nnz + 2.0 * (lp.num_rows + lp.num_cols))
-
-
-def test_row_parts_rejects_other_counts():
-    lp = lpgen.c0_plain(seed=1, m=50, n=120).lp
-    with pytest.raises(RuntimeError):
-        native.solve(lp, spmv_row_parts=3, **TIGHT)
