@@ -1380,14 +1380,16 @@ void SingleSolver::solve(const std::chrono::steady_clock::time_point t0, double*
             // A correction phase that stops improving continues in FP64
             // from its evaluated point.  Re-centring keeps the FP32
             // correction small, so only a KKT ratio that no longer improves
-            // (not 1% below its best in 10 kRefineStall evaluations) counts:
-            // slow but steady PDHG convergence (C1 halves its ratio every
-            // ~750 evaluations) stays in FP32.
+            // counts: not 1% below its best for 10 kRefineStall evaluations
+            // and for half of the phase so far.  The second bound follows
+            // PDHG's own pace: on C1 the gap sits near 1e-5 for ~100,000
+            // iterations while FP64 PDHG does the same from the same point.
             const double k = kktOf(e);
+            const int window = std::max(10 * kRefineStall, refine_iters / (2 * N));
             if (k < kRefineGain * rbest) {
                 rbest = k;
                 rstall = 0;
-            } else if (++rstall >= 10 * kRefineStall) {
+            } else if (++rstall >= window) {
                 refine_ = false;
                 restartNow(false);
                 ++restarts;
