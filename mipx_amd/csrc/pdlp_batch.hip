@@ -550,13 +550,15 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
                    e.gap <= (o.mixed ? o.refine_g_tol : o.opt_tol)) {
             act = kActConverged;
         } else if (refining) {
-            // Only a ratio that stops improving (not 1% below its best in
-            // kRefineStall evaluations) ends the correction phase.
+            // Only a ratio that stops improving (not 1% below its best for
+            // kRefineStall evaluations and for half of the phase so far)
+            // ends the correction phase.
             const double k = kktRatio(e, o);
+            const int window = max(kRefineStall, c.refine_iters / (2 * o.N));
             if (k < 0.99 * c.best_kkt) {
                 c.best_kkt = k;
                 c.stall = 0;
-            } else if (++c.stall >= kRefineStall) {
+            } else if (++c.stall >= window) {
                 act = kActRefineToFp64;
             }
         }
@@ -2535,6 +2537,7 @@ public:
         MPX_HIP(hipStreamSynchronize(st_));
         for (int s = S_ - 1; s >= 0; --s) free_slots_.push_back(s);
         slot_req_.resize(S);
+        starting_.assign(S, 0);
         int dev = 0, cus = 256;
         MPX_HIP(hipGetDevice(&dev));
         MPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2897,6 +2900,7 @@ private:
             }
             const int s = q->slot;
             slot_req_[s] = std::move(q);
+            starting_[s] = 1;
             ++running_;
         }
         preparing_ = std::move(still);
@@ -2969,24 +2973,48 @@ private:
         kScan4<<<1, 1024, 0, st_>>>(cnt, K, di, totals_);
         kTileFill<<<gk, kBlock, 0, st_>>>(di, K, cnt, tc32_, tr32_, tc64_, tr64_, rec32_, rec64_);
         const bool f32 = mixed_ && !fp32_gone_;
-        for (auto& b : bases_) ilGather(st_, grid_, di, K, b->owner, b->il, setsFor(f32));
+        const int sets = roundSets();
+        const int il = sets & kSetAll;
+        const bool p32 = f32 && (sets & kPacked32), p64 = (sets & kPacked64) != 0;
+        for (auto& b : bases_) ilGather(st_, grid_, di, K, b->owner, b->il, il);
         for (int k = 1; k <= bo_.N; ++k) {
-            if (f32) {
+            if (p32) {
                 kPackedPrimal<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tc32_, totals_ + 0, k);
                 kPackedDual<float><<<grid_, kBlock, 0, st_>>>(di, rec32_, tr32_, totals_ + 1, k);
             }
-            kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tc64_, totals_ + 2, k);
-            kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
+            if (p64) {
+                kPackedPrimal<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tc64_, totals_ + 2, k);
+                kPackedDual<double><<<grid_, kBlock, 0, st_>>>(di, rec64_, tr64_, totals_ + 3, k);
+            }
             // One column / row per wave in both precisions: the pair the
             // batched engine's autotuner settles on for C2-sized bases (FP32
             // primal 174.5 us against 187 us at four columns per wave).
             for (auto& b : bases_)
-                ilIterate(st_, grid_, di, b->owner, b->il, setsFor(f32), 1, 1, kIlFp64PerWave,
+                ilIterate(st_, grid_, di, b->owner, b->il, il, 1, 1, kIlFp64PerWave,
                           kIlFp64PerWave, k);
         }
-        for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, setsFor(f32));
+        for (auto& b : bases_) ilScatter(st_, grid_, di, b->owner, b->il, il);
         if (mixed_) enqueueEval<float>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
         enqueueEval<double>(st_, grid_, di, K, bo_, ev_.off, ev_.parts);
+    }
+
+    // Kernel sets of the coming round: packed FP32 / FP64 for fresh LPs,
+    // the lane-per-instance sets for bounds variants, from each running
+    // slot's phase after the last round (a slot activated since then starts
+    // in FP32 when mixed, else FP64).
+    static constexpr int kPacked32 = 8, kPacked64 = 16;
+    int roundSets() const {
+        int sets = 0;
+        for (int s = 0; s < S_; ++s) {
+            if (!slot_req_[s]) continue;
+            const int ph = starting_[s] ? (mixed_ ? kPhaseFp32 : kPhaseFp64) : h_ctrl_.p[s].phase;
+            if (ph == kPhaseDone) continue;
+            if (slot_req_[s]->inst.il)
+                sets |= ph == kPhaseFp32 ? kSetPlain : ph == kPhaseRefine ? kSetRefine : kSetFp64;
+            else
+                sets |= iteratesFp32(ph) ? kPacked32 : kPacked64;
+        }
+        return sets;
     }
 
     void launchRound() {
@@ -2995,7 +3023,7 @@ private:
             MPX_HIP(hipGetLastError());
             return;
         }
-        G& g = graphs_[fp32_gone_ ? 1 : 0];
+        G& g = graphs_[roundSets() * 2 + (fp32_gone_ ? 1 : 0)];
         if (g.e == nullptr) {
             bool ok = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
@@ -3019,10 +3047,34 @@ private:
     void iterate() {
         if (running_ == 0) return;
         const auto t0 = std::chrono::steady_clock::now();
+        const size_t need = 4 + 3 * bases_.size();
+        if (need > h_cnt_cap_) {
+            h_cnt_ = std::make_unique<PinnedArray<int>>(need);
+            h_cnt_cap_ = need;
+        }
+        const int ran = roundSets();
         launchRound();
         ++rounds_;
         MPX_HIP(hipMemcpyAsync(h_ctrl_.p, ctrl_->p, sizeof(BCtrl) * S_, hipMemcpyDeviceToHost, st_));
+        // The round's own tile and slot counts: no kernel set that had work
+        // may have been left out.
+        int* hc = h_cnt_->p;
+        MPX_HIP(hipMemcpyAsync(hc, totals_, 4 * sizeof(int), hipMemcpyDeviceToHost, st_));
+        for (size_t b = 0; b < bases_.size(); ++b)
+            MPX_HIP(hipMemcpyAsync(hc + 4 + 3 * b, bases_[b]->il.counts, 3 * sizeof(int),
+                                   hipMemcpyDeviceToHost, st_));
         MPX_HIP(hipStreamSynchronize(st_));
+        bool missed = (hc[0] > 0 && !(ran & kPacked32)) || (hc[2] > 0 && !(ran & kPacked64));
+        for (size_t b = 0; b < bases_.size(); ++b) {
+            const int* c = hc + 4 + 3 * b;
+            missed = missed || (c[0] > 0 && !(ran & kSetPlain)) ||
+                     (c[2] > 0 && !(ran & kSetRefine)) || (c[1] > 0 && !(ran & kSetFp64));
+        }
+        if (missed) {
+            setError("internal: a serving round left out a kernel set with instances");
+            throw DeviceError();
+        }
+        std::fill(starting_.begin(), starting_.end(), 0);
         if (mixed_ && !fp32_gone_) {
             // FP32 kernels stay in the round graph while a running instance
             // is in its FP32 phase (activation of new ones resets this).
@@ -3150,6 +3202,11 @@ private:
     PRec<double>* rec64_ = nullptr;
     void* tiles_mem_ = nullptr;
     PinnedArray<BCtrl> h_ctrl_;
+    // A round's tile counts, then three slot counts per base (grown with
+    // the bases).
+    std::unique_ptr<PinnedArray<int>> h_cnt_;
+    size_t h_cnt_cap_ = 0;
+    std::vector<char> starting_;  // per slot: activated since the last round
     PinnedArray<BInst> h_new_;
     PinnedArray<BInitArgs> h_args_;
     PinnedArray<int> h_slots_;
