@@ -11,8 +11,13 @@ from oracle import binding as ob
 pytestmark = pytest.mark.gpu
 
 
-def _check_vs_oracle(lp, r, x, y, tag="", mixed=False):
-    """FP64: objective within 1e-6 of the oracle's (same trajectory).
+def _check_vs_oracle(lp, r, x, y, tag="", mixed=False, same_order=False):
+    """FP64: objective within 1e-6 of the oracle's (same trajectory); with
+    `same_order` (every column and row summed by one lane in storage order:
+    the lane-per-instance kernels, or packed tiles with one lane per column)
+    the iteration count is the oracle's to within one evaluation block and
+    the restart count to within one (the power iteration's last bits are
+    device reductions; measured equal on the C2 family).
     Mixed precision (FP32 iterations, FP64 evaluation and refinement) lands
     on a different point of the 1e-6 KKT region.  A 1e-6 relative KKT
     tolerance does not bound the objective error by 1e-6: the oracle itself
@@ -30,6 +35,9 @@ def _check_vs_oracle(lp, r, x, y, tag="", mixed=False):
         else:
             rel = abs(r["objective"] - ro["objective"]) / (1 + abs(ro["objective"]))
             assert rel <= 1e-6, (tag, r["objective"], ro["objective"], rel)
+            if same_order:
+                assert abs(r["iterations"] - ro["iterations"]) <= 200, (tag, r, ro["iterations"])
+                assert abs(r["restarts"] - ro["restarts"]) <= 1, (tag, r, ro["restarts"])
         certify(lp, r, x, y, tag=tag)
     return ro
 
@@ -48,7 +56,7 @@ def test_batch_shared_bounds_vs_oracle(precision):
     print("statuses", statuses, "iters", [r["iterations"] for r in res])
     for i in range(count):
         _check_vs_oracle(g.lp.with_col_bounds(lo[i], hi[i]), res[i], X[i], Y[i], f"inst{i}",
-                         mixed=precision == 1)
+                         mixed=precision == 1, same_order=True)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -74,7 +82,8 @@ def test_batch_shared_infeasible_and_one_sided_vs_oracle(precision):
     print("statuses", [r["status"] for r in res], "iters", [r["iterations"] for r in res])
     for i in range(count):
         lp = g.lp.with_col_bounds(lo[i], hi[i])
-        ro = _check_vs_oracle(lp, res[i], X[i], Y[i], f"inst{i}", mixed=precision == 1)
+        ro = _check_vs_oracle(lp, res[i], X[i], Y[i], f"inst{i}", mixed=precision == 1,
+                              same_order=True)
         if i in bad:
             assert res[i]["status"] == ro["status"] == 1 and ro["iterations"] == 2000
             # FP32 iterates pass the same test at the same check (FP64) or
@@ -87,14 +96,20 @@ def test_batch_shared_infeasible_and_one_sided_vs_oracle(precision):
             assert res[i]["status"] == 0, i
 
 
+@pytest.mark.parametrize("group", [0, 1])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batch_heterogeneous_vs_oracle(precision):
+def test_batch_heterogeneous_vs_oracle(precision, group):
+    """Independent LPs in packed tiles: each LP's own (autotuned) lane
+    groups, or one lane per column and row (spmv_group=1), whose FP64 runs
+    follow the oracle's trajectory."""
     sizes = [40, 75, 120, 200, 333, 60, 90, 150]
     gens = [lpgen.c3_instance(i, m) for i, m in enumerate(sizes)]
     lps = [g.lp for g in gens]
-    res, xs, ys = native.solve_batch(lps, precision=precision, max_iter=200000, **TIGHT)
+    res, xs, ys = native.solve_batch(lps, precision=precision, max_iter=200000,
+                                     spmv_group=group, **TIGHT)
     for i, lp in enumerate(lps):
-        _check_vs_oracle(lp, res[i], xs[i], ys[i], f"lp{i}", mixed=precision == 1)
+        _check_vs_oracle(lp, res[i], xs[i], ys[i], f"lp{i}", mixed=precision == 1,
+                         same_order=group == 1)
 
 
 def test_batch_result_same_alone_and_in_mix():
@@ -177,7 +192,7 @@ def test_batch_shared_warm_start_vs_oracle(precision):
             rel = abs(res[i]["objective"] - ro["objective"]) / (1 + abs(ro["objective"]))
             assert rel <= (5e-6 if precision else 1e-6), (i, res[i]["objective"], ro["objective"])
             if not precision:
-                assert abs(res[i]["iterations"] - ro["iterations"]) <= 0.1 * ro["iterations"], i
+                assert abs(res[i]["iterations"] - ro["iterations"]) <= 200, (i, res[i], ro)
             k = ob.kkt_check(lp, X[i], Y[i])
             assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, (i, k)
             both.append(i)

@@ -148,7 +148,10 @@ def test_c2_full_batch(precision):
         if ro["status"] == 0:
             _objective_close(res[i], ro, precision == 1, one, f"inst{i}")
             if precision == 0:
-                assert abs(res[i]["iterations"] - ro["iterations"]) <= 0.25 * ro["iterations"]
+                # One lane per instance sums each column / row in storage
+                # order: the oracle's trajectory, to within one evaluation.
+                assert abs(res[i]["iterations"] - ro["iterations"]) <= 200, (i, res[i], ro)
+                assert abs(res[i]["restarts"] - ro["restarts"]) <= 1, (i, res[i], ro)
         else:
             assert ro["iterations"] == 2000 and res[i]["iterations"] % 2000 == 0
             ray = Y[i] / np.linalg.norm(Y[i])

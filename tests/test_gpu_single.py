@@ -221,7 +221,9 @@ OPTIONAL = [
 def test_optional_features_vs_oracle(opts, group):
     """Anderson acceleration at restarts and the GPU loop's adaptive step:
     same status and objective (1e-6) as the oracle's GPU-loop solve, KKT
-    certified, iteration count tracking the oracle's."""
+    certified; with one lane per row / column the iteration count is the
+    oracle's to within one evaluation block (the Gram matrix and safeguard
+    sums are device reductions) and the restart count to within one."""
     lp = lpgen.c0_plain(seed=101, m=300, n=750).lp
     o = dict(TIGHT)
     o.update(opts)
@@ -232,7 +234,11 @@ def test_optional_features_vs_oracle(opts, group):
     assert r["status"] == ro["status"] == 0
     assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
     certify(lp, r, x, y)
-    assert abs(r["iterations"] - ro["iterations"]) <= 0.25 * ro["iterations"]
+    if group == 1:
+        assert abs(r["iterations"] - ro["iterations"]) <= 200
+        assert abs(r["restarts"] - ro["restarts"]) <= 1
+    else:
+        assert abs(r["iterations"] - ro["iterations"]) <= 0.25 * ro["iterations"]
 
 
 @pytest.mark.parametrize("N,up,down", [(200, 1.05, 0.5), (20, 3.0, 0.5), (50, 4.0, 0.25)])
