@@ -47,7 +47,22 @@ struct BCtrl {
     int32_t power_iters, g_row, g_col, use_lds;
     int32_t round_iters, skip_eval;
     int32_t refined, refine_iters;  // 1 once the instance entered kPhaseRefine; its iterations
+    int32_t pending;  // BPending: vector work of this round's decision, done in parts
+    int32_t pad;
     unsigned long long t_finish;  // device wall clock when the instance finished
+};
+
+// Vector work that a control decision leaves to the parted kernels after it
+// (kControlCopy, then kControlRecenter): restart copies, phase changes and
+// the refinement's re-centring, each column / row slice by its own part.
+enum BPending {
+    kPendNone = 0,
+    kPendPolish = 1,        // x0 = x = x^, y0 = y = y^ (FP64)
+    kPendToFp64 = 2,        // FP64 x^ = x = x0 = FP32 x^ (and y)
+    kPendToRefine = 3,      // FP64 base = FP32 x^, y^; then re-centre
+    kPendRefineToFp64 = 4,  // x = x0 = x^, y = y0 = y^ (FP64)
+    kPendRecenter = 5,      // FP64 base = x^, y^; then re-centre
+    kPendRestart = 6        // x = x0 = x^, y = y0 = y^ in the phase's precision
 };
 
 // kPhaseRefine: FP32 iterations on the correction around an FP64 base point
@@ -441,19 +456,99 @@ __device__ inline void refineRecenter(const BInst& in) {
     __syncthreads();
 }
 
-// The FP64 evaluation point of a refining instance (one workgroup), as
-// kRefineMaterialize of the single-LP path: d.xh / x0 / yh / y0 from the
-// base and the FP32 correction, snapped to the FP64 bounds and to exact
-// zero duals where the iteration put them.
-__device__ inline void refineMaterialize(const BInst& in) {
-    const int tid = threadIdx.x;
-    for (int j = tid; j < in.n; j += kBlock) {
+// Column slice [jb, je) and row slice [ib, ie) of a re-centring whose FP64
+// base (x_b in d.x, y_b in d.y) is in place: the same per-element sums and
+// roundings as refineRecenter (one thread per column / row, entries in
+// storage order), so a parted re-centring is bit-identical to it.  The FP32
+// pointer switch (f.c, f.rl, f.ru, f.q) is the control kernel's.
+__device__ inline void refineRecenterSlice(const BInst& in, int jb, int je, int ib, int ie) {
+    float* lf = const_cast<float*>(in.f.l);
+    float* uf = const_cast<float*>(in.f.u);
+    for (int j = jb + static_cast<int>(threadIdx.x); j < je; j += kBlock) {
+        double g = 0.0;
+        for (int k = in.cs[j]; k < in.cs[j + 1]; ++k) g += in.valt[k] * in.d.y[in.ri[k]];
+        const double xb = in.d.x[j];
+        in.rc[j] = static_cast<float>(in.c[j] + g);
+        lf[j] = static_cast<float>(in.l[j] - xb);
+        uf[j] = static_cast<float>(in.u[j] - xb);
+        in.f.x[j] = in.f.x0[j] = in.f.xh[j] = 0.0f;
+    }
+    for (int i = ib + static_cast<int>(threadIdx.x); i < ie; i += kBlock) {
+        double r = 0.0;
+        for (int k = in.rs[i]; k < in.rs[i + 1]; ++k) r += in.val[k] * in.d.x[in.ci[k]];
+        in.rrl[i] = static_cast<float>(in.rl[i] - r);
+        in.rru[i] = static_cast<float>(in.ru[i] - r);
+        in.rq[i] = static_cast<float>(in.d.y[i]);
+        in.f.y[i] = in.f.y0[i] = in.f.yh[i] = 0.0f;
+    }
+}
+
+// Copies of a pending decision over one column / row slice (element-wise,
+// so parts never read what another part writes).
+template <typename T>
+__device__ inline void pendingCopySlice(const BInst& in, int code, int jb, int je, int ib,
+                                        int ie) {
+    const int t0 = static_cast<int>(threadIdx.x);
+    switch (code) {
+        case kPendPolish:
+        case kPendRefineToFp64:
+            for (int j = jb + t0; j < je; j += kBlock) in.d.x0[j] = in.d.x[j] = in.d.xh[j];
+            for (int i = ib + t0; i < ie; i += kBlock) in.d.y0[i] = in.d.y[i] = in.d.yh[i];
+            break;
+        case kPendToFp64:
+            for (int j = jb + t0; j < je; j += kBlock) {
+                const double h = static_cast<double>(in.f.xh[j]);
+                in.d.xh[j] = h;
+                in.d.x[j] = h;
+                in.d.x0[j] = h;
+            }
+            for (int i = ib + t0; i < ie; i += kBlock) {
+                const double h = static_cast<double>(in.f.yh[i]);
+                in.d.yh[i] = h;
+                in.d.y[i] = h;
+                in.d.y0[i] = h;
+            }
+            break;
+        case kPendToRefine:
+            for (int j = jb + t0; j < je; j += kBlock) in.d.x[j] = static_cast<double>(in.f.xh[j]);
+            for (int i = ib + t0; i < ie; i += kBlock) in.d.y[i] = static_cast<double>(in.f.yh[i]);
+            break;
+        case kPendRecenter:
+            for (int j = jb + t0; j < je; j += kBlock) in.d.x[j] = in.d.xh[j];
+            for (int i = ib + t0; i < ie; i += kBlock) in.d.y[i] = in.d.yh[i];
+            break;
+        case kPendRestart: {
+            const BVecs<T>* vp;
+            if constexpr (sizeof(T) == 4) vp = &in.f;
+            else vp = &in.d;
+            const BVecs<T>& v = *vp;
+            for (int j = jb + t0; j < je; j += kBlock) {
+                const T h = v.xh[j];
+                v.x[j] = h;
+                v.x0[j] = h;
+            }
+            for (int i = ib + t0; i < ie; i += kBlock) {
+                const T h = v.yh[i];
+                v.y[i] = h;
+                v.y0[i] = h;
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+// The FP64 evaluation point of a refining instance over one column / row
+// slice: element-wise, as refineMaterialize.
+__device__ inline void refineMaterializeSlice(const BInst& in, int jb, int je, int ib, int ie) {
+    for (int j = jb + static_cast<int>(threadIdx.x); j < je; j += kBlock) {
         const float d = in.f.xh[j];
         const double xb = in.d.x[j];
         in.d.xh[j] = d == in.f.l[j] ? in.l[j] : (d == in.f.u[j] ? in.u[j] : xb + static_cast<double>(d));
         in.d.x0[j] = xb + static_cast<double>(in.f.x0[j]);
     }
-    for (int i = tid; i < in.m; i += kBlock) {
+    for (int i = ib + static_cast<int>(threadIdx.x); i < ie; i += kBlock) {
         const float d = in.f.yh[i];
         const bool inactive = in.rrl[i] != in.rru[i] && d == -in.rq[i];
         in.d.yh[i] = inactive ? 0.0 : in.d.y[i] + static_cast<double>(d);

@@ -13,7 +13,9 @@
 //   * kEvalScan / kEvalParts / kEvalControl evaluate every instance in
 //     size-based parts and run the reference host loop's termination,
 //     certificate, restart, primal-weight and polish decisions per instance
-//     (src/lp/pdlp.cpp:2228-2676) on the device.
+//     (src/lp/pdlp.cpp:2228-2676) on the device; kControlCopy /
+//     kControlRecenter then do the decisions' vector work (restart copies,
+//     phase changes, the refinement's re-centring) in the same parts.
 // The host reads one count of unfinished instances per round.
 //
 // Setup is on the device too: the scaling (Ruiz, Pock-Chambolle,
@@ -476,6 +478,7 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
     const bool polish = s_phase == kPhasePolish;
     const bool refining = s_phase == kPhaseRefine;
     const int ran = s_ran;
+    if (tid == 0) c.pending = kPendNone;
     // The sums of the instance's evaluation parts (kEvalParts, same round),
     // combined in part order.
     __shared__ double fs[kNumEval];
@@ -609,17 +612,17 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         __syncthreads();
         act = s_act;
     }
+    // The vector work of a decision (copies, the refinement's re-centring)
+    // is left to kControlCopy / kControlRecenter, which split every
+    // instance into the column and row slices of its evaluation parts, so
+    // a large LP's restart does not hold up the round in one workgroup.
     if (act == kActConverged) {
         const bool needPolish = o.enable_polish &&
                                 ((o.norm_inf ? e.rel_p_inf : e.rel_p) > o.polish_p_tol ||
                                  (o.norm_inf ? e.rel_d_inf : e.rel_d) > o.polish_d_tol);
-        if (needPolish) {
-            for (int j = tid; j < in.n; j += kBlock) in.d.x0[j] = in.d.x[j] = in.d.xh[j];
-            for (int i = tid; i < in.m; i += kBlock) in.d.y0[i] = in.d.y[i] = in.d.yh[i];
-        }
-        __syncthreads();
         if (tid == 0) {
             if (needPolish) {
+                c.pending = kPendPolish;
                 c.phase = kPhasePolish;
                 c.inner = 0;
                 c.pcount = 0;
@@ -637,19 +640,8 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         }
     } else if (act == kActToFp64) {
         // Continue in FP64 from the FP32 point, restarted there.
-        for (int j = tid; j < in.n; j += kBlock) {
-            const double h = static_cast<double>(in.f.xh[j]);
-            in.d.xh[j] = h;
-            in.d.x[j] = h;
-            in.d.x0[j] = h;
-        }
-        for (int i = tid; i < in.m; i += kBlock) {
-            const double h = static_cast<double>(in.f.yh[i]);
-            in.d.yh[i] = h;
-            in.d.y[i] = h;
-            in.d.y0[i] = h;
-        }
         if (tid == 0) {
+            c.pending = kPendToFp64;
             c.phase = kPhaseFp64;
             c.skip_eval = 1;
             c.inner = 0;
@@ -659,12 +651,15 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         }
     } else if (act == kActToRefine) {
         // Refinement from the FP32 point: it becomes the FP64 base and the
-        // correction starts at zero.
-        for (int j = tid; j < in.n; j += kBlock) in.d.x[j] = static_cast<double>(in.f.xh[j]);
-        for (int i = tid; i < in.m; i += kBlock) in.d.y[i] = static_cast<double>(in.f.yh[i]);
-        __syncthreads();
-        refineRecenter(in);
+        // correction starts at zero (the FP32 iteration then reads the
+        // correction data).
         if (tid == 0) {
+            c.pending = kPendToRefine;
+            BInst& w = const_cast<BInst&>(in);
+            w.f.c = in.rc;
+            w.f.rl = in.rrl;
+            w.f.ru = in.rru;
+            w.f.q = in.rq;
             c.phase = kPhaseRefine;
             c.refined = 1;
             c.skip_eval = 1;
@@ -674,9 +669,8 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
             ++c.restarts;
         }
     } else if (act == kActRefineToFp64) {
-        for (int j = tid; j < in.n; j += kBlock) in.d.x[j] = in.d.x0[j] = in.d.xh[j];
-        for (int i = tid; i < in.m; i += kBlock) in.d.y[i] = in.d.y0[i] = in.d.yh[i];
         if (tid == 0) {
+            c.pending = kPendRefineToFp64;
             c.phase = kPhaseFp64;
             c.inner = 0;
             c.fpe0 = -1.0;
@@ -685,27 +679,15 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
         }
     } else if (act == kActRestart && refining) {
         // The base moves to the evaluated point; the correction restarts.
-        for (int j = tid; j < in.n; j += kBlock) in.d.x[j] = in.d.xh[j];
-        for (int i = tid; i < in.m; i += kBlock) in.d.y[i] = in.d.yh[i];
-        __syncthreads();
-        refineRecenter(in);
         if (tid == 0) {
+            c.pending = kPendRecenter;
             c.inner = 0;
             c.fpe0 = -1.0;
             c.last_fpe = INFINITY;
         }
     } else if (act == kActRestart) {
-        for (int j = tid; j < in.n; j += kBlock) {
-            const T h = v.xh[j];
-            v.x[j] = h;
-            v.x0[j] = h;
-        }
-        for (int i = tid; i < in.m; i += kBlock) {
-            const T h = v.yh[i];
-            v.y[i] = h;
-            v.y0[i] = h;
-        }
         if (tid == 0) {
+            c.pending = kPendRestart;
             c.inner = 0;
             c.fpe0 = -1.0;
             c.last_fpe = INFINITY;
@@ -829,28 +811,77 @@ __global__ __launch_bounds__(kBlock) void kEvalControl(const BInst* __restrict__
     }
 }
 
-// FP64 evaluation points of the refining instances evaluated this round.
-__global__ __launch_bounds__(kBlock) void kRefineMaterialize(const BInst* __restrict__ insts,
-                                                             int K) {
-    if (static_cast<int>(blockIdx.x) >= K) return;
-    const BInst& in = insts[blockIdx.x];
-    const BCtrl& c = *in.ctrl;
-    if (c.phase != kPhaseRefine || !evaluatedNow<double>(c)) return;
-    refineMaterialize(in);
+// Calls f(instance, jb, je, ib, ie) for every part of this round's
+// evaluations (offsets of kEvalScan<T>), over a fixed grid: the column and
+// row slices of evalPart.
+template <typename F>
+__device__ inline void forEvalParts(const BInst* __restrict__ insts, int K,
+                                    const int* __restrict__ off, F&& f) {
+    const int total = off[K];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = K - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (off[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BInst& in = insts[lo];
+        const int p = t - off[lo], P = evalParts(in);
+        const long long cspan = (in.n + P - 1) / P, rspan = (in.m + P - 1) / P;
+        const int jb = static_cast<int>(min(static_cast<long long>(in.n), p * cspan));
+        const int je = static_cast<int>(min(static_cast<long long>(in.n), jb + cspan));
+        const int ib = static_cast<int>(min(static_cast<long long>(in.m), p * rspan));
+        const int ie = static_cast<int>(min(static_cast<long long>(in.m), ib + rspan));
+        f(in, jb, je, ib, ie);
+    }
+}
+
+// FP64 evaluation points of the refining instances evaluated this round
+// (offsets of kEvalScan<double>), slice by slice.
+__global__ __launch_bounds__(kBlock) void kRefineMaterializeParts(const BInst* __restrict__ insts,
+                                                                  int K,
+                                                                  const int* __restrict__ off) {
+    forEvalParts(insts, K, off, [&](const BInst& in, int jb, int je, int ib, int ie) {
+        if (in.ctrl->phase == kPhaseRefine) refineMaterializeSlice(in, jb, je, ib, ie);
+    });
+}
+
+// The copies of this round's decisions (kEvalControl<T>), slice by slice.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kControlCopy(const BInst* __restrict__ insts, int K,
+                                                        const int* __restrict__ off) {
+    forEvalParts(insts, K, off, [&](const BInst& in, int jb, int je, int ib, int ie) {
+        const int code = in.ctrl->pending;
+        if (code != kPendNone) pendingCopySlice<T>(in, code, jb, je, ib, ie);
+    });
+}
+
+// The re-centrings of this round's decisions, once their FP64 bases are in
+// place (kControlCopy), slice by slice.
+__global__ __launch_bounds__(kBlock) void kControlRecenter(const BInst* __restrict__ insts, int K,
+                                                           const int* __restrict__ off) {
+    forEvalParts(insts, K, off, [&](const BInst& in, int jb, int je, int ib, int ie) {
+        const int code = in.ctrl->pending;
+        if (code == kPendToRefine || code == kPendRecenter)
+            refineRecenterSlice(in, jb, je, ib, ie);
+    });
 }
 
 // The evaluation and control of one round for the instances in precision T:
-// part offsets, the parts, then the per-instance control.  The FP64
-// evaluation also covers refining instances, at their materialized points.
+// part offsets, the parts, the per-instance control, then the vector work
+// of its decisions in the same parts.  The FP64 evaluation also covers
+// refining instances, at their materialized points.
 template <typename T>
 void enqueueEval(hipStream_t st, int grid, const BInst* di, int K, const BOptions& bo, int* off,
                  double* parts) {
-    if constexpr (std::is_same_v<T, double>) {
-        if (bo.refine) kRefineMaterialize<<<K, kBlock, 0, st>>>(di, K);
-    }
     kEvalScan<T><<<1, 1024, 0, st>>>(di, K, off);
+    if constexpr (std::is_same_v<T, double>) {
+        if (bo.refine) kRefineMaterializeParts<<<grid, kBlock, 0, st>>>(di, K, off);
+    }
     kEvalParts<T><<<grid, kBlock, 0, st>>>(di, K, off, parts);
     kEvalControl<T><<<K, kBlock, 0, st>>>(di, K, bo, parts);
+    kControlCopy<T><<<grid, kBlock, 0, st>>>(di, K, off);
+    if (bo.refine) kControlRecenter<<<grid, kBlock, 0, st>>>(di, K, off);
 }
 
 // Counts unfinished instances (one thread per instance, sum by atomics on an
