@@ -230,7 +230,7 @@ REFINED = dict(primal_tol=1e-8, dual_tol=1e-8, optimality_tol=1e-8, verbose=0)
 
 def test_mixed_refined_to_1e8_matches_oracle():
     """Mixed precision refined to 1e-8 (FP32 iterations, FP64 phase to
-    refine_tol; BASELINE config 1's "final refinement to 1e-8") against the
+    refine_tol, the final refinement to 1e-8 of the large-LP workload) against the
     oracle solved to 1e-8: objective within 1e-6 of the oracle's and of the
     exact optimum, the returned point certified at 1e-8, on the single-LP
     path, a shared-matrix batch (lane-per-instance kernels) and a packed

@@ -111,7 +111,8 @@ C2_INFEASIBLE = {7: 3, 1000: 17, 2049: 101, 4000: 1200}
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("precision", [0, 1])
 def test_c2_full_batch(precision):
-    """BASELINE config 2 at full size: 4,096 bounds variants of one A in one
+    """The node-relaxation workload at full size: 4,096 bounds variants of
+    one A (m = 3,000, n = 8,000, 6 entries per column) in one
     batched solve.  Every Optimal instance is certified; four variants made
     infeasible (farkas_bounds) come back Infeasible; 16 seeded instances and
     the infeasible ones are compared with the oracle."""
@@ -170,7 +171,7 @@ def _c3_subset(k=32):
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("precision", [0, 1])
 def test_c3_stratified_subset(precision):
-    """32 of BASELINE config 3's 1,024 LPs, one per size stratum (the
+    """32 of the heterogeneous workload's 1,024 LPs, one per size stratum (the
     smallest and largest included), in one packed batch: every LP certified;
     those with m <= 2,500 (the oracle solves them in seconds) compared with
     the oracle's objective."""
@@ -195,7 +196,7 @@ def test_c3_stratified_subset(precision):
 
 @pytest.mark.timeout(900)
 def test_c4_poisson_stream_certified_and_same_alone():
-    """BASELINE config 4's serving mix for 10 s of arrivals at 200 requests/s
+    """The serving mix for 10 s of Poisson arrivals at 200 requests/s
     (70% bounds variants of the C2 matrix, 30% fresh LPs with lognormal
     nnz), submitted at their arrival times into one server (mixed
     precision): every request finishes Optimal and certified; six variants
