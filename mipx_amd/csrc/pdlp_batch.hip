@@ -891,14 +891,23 @@ __global__ void kCountActive(const BInst* __restrict__ insts, int count, int* __
     if (t < count && insts[t].ctrl->phase != kPhaseDone) atomicAdd(out, 1);
 }
 
-// Marks unfinished instances with `status` (time limit / cancellation).
+// Ends an unfinished instance at a time limit or cancellation: `status`
+// (TIME_LIMIT / ITER_LIMIT), except that an instance in its polish phase
+// has converged and ends OPTIMAL at its last evaluated point, as the
+// reference's polish loop does when the clock or the stop flag breaks it
+// (pdlp.cpp:1473-1478, 2545-2550 and the converged branch of 1677-1706).
+__device__ inline void abortInstance(BCtrl& c, int status) {
+    if (c.phase == kPhaseDone) return;
+    c.status = c.phase == kPhasePolish ? MPX_STATUS_OPTIMAL : status;
+    c.phase = kPhaseDone;
+    c.round_iters = 0;
+    stampDone(c);
+}
+
+// Ends every unfinished instance (time limit / cancellation of a batch).
 __global__ void kAbortActive(const BInst* __restrict__ insts, int count, int status) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t < count && insts[t].ctrl->phase != kPhaseDone) {
-        insts[t].ctrl->phase = kPhaseDone;
-        insts[t].ctrl->status = status;
-        stampDone(*insts[t].ctrl);
-    }
+    if (t < count) abortInstance(*insts[t].ctrl, status);
 }
 
 // Per-instance initialisation: scaled column bounds (its own bounds when the
@@ -2430,14 +2439,7 @@ __global__ void kPlaceInsts(const BInst* __restrict__ src, const int* __restrict
     if (t < count) table[slots[t]] = src[t];
 }
 
-__global__ void kAbortSlot(BCtrl* __restrict__ c, int status) {
-    if (c->phase != kPhaseDone) {
-        c->phase = kPhaseDone;
-        c->status = status;
-        c->round_iters = 0;
-        stampDone(*c);
-    }
-}
+__global__ void kAbortSlot(BCtrl* __restrict__ c, int status) { abortInstance(*c, status); }
 
 // An LP the server holds a copy of: a registered base, or a fresh request.
 struct ServeLp {
