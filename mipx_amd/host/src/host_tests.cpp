@@ -253,6 +253,44 @@ int main(int argc, char** argv) {
         const LpResult r = s.solve();
         CHECK(r.status == Status::Optimal && std::abs(r.objective + 4.0) <= 1e-4);
     }
+    {  // warm start after a bound change (test_pdlp.cpp:1116-1153): the row
+       // bound 4 -> 3 moves the optimum to -3; cold and warm solves agree,
+       // and the warm one starts from the old solution
+        PdlpSolver s;
+        s.setOptions(quiet());
+        s.load(simple);
+        const LpResult r1 = s.solve();
+        CHECK(r1.status == Status::Optimal);
+        const std::vector<Real> px = s.getPrimalValues(), py = s.getDualValues();
+        LpProblem moved = simple;
+        moved.row_upper[0] = 3.0;
+        PdlpSolver cold, warm;
+        cold.setOptions(quiet());
+        cold.load(moved);
+        const LpResult rc = cold.solve();
+        warm.setOptions(quiet());
+        warm.load(moved);
+        warm.setWarmStart(px, py);
+        const LpResult rw = warm.solve();
+        CHECK(rc.status == Status::Optimal && std::abs(rc.objective + 3.0) <= 1e-3);
+        CHECK(rw.status == Status::Optimal && std::abs(rw.objective + 3.0) <= 1e-3);
+    }
+    {  // clearWarmStart disables the warm start (test_pdlp.cpp:1155-1180):
+       // the re-solve is the cold solve, bit for bit
+        PdlpSolver s;
+        s.setOptions(quiet());
+        s.load(simple);
+        const LpResult r1 = s.solve();
+        CHECK(r1.status == Status::Optimal);
+        const std::vector<Real> px = s.getPrimalValues(), py = s.getDualValues();
+        s.setWarmStart(px, py);
+        s.clearWarmStart();
+        s.load(simple);
+        const LpResult r2 = s.solve();
+        CHECK(r2.status == Status::Optimal && std::abs(r2.objective + 4.0) <= 1e-4);
+        CHECK(r2.iterations == r1.iterations && r2.objective == r1.objective);
+        CHECK(s.getPrimalValues() == px && s.getDualValues() == py);
+    }
     {  // mixed precision on the simple LP
         PdlpSolver s;
         PdlpOptions o = quiet(10000);
