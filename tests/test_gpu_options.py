@@ -166,3 +166,38 @@ def test_polishing_respects_time_limit_single_batch_and_server():
     el = time.perf_counter() - t0
     srv.close()
     ok(got[0], got[1], got[2], el, "server")
+
+
+MIXED_OPTIONS = ["no_polish", "tight_polish", "loose_main_then_polish", "no_rescaling",
+                 "no_scaling", "linf", "fixed_primal_weight", "eval_every_50", "restart_rules",
+                 "pid_gains"]
+
+
+@pytest.mark.parametrize("opt", MIXED_OPTIONS)
+def test_mixed_precision_options_certified(opt):
+    """The option sets in mixed precision (FP32 iterations, FP64 evaluation
+    and refinement) on the single-LP path, a packed batch and a batch of
+    bounds variants: the oracle's status, the objective within the 5e-6
+    envelope of two points of the 1e-6 KKT region, every point certified
+    (L2 norms) at the tolerance."""
+    o = _opts(opt)
+    lps = [lpgen.c0_plain(seed=101, m=300, n=750).lp] + [
+        lpgen.c3_instance(i, m).lp for i, m in enumerate([60, 150, 240])]
+    runs = [(lps[0],) + native.solve(lps[0], precision=1, **o)]
+    res, xs, ys = native.solve_batch(lps[1:], precision=1, **o)
+    runs += list(zip(lps[1:], res, xs, ys))
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    lo = np.empty((4, 800))
+    hi = np.empty((4, 800))
+    for i in range(4):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    res, X, Y = native.solve_batch_shared(g.lp, lo, hi, precision=1, **o)
+    runs += [(g.lp.with_col_bounds(lo[i], hi[i]), res[i], X[i], Y[i]) for i in range(4)]
+    for k, (lp, r, x, y) in enumerate(runs):
+        ro, _, _ = ob.solve(lp, **o)
+        tag = f"run{k}/{opt}"
+        assert r["status"] == ro["status"] == 0, (tag, r, ro)
+        assert r["iterations_fp32"] > 0, (tag, r)
+        assert abs(r["objective"] - ro["objective"]) <= 5e-6 * (1 + abs(ro["objective"])), (tag, r, ro)
+        if o.get("optimality_norm", 0) != 1:
+            certify(lp, r, x, y, tol=max(o["primal_tol"], o["optimality_tol"]), tag=tag)
