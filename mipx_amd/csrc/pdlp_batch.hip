@@ -480,18 +480,26 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
     const int ran = s_ran;
     if (tid == 0) c.pending = kPendNone;
     // The sums of the instance's evaluation parts (kEvalParts, same round),
-    // combined in part order.
+    // combined in part order: the workgroup stages all parts in LDS with one
+    // coalesced pass, then one thread per field adds (or takes the maximum
+    // of) that field over the parts in order.  A large LP has up to
+    // kEvalMaxParts parts; one thread loading them field by field was a
+    // chain of P * kNumEval dependent global loads.
     __shared__ double fs[kNumEval];
-    if (tid == 0) {
-        const int P = evalParts(in);
-        for (int f = 0; f < kNumEval; ++f) fs[f] = parts[f];
-        for (int p = 1; p < P; ++p)
-            for (int f = 0; f < kNumEval; ++f) {
-                const double x = parts[p * kNumEval + f];
-                const bool mx = f == static_cast<int>(kDmax) ||
-                                f == static_cast<int>(kNumColFields) + static_cast<int>(kPmax);
-                fs[f] = mx ? maxRef(fs[f], x) : fs[f] + x;
-            }
+    __shared__ double staged[kEvalMaxParts * kNumEval];
+    const int P = evalParts(in);
+    for (int t = tid; t < P * kNumEval; t += kBlock) staged[t] = parts[t];
+    __syncthreads();
+    if (tid < kNumEval) {
+        const int f = tid;
+        const bool mx = f == static_cast<int>(kDmax) ||
+                        f == static_cast<int>(kNumColFields) + static_cast<int>(kPmax);
+        double a = staged[f];
+        for (int p = 1; p < P; ++p) {
+            const double x = staged[p * kNumEval + f];
+            a = mx ? maxRef(a, x) : a + x;
+        }
+        fs[f] = a;
     }
     __syncthreads();
     const BEval e = bEvalFinish(fs, c);
