@@ -48,7 +48,7 @@ struct SuPower {
 };
 
 constexpr int kSuEntries = 4 * kBlock;  // entries per entry tile
-constexpr int kPowerGroup = 16;         // power steps per captured group
+constexpr int kPowerGroup = 16;         // power steps between host checks
 
 struct SuArgs {
     BOwner* owners;
@@ -546,11 +546,10 @@ public:
         a_.maxT = maxT_;
     }
 
-    // Uploads the cohort tables and runs the whole setup on `st`; the power
-    // iteration is captured in graphs only if `graphs` is true (the engines
-    // launch directly: sixteen launches per host check cost about what a
-    // graph launch does, and no two host threads ever capture at once).
-    void run(hipStream_t st, bool graphs = true) {
+    // Uploads the cohort tables and runs the whole setup on `st`, launching
+    // every kernel directly (sixteen power steps per host check cost about
+    // what one graph launch of them does).
+    void run(hipStream_t st) {
         const int k = count();
         if (k == 0) return;
         auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -583,41 +582,28 @@ public:
         kSuRescale<<<gAll, kBlock, 0, st>>>(a_);
         kPwInit<<<ceilDiv(k, kBlock), kBlock, 0, st>>>(a_, k);
         MPX_HIP(hipGetLastError());
-        // Power iteration in captured groups; the host checks after each.
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
+        // Power iteration in groups of kPowerGroup steps; the host checks
+        // after each.
         SuArgs pa = a_;  // tile lists of the power steps (compacted below)
-        bool graphed =
-            graphs && !graphsDisabled() &&
-            hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
-        if (graphed) {
-            powerGroup(st, k, pa);
-            graphed = hipStreamEndCapture(st, &g) == hipSuccess && g != nullptr &&
-                      hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
-            if (!graphed) (void)hipGetLastError();
-        }
         int left = 1, running = k;
         int maxIter = 0;
         for (const BOwner& w : owners_) maxIter = std::max(maxIter, w.sv_max_iter);
         std::vector<SuPower> hp;
         for (int done = 0; left > 0 && done <= maxIter + kPowerGroup; done += kPowerGroup) {
-            if (graphed) MPX_HIP(hipGraphLaunch(ge, st));
-            else powerGroup(st, k, pa);
+            powerGroup(st, k, pa);
             kPwCount<<<1, kBlock, 0, st>>>(pa, k);
             MPX_HIP(hipMemcpyAsync(&left, a_.remaining, sizeof(int), hipMemcpyDeviceToHost, st));
             MPX_HIP(hipStreamSynchronize(st));
             // Once at most half of the listed matrices still iterate, the
             // power steps get tile lists of those alone (same tiles in the
             // same order per matrix, so every matrix's sums are unchanged).
-            if (!graphed && left > 0 && 2 * left <= running) {
+            if (left > 0 && 2 * left <= running) {
                 hp.resize(static_cast<size_t>(k));
                 MPX_HIP(hipMemcpyAsync(hp.data(), a_.pw, sizeof(SuPower) * k, hipMemcpyDeviceToHost, st));
                 MPX_HIP(hipStreamSynchronize(st));
                 running = compactPower(st, k, hp, pa);
             }
         }
-        if (ge) (void)hipGraphExecDestroy(ge);
-        if (g) (void)hipGraphDestroy(g);
         kPwFinish<<<ceilDiv(k, kBlock), kBlock, 0, st>>>(a_, k);
         MPX_HIP(hipGetLastError());
         MPX_HIP(hipStreamSynchronize(st));

@@ -1872,7 +1872,7 @@ void BatchEngine::prepare() {
     rounds_ = 0;
     fp32_gone_ = false;
     next_known_ = false;
-    if (no) cohort_->run(st_, false);
+    if (no) cohort_->run(st_);
     if (ni)
         kInstInit<<<ni, kBlock, 0, st_>>>(static_cast<const BInst*>(d_insts_buf_->p),
                                           static_cast<const BInitArgs*>(d_args_buf_->p), bo_,
@@ -2625,7 +2625,7 @@ public:
             throw OutOfMemory();
         }
         cohort.carve(static_cast<char*>(scratch));
-        cohort.run(st_, false);
+        cohort.run(st_);
         pool_->release(scratch);
         bases_.push_back(std::move(b));
         // The round graphs include every base's lane-per-instance kernels.
@@ -2869,7 +2869,7 @@ private:
                 try {
                     for (const auto& u : job->uploads)
                         uploadOwner(u.first->lp, *u.second, u.first->owner, ws);
-                    job->cohort->run(ws, false);
+                    job->cohort->run(ws);
                 } catch (const DeviceError&) {
                     good = false;
                     job->error = lastError();
