@@ -170,8 +170,7 @@ def test_solve_mixed_precision(seed, m, n):
     assert r["status"] == 0 and r["iterations_fp32"] > 0
     assert abs(r["objective"] - opt) <= 3e-6 * (1 + abs(opt))
     assert abs(r["objective"] - ro["objective"]) <= 5e-6 * (1 + abs(ro["objective"]))
-    k = ob.kkt_check(lp, x, y)
-    assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
+    certify(lp, r, x, y)
 
 
 def test_medium_lp_gpu_matches_cpu():
@@ -269,8 +268,7 @@ def test_adaptive_and_anderson_mixed_precision():
     print(r)
     assert r["status"] == ro["status"] == 0 and r["iterations_fp32"] > 0
     assert abs(r["objective"] - ro["objective"]) <= 5e-6 * (1 + abs(ro["objective"]))
-    k = ob.kkt_check(lp, x, y)
-    assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
+    certify(lp, r, x, y)
 
 
 def test_matrix_zero_tol_on_gpu():
@@ -362,8 +360,7 @@ def test_chunked_dual_spmv(precision):
     assert abs(r1["objective"] - ro["objective"]) <= tol * (1 + abs(ro["objective"]))
     assert abs(r1["objective"] - r0["objective"]) <= tol * (1 + abs(r0["objective"]))
     assert abs(r1["iterations"] - r0["iterations"]) <= 0.25 * r0["iterations"]
-    k = ob.kkt_check(lp, x1, y1)
-    assert k["rel_primal"] <= 1.01e-6 and k["rel_gap"] <= 2e-6, k
+    certify(lp, r1, x1, y1)
 
 
 def test_preconditioner_refresh_single_path():
