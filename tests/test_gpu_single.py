@@ -363,6 +363,22 @@ def test_chunked_dual_spmv(precision):
     certify(lp, r1, x1, y1)
 
 
+def test_chunked_dual_one_lane_follows_oracle():
+    """With one lane per row the chunked A x_r continues each row's running
+    sum chunk after chunk, the reference's order: the FP64 solve follows the
+    oracle's trajectory (iterations within one evaluation block, restarts
+    within one), as the one-launch SpMV does."""
+    lp = lpgen.c0_plain(seed=0xC0, m=2000, n=5000).lp
+    ro, _, _ = ob.solve(lp, **TIGHT)
+    for chunk in (1300, -1):
+        r, x, y = native.solve(lp, spmv_group=1, spmv_chunk_cols=chunk, **TIGHT)
+        assert r["status"] == ro["status"] == 0, (chunk, r)
+        assert abs(r["iterations"] - ro["iterations"]) <= 200, (chunk, r["iterations"], ro)
+        assert abs(r["restarts"] - ro["restarts"]) <= 1, (chunk, r, ro)
+        assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
+        certify(lp, r, x, y)
+
+
 def test_preconditioner_refresh_single_path():
     """pdlp.cpp:1325-1340 / 713-742: a refresh recomputes sigma/tau from the
     unchanged scaled matrix, which leaves them as they were (pinned against
