@@ -539,9 +539,9 @@ def main():
         return
     log(f"building workload {a.workload} count={a.count}")
     lps = None
-    # --profile-only: six rounds are enough for the columns-per-wave
-    # autotuning that precedes the profiled launches.
-    cap = dict(max_iter=6 * 200) if a.profile_only else {}
+    # --profile-only: eleven rounds cover the columns-per-wave autotuning
+    # (two passes over five candidates) that precedes the profiled launches.
+    cap = dict(max_iter=11 * 200) if a.profile_only else {}
     if a.handover > 0:
         cap["mixed_handover"] = a.handover
     if a.spmv_group > 0:
@@ -645,7 +645,7 @@ def main():
     if prec == 1 and lps is None:
         # The FP64 primal kernel of the same instances, timed the same way
         # (an FP64 batch, every instance active).
-        b64 = native.ResidentBatch(lp, lo, hi, precision=0, max_iter=6 * N, **TOL)
+        b64 = native.ResidentBatch(lp, lo, hi, precision=0, max_iter=11 * N, **TOL)
         b64.run()
         p64 = b64.profile(launches=50)
         b64.close()
