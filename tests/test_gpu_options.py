@@ -201,3 +201,32 @@ def test_mixed_precision_options_certified(opt):
         assert abs(r["objective"] - ro["objective"]) <= 5e-6 * (1 + abs(ro["objective"])), (tag, r, ro)
         if o.get("optimality_norm", 0) != 1:
             certify(lp, r, x, y, tol=max(o["primal_tol"], o["optimality_tol"]), tag=tag)
+
+
+def test_mixed_precision_handover_to_fp64():
+    """mixed_refine = 0: the FP32 phase hands over to plain FP64 PDHG (the
+    FP32 point becomes the FP64 iterate, restarted there) instead of the
+    refinement, on the single-LP path, a packed batch and a batch of bounds
+    variants: both phases run, the result has the oracle's status, is
+    certified and lies within the 5e-6 objective envelope."""
+    o = dict(TIGHT, max_iter=500000, precision=1, mixed_refine=0)
+    lps = [lpgen.c0_plain(seed=101, m=300, n=750).lp] + [
+        lpgen.c3_instance(i, m).lp for i, m in enumerate([60, 150, 240])]
+    runs = [(lps[0],) + native.solve(lps[0], **o)]
+    res, xs, ys = native.solve_batch(lps[1:], **o)
+    runs += list(zip(lps[1:], res, xs, ys))
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    lo = np.empty((4, 800))
+    hi = np.empty((4, 800))
+    for i in range(4):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    res, X, Y = native.solve_batch_shared(g.lp, lo, hi, **o)
+    runs += [(g.lp.with_col_bounds(lo[i], hi[i]), res[i], X[i], Y[i]) for i in range(4)]
+    for k, (lp, r, x, y) in enumerate(runs):
+        ro, _, _ = ob.solve(lp, **TIGHT)
+        tag = f"run{k}"
+        assert r["status"] == ro["status"] == 0, (tag, r, ro)
+        assert 0 < r["iterations_fp32"] < r["iterations"], (tag, r)
+        assert r["iterations_refine"] == 0, (tag, r)
+        assert abs(r["objective"] - ro["objective"]) <= 5e-6 * (1 + abs(ro["objective"])), (tag, r, ro)
+        certify(lp, r, x, y, tag=tag)
