@@ -230,3 +230,56 @@ def test_mixed_precision_handover_to_fp64():
         assert r["iterations_refine"] == 0, (tag, r)
         assert abs(r["objective"] - ro["objective"]) <= 5e-6 * (1 + abs(ro["objective"])), (tag, r, ro)
         certify(lp, r, x, y, tag=tag)
+
+
+def _same(a, b, tag):
+    """Bit-identical results, the wall-clock fields aside."""
+    (ra, xa, ya), (rb, xb, yb) = a, b
+    for k in ra:
+        if "seconds" not in k:
+            assert ra[k] == rb[k], (tag, k, ra[k], rb[k])
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb), tag
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_identical_options_identical_runs(precision):
+    """tests/test_pdlp.cpp:1310-1340: Anderson acceleration switched off
+    explicitly and left at its default give the same iterations and
+    objective.  The device holds this bit for bit, and for any repeat of a
+    solve: every reduction runs in a fixed order, without atomics, and the
+    lane-per-instance autotuner's choice of columns per wave leaves each
+    lane's sums unchanged.  Single LP, packed batch, bounds variants (enough
+    rounds for the autotuner to run) and the server, each solved twice."""
+    base = dict(TIGHT, max_iter=500000, precision=precision)
+    lp = lpgen.c0_plain(seed=101, m=300, n=750).lp
+    one = native.solve(lp, **base)
+    _same(one, native.solve(lp, anderson_acceleration=0, **base), "single/explicit-off")
+    _same(one, native.solve(lp, **base), "single/repeat")
+    lps = [lpgen.c3_instance(i, m).lp for i, m in enumerate([60, 150, 240])] + [medium_lp()]
+    a = native.solve_batch(lps, **base)
+    b = native.solve_batch(lps, anderson_acceleration=0, **base)
+    for i in range(len(lps)):
+        _same((a[0][i], a[1][i], a[2][i]), (b[0][i], b[1][i], b[2][i]), f"packed{i}")
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    count = 70
+    lo = np.empty((count, 800))
+    hi = np.empty((count, 800))
+    for i in range(count):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    a = native.solve_batch_shared(g.lp, lo, hi, **base)
+    b = native.solve_batch_shared(g.lp, lo, hi, **base)
+    for i in range(count):
+        _same((a[0][i], a[1][i], a[2][i]), (b[0][i], b[1][i], b[2][i]), f"shared{i}")
+    srv_res = []
+    for _ in range(2):
+        srv = native.Server(hbm_budget_bytes=2e9, max_slots=8, **base)
+        tk = srv.submit(lp)
+        got = None
+        while got is None:
+            srv.step()
+            for t in srv.poll():
+                if t == tk:
+                    got = srv.result(tk)
+        srv.close()
+        srv_res.append(got)
+    _same(tuple(srv_res[0]), tuple(srv_res[1]), "server/repeat")
