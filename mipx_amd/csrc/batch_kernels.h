@@ -1,22 +1,19 @@
 // batch_kernels.h -- batched PDLP: per-instance state, evaluation sums and
-// certificates (the control logic runs one workgroup per LP instance).
+// certificates.
 //
 // Many LPs (branch-and-bound node relaxations sharing one A, or independent
-// LPs of uneven size) run in one launch; each 256-thread workgroup owns one
-// instance and runs its whole restarted-Halpern PDHG loop: iterations, the
-// FP64 evaluation every N iterations, the restart / PI primal-weight /
-// termination / certificate decisions (the reference's host logic,
-// src/lp/pdlp.cpp:2228-2469, executed by thread 0 in FP64) and the polish
-// phase.  Instances never share mutable state, so an instance's result does
-// not depend on what else is in the launch.
+// LPs of uneven size) are iterated together: the iteration kernels
+// (packed tiles in pdlp_batch.hip, lane-per-instance groups in
+// batch_interleaved.h) run N iterations per round, then every instance is
+// evaluated in FP64 in size-based parts and its restart / PI primal-weight /
+// termination / certificate / polish decisions (the reference's host logic,
+// src/lp/pdlp.cpp:2228-2469) run on the device in one workgroup per
+// instance.  Instances never share mutable state, so an instance's result
+// does not depend on what else is in the launch.
 //
 // Precision: iterations in T (float for MPX_PREC_MIXED, with float
 // accumulation), evaluation always in FP64 against the FP64 scaled matrix.
-// A launch runs at most `budget` iterations per instance and returns; state
-// stays resident in HBM between launches (hipGraph-captured rounds).
-//
-// When y (m) and x_r (n) fit the workgroup's LDS they are kept there for the
-// whole block of iterations, so both SpMV gathers hit LDS instead of L2/HBM.
+// State stays resident in HBM between rounds (hipGraph-captured rounds).
 #pragma once
 
 #include "device_common.h"
@@ -44,7 +41,7 @@ struct BCtrl {
     // last evaluation
     double rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj;
     int32_t inner, total, pcount, phase, status, restarts, fp32_iters, stall;
-    int32_t power_iters, g_row, g_col, use_lds;
+    int32_t power_iters;
     int32_t round_iters, skip_eval;
     int32_t refined, refine_iters;  // 1 once the instance entered kPhaseRefine; its iterations
     int32_t pending;  // BPending: vector work of this round's decision, done in parts
@@ -94,7 +91,7 @@ struct BVecs {
 
 struct BInst {
     int32_t m, n, nnz, sense_max;
-    int32_t preset, use_lds, owner;
+    int32_t preset, owner;
     int32_t il;  // iterated by the lane-per-instance kernels of its shared matrix
     const int *rs, *ci, *cs, *ri;
     // original bounds / objective (certificates)

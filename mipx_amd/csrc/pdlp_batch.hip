@@ -1005,9 +1005,6 @@ __global__ __launch_bounds__(kBlock) void kInstInit(const BInst* __restrict__ in
         c.status = MPX_STATUS_ITER_LIMIT;
         c.restarts = c.fp32_iters = c.stall = 0;
         c.power_iters = static_cast<int>(a.scal[5]);
-        c.g_col = presetGC(in.preset);
-        c.g_row = presetGR(in.preset);
-        c.use_lds = 0;
         c.skip_eval = 0;
         c.t_finish = 0ull;
         c.round_iters = o.max_iter < o.N ? o.max_iter : o.N;
