@@ -168,14 +168,41 @@ struct BEval {
     double fpe, rel_p, rel_d, rel_p_inf, rel_d_inf, gap, pobj, dobj, dx2, dy2;
 };
 
+// Elements of the LDS stage of a staged evaluation (kStage): y^ and y0 for
+// the column pass (2m), then x^ for the row pass (n).
+constexpr int kEvalStageElems = 8192;
+
+// The one LDS stage of a kernel's staged evaluations (a single array for all
+// lane-group presets).
+template <typename T>
+__device__ inline T* evalStage() {
+    __shared__ T stage[kEvalStageElems];
+    return stage;
+}
+
 // FP64 evaluation sums of (xh, yh) against (x0, y0) over the columns
 // [jb, je) and rows [ib, ie), same formulas as the single path
 // (src/lp/pdlp.cpp:1076-1202): the kNumColFields column fields then the
 // kNumRowFields row fields, stored to dst by thread 0.
-template <int GC, int GR, typename T>
+//
+// kStage (whole evaluations with 2m and n at most kEvalStageElems): the
+// gathered vectors are first copied into LDS, y^ and y0 for the column
+// pass, then x^ for the row pass, and gathered from there.  With thousands
+// of instances in flight their gathered vectors do not stay in L2; the
+// values, and so the fields, are the same.
+template <int GC, int GR, typename T, bool kStage = false>
 __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCtrl& c,
                                    double* dst, int jb, int je, int ib, int ie) {
     const int tid = threadIdx.x;
+    T* stage = kStage ? evalStage<T>() : nullptr;
+    if constexpr (kStage) {
+        __syncthreads();
+        for (int e = tid; e < in.m; e += kBlock) {
+            stage[e] = v.yh[e];
+            stage[in.m + e] = v.y0[e];
+        }
+        __syncthreads();
+    }
     // Each lane group takes kEvalCW columns (rows) per pass, PC (PR) apart,
     // with their loads issued together: a thread still visits its columns in
     // the order grp, grp + PC, grp + 2 PC, ... and sums each column's entries
@@ -187,6 +214,9 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
     for (int base = jb; base < je; base += kEvalCW * PC) {
         int j[kEvalCW], qb[kEvalCW], qe[kEvalCW];
         double aty[kEvalCW], atdy[kEvalCW], xj[kEvalCW], x0j[kEvalCW];
+        // The column's own operands, loaded with its first loads (lane 0 of
+        // the group uses them) instead of after its sums.
+        double cj[kEvalCW], lbj[kEvalCW], ubj[kEvalCW], csj[kEvalCW];
         int most = 0;
 #pragma unroll
         for (int w = 0; w < kEvalCW; ++w) {
@@ -196,6 +226,11 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
             qe[w] = ok ? in.cs[j[w] + 1] : 0;
             xj[w] = ok ? static_cast<double>(v.xh[j[w]]) : 0.0;
             x0j[w] = ok ? static_cast<double>(v.x0[j[w]]) : 0.0;
+            const bool own = ok && lc == 0;
+            cj[w] = own ? in.c[j[w]] : 0.0;
+            lbj[w] = own ? in.l[j[w]] : 0.0;
+            ubj[w] = own ? in.u[j[w]] : 0.0;
+            csj[w] = own ? in.col_scale[j[w]] : 1.0;
             const int cnt = qe[w] > qb[w] ? (qe[w] - qb[w] + GC - 1) / GC : 0;
             most = cnt > most ? cnt : most;
             aty[w] = atdy[w] = 0.0;
@@ -207,9 +242,10 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
                 if (q < qe[w]) {
                     const int i = in.ri[q];
                     const double a = in.valt[q];
-                    const double yi = static_cast<double>(v.yh[i]);
+                    const double yi = static_cast<double>(kStage ? stage[i] : v.yh[i]);
+                    const double y0i = static_cast<double>(kStage ? stage[in.m + i] : v.y0[i]);
                     aty[w] += a * yi;
-                    atdy[w] += a * (yi - static_cast<double>(v.y0[i]));
+                    atdy[w] += a * (yi - y0i);
                 }
             }
         }
@@ -218,12 +254,11 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
             aty[w] = groupSum<GC>(aty[w]);
             atdy[w] = groupSum<GC>(atdy[w]);
             if (lc == 0 && j[w] < je) {
-                const int jj = j[w];
                 const double dx = xj[w] - x0j[w];
                 cf[kDx2] += dx * dx;
                 cf[kInter] += atdy[w] * dx;
-                const double g = in.c[jj] + aty[w];
-                const double lb = in.l[jj], ub = in.u[jj];
+                const double g = cj[w] + aty[w];
+                const double lb = lbj[w], ub = ubj[w];
                 // Activity against the bounds the iteration projected onto.
                 const double lbi = static_cast<double>(static_cast<T>(lb));
                 const double ubi = static_cast<double>(static_cast<T>(ub));
@@ -231,10 +266,10 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
                 if (xj[w] <= lbi + 1e-12 && isfinite(lb)) dr = minRef(g, 0.0);
                 else if (xj[w] >= ubi - 1e-12 && isfinite(ub)) dr = maxRef(g, 0.0);
                 else dr = g;
-                const double d = dr / (c.os * in.col_scale[jj]);
+                const double d = dr / (c.os * csj[w]);
                 cf[kDsq] += d * d;
                 cf[kDmax] = maxRef(cf[kDmax], fabs(d));
-                cf[kPobj] += in.c[jj] * xj[w];
+                cf[kPobj] += cj[w] * xj[w];
                 double gp = g;
                 if (!isfinite(lb)) gp = minRef(gp, 0.0);
                 if (!isfinite(ub)) gp = maxRef(gp, 0.0);
@@ -243,11 +278,16 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
             }
         }
     }
+    if constexpr (kStage) {
+        __syncthreads();
+        for (int e = tid; e < in.n; e += kBlock) stage[e] = v.xh[e];
+        __syncthreads();
+    }
     const int lr = tid % GR, gr = tid / GR;
     double rf[kNumRowFields] = {0, 0, 0, 0};
     for (int base = ib; base < ie; base += kEvalCW * PR) {
         int i[kEvalCW], qb[kEvalCW], qe[kEvalCW];
-        double ax[kEvalCW], yv[kEvalCW], y0v[kEvalCW];
+        double ax[kEvalCW], yv[kEvalCW], y0v[kEvalCW], loi[kEvalCW], hii[kEvalCW], rsi[kEvalCW];
         int most = 0;
 #pragma unroll
         for (int w = 0; w < kEvalCW; ++w) {
@@ -257,6 +297,10 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
             qe[w] = ok ? in.rs[i[w] + 1] : 0;
             yv[w] = ok ? static_cast<double>(v.yh[i[w]]) : 0.0;
             y0v[w] = ok ? static_cast<double>(v.y0[i[w]]) : 0.0;
+            const bool own = ok && lr == 0;
+            loi[w] = own ? in.rl[i[w]] : 0.0;
+            hii[w] = own ? in.ru[i[w]] : 0.0;
+            rsi[w] = own ? in.row_scale[i[w]] : 1.0;
             const int cnt = qe[w] > qb[w] ? (qe[w] - qb[w] + GR - 1) / GR : 0;
             most = cnt > most ? cnt : most;
             ax[w] = 0.0;
@@ -265,17 +309,19 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
 #pragma unroll
             for (int w = 0; w < kEvalCW; ++w) {
                 const int q = qb[w] + t * GR;
-                if (q < qe[w]) ax[w] += in.val[q] * static_cast<double>(v.xh[in.ci[q]]);
+                if (q < qe[w]) {
+                    const int col = in.ci[q];
+                    ax[w] += in.val[q] * static_cast<double>(kStage ? stage[col] : v.xh[col]);
+                }
             }
         }
 #pragma unroll
         for (int w = 0; w < kEvalCW; ++w) {
             ax[w] = groupSum<GR>(ax[w]);
             if (lr == 0 && i[w] < ie) {
-                const int ii = i[w];
-                const double lo = in.rl[ii], hi = in.ru[ii];
+                const double lo = loi[w], hi = hii[w];
                 const double vs = ax[w] - clampRef(ax[w], lo, hi);
-                const double p = vs / (c.cs * in.row_scale[ii]);
+                const double p = vs / (c.cs * rsi[w]);
                 rf[kPsq] += p * p;
                 rf[kPmax] = maxRef(rf[kPmax], fabs(p));
                 const double dy = yv[w] - y0v[w];

@@ -760,20 +760,28 @@ __global__ __launch_bounds__(1024) void kEvalScan(const BInst* __restrict__ inst
 }
 
 // Part p of P: columns and rows in P equal slices.
-template <int GC, int GR, typename T>
+template <int GC, int GR, typename T, bool kStage>
 __device__ inline void evalPart(const BInst& in, int p, int P, double* dst) {
     const long long cspan = (in.n + P - 1) / P, rspan = (in.m + P - 1) / P;
     const int jb = static_cast<int>(min(static_cast<long long>(in.n), p * cspan));
     const int je = static_cast<int>(min(static_cast<long long>(in.n), jb + cspan));
     const int ib = static_cast<int>(min(static_cast<long long>(in.m), p * rspan));
     const int ie = static_cast<int>(min(static_cast<long long>(in.m), ib + rspan));
-    bEvalFields<GC, GR, T>(in, pickVecs<T>(in), *in.ctrl, dst, jb, je, ib, ie);
+    bEvalFields<GC, GR, T, kStage>(in, pickVecs<T>(in), *in.ctrl, dst, jb, je, ib, ie);
 }
 
-// Every part of this round's evaluations, over a fixed grid: part t belongs
-// to the last instance i with off[i] <= t; its sums go to
-// parts[(i * kEvalMaxParts + p) * kNumEval ...].
-template <typename T>
+// Evaluations done in one part whose gathered vectors fit the LDS stage
+// (bEvalFields' kStage): kEvalParts<T, true> takes these, kEvalParts<T,
+// false> the rest.
+__device__ inline bool evalStaged(const BInst& in) {
+    return evalParts(in) == 1 && 2 * static_cast<long long>(in.m) <= kEvalStageElems &&
+           in.n <= kEvalStageElems;
+}
+
+// Every part of this round's evaluations (of the kind kStage selects), over
+// a fixed grid: part t belongs to the last instance i with off[i] <= t; its
+// sums go to parts[(i * kEvalMaxParts + p) * kNumEval ...].
+template <typename T, bool kStage>
 __global__ __launch_bounds__(kBlock) void kEvalParts(const BInst* __restrict__ insts, int K,
                                                      const int* __restrict__ off,
                                                      double* __restrict__ parts) {
@@ -786,17 +794,18 @@ __global__ __launch_bounds__(kBlock) void kEvalParts(const BInst* __restrict__ i
             else hi = mid - 1;
         }
         const BInst& in = insts[lo];
+        if (evalStaged(in) != kStage) continue;
         const int p = t - off[lo], P = evalParts(in);
         double* dst = parts + (static_cast<size_t>(lo) * kEvalMaxParts + p) * kNumEval;
         switch (in.preset) {
-            case 0: evalPart<1, 1, T>(in, p, P, dst); break;
-            case 1: evalPart<2, 2, T>(in, p, P, dst); break;
-            case 2: evalPart<2, 4, T>(in, p, P, dst); break;
-            case 3: evalPart<4, 4, T>(in, p, P, dst); break;
-            case 4: evalPart<4, 8, T>(in, p, P, dst); break;
-            case 5: evalPart<8, 8, T>(in, p, P, dst); break;
-            case 6: evalPart<8, 16, T>(in, p, P, dst); break;
-            default: evalPart<16, 16, T>(in, p, P, dst); break;
+            case 0: evalPart<1, 1, T, kStage>(in, p, P, dst); break;
+            case 1: evalPart<2, 2, T, kStage>(in, p, P, dst); break;
+            case 2: evalPart<2, 4, T, kStage>(in, p, P, dst); break;
+            case 3: evalPart<4, 4, T, kStage>(in, p, P, dst); break;
+            case 4: evalPart<4, 8, T, kStage>(in, p, P, dst); break;
+            case 5: evalPart<8, 8, T, kStage>(in, p, P, dst); break;
+            case 6: evalPart<8, 16, T, kStage>(in, p, P, dst); break;
+            default: evalPart<16, 16, T, kStage>(in, p, P, dst); break;
         }
     }
 }
@@ -886,7 +895,8 @@ void enqueueEval(hipStream_t st, int grid, const BInst* di, int K, const BOption
     if constexpr (std::is_same_v<T, double>) {
         if (bo.refine) kRefineMaterializeParts<<<grid, kBlock, 0, st>>>(di, K, off);
     }
-    kEvalParts<T><<<grid, kBlock, 0, st>>>(di, K, off, parts);
+    kEvalParts<T, true><<<grid, kBlock, 0, st>>>(di, K, off, parts);
+    kEvalParts<T, false><<<grid, kBlock, 0, st>>>(di, K, off, parts);
     kEvalControl<T><<<K, kBlock, 0, st>>>(di, K, bo, parts);
     kControlCopy<T><<<grid, kBlock, 0, st>>>(di, K, off);
     if (bo.refine) kControlRecenter<<<grid, kBlock, 0, st>>>(di, K, off);
