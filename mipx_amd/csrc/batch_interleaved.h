@@ -21,6 +21,9 @@
 // after which the per-instance evaluation / control kernel runs as usual.
 #pragma once
 
+#include <cstddef>
+#include <type_traits>
+
 #include "batch_kernels.h"
 #include "pdlp_refine.h"
 
@@ -153,6 +156,43 @@ __device__ inline bool sameBits(T a, T b) {
         return __builtin_bit_cast(unsigned, a) == __builtin_bit_cast(unsigned, b);
 }
 
+// Byte offset, within an instance's BInst, of the pointer to source vector
+// `a` of the vector set `which` (0 x, x0, l, u; 1 y, y0; 2 the refinement's
+// c'; 3 its rl', ru', q; sets 2 and 3 are FP32 only).
+template <typename T>
+__device__ inline size_t ilSourceOffset(int which, int a) {
+    using V = BVecs<T>;
+    const size_t vb = std::is_same_v<T, float> ? offsetof(BInst, f) : offsetof(BInst, d);
+    if (which == 0)
+        return vb + (a == 0 ? offsetof(V, x) : a == 1 ? offsetof(V, x0) : a == 2 ? offsetof(V, l)
+                                                                               : offsetof(V, u));
+    if (which == 1) return vb + (a == 0 ? offsetof(V, y) : offsetof(V, y0));
+    if (which == 2) return offsetof(BInst, rc);
+    return a == 0 ? offsetof(BInst, rrl) : a == 1 ? offsetof(BInst, rru) : offsetof(BInst, rq);
+}
+
+template <typename T>
+__device__ inline const T* ilSource(const BInst* in, size_t off) {
+    return *reinterpret_cast<const T* const*>(reinterpret_cast<const char*>(in) + off);
+}
+
+// Rows of a 64 x 64 transpose tile that one thread moves (kBlock / 64 rows
+// per pass).
+constexpr int kTileRows = kLanes / (kBlock / kLanes);
+
+// The instances of the tile rows a thread moves, resolved once per tile:
+// the row is the same for a whole wave, so its slot and instance come from
+// scalar loads; all of a thread's row loads are then issued together.
+__device__ inline void tileRowInsts(const BInst* __restrict__ insts, const int* __restrict__ slots,
+                                    int nslots, int g, const BInst* (&row)[kTileRows]) {
+    const int r0 = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kLanes);
+#pragma unroll
+    for (int k = 0; k < kTileRows; ++k) {
+        const int s = g * kLanes + r0 + k * (kBlock / kLanes);
+        row[k] = s < nslots ? insts + __builtin_amdgcn_readfirstlane(slots[s]) : nullptr;
+    }
+}
+
 // For which = 0 with `flags` given, also the bound flags of the group's
 // 64-column tile: bit e - e0 set when some taking-part lane's l or u at
 // column e differs (bitwise) from the owner's BL[e] / BU[e].
@@ -166,44 +206,38 @@ __global__ __launch_bounds__(kBlock) void kInterleave(
     __shared__ unsigned long long own;
     const int nslots = *count;
     const int per_group = (len + kLanes - 1) / kLanes;
-    // which: 0 x, x0, l, u; 1 y, y0; 2 the refinement's c'; 3 its rl', ru', q.
     const int nsrc = which == 0 ? 4 : which == 1 ? 2 : which == 2 ? 1 : 3;
     const bool mark = which == 0 && flags != nullptr;
+    const int lane = static_cast<int>(threadIdx.x % kLanes);
+    const int r0 = static_cast<int>(threadIdx.x / kLanes);
     forGroupItems(nslots, per_group, [&](int g, int tb) {
         const int e0 = tb * kLanes;
         // Cleared before the first barrier below; read after the last one.
         if (mark && threadIdx.x == 0) own = 0ull;
+        const BInst* row[kTileRows];
+        tileRowInsts(insts, slots, nslots, g, row);
         for (int a = 0; a < nsrc; ++a) {
             // load: row r = slot within the group, column = element offset
-            for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
-                const int s = g * kLanes + r;
-                const int e = e0 + static_cast<int>(threadIdx.x % kLanes);
-                T v = T(0);
-                if (s < nslots && e < len) {
-                    const BInst& in = insts[slots[s]];
-                    const BVecs<T>& vv = vecsOf<T>(in);
-                    const T* src = nullptr;
-                    if (which == 0) src = a == 0 ? vv.x : a == 1 ? vv.x0 : a == 2 ? vv.l : vv.u;
-                    else if (which == 1) src = a == 0 ? vv.y : vv.y0;
-                    if constexpr (sizeof(T) == 4) {
-                        if (which == 2) src = in.rc;
-                        else if (which == 3) src = a == 0 ? in.rrl : a == 1 ? in.rru : in.rq;
-                    }
-                    v = src[e];
-                }
-                tile[r][threadIdx.x % kLanes] = v;
-            }
+            const int e = e0 + lane;
+            const size_t off = ilSourceOffset<T>(which, a);
+            T v[kTileRows];
+#pragma unroll
+            for (int k = 0; k < kTileRows; ++k)
+                v[k] = row[k] != nullptr && e < len ? ilSource<T>(row[k], off)[e] : T(0);
+#pragma unroll
+            for (int k = 0; k < kTileRows; ++k) tile[r0 + k * (kBlock / kLanes)][lane] = v[k];
             __syncthreads();
             T* dst = a == 0 ? A : a == 1 ? B : a == 2 ? C : D;
-            for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
-                const int e = e0 + r;
-                const int lane = static_cast<int>(threadIdx.x % kLanes);
-                if (e >= len) continue;
-                const T v = tile[lane][r];
-                dst[(static_cast<size_t>(g) * len + e) * kLanes + lane] = v;
+#pragma unroll
+            for (int k = 0; k < kTileRows; ++k) {
+                const int r = r0 + k * (kBlock / kLanes);
+                const int er = e0 + r;
+                if (er >= len) continue;
+                const T w = tile[lane][r];
+                dst[(static_cast<size_t>(g) * len + er) * kLanes + lane] = w;
                 if (mark && a >= 2) {
-                    const T b = (a == 2 ? BL : BU)[e];
-                    const bool diff = g * kLanes + lane < nslots && !sameBits(v, b);
+                    const T b = (a == 2 ? BL : BU)[er];
+                    const bool diff = g * kLanes + lane < nslots && !sameBits(w, b);
                     if (__ballot(diff) != 0ull && lane == 0) atomicOr(&own, 1ull << r);
                 }
             }
@@ -222,20 +256,26 @@ __global__ __launch_bounds__(kBlock) void kDeinterleave(const BInst* __restrict_
     __shared__ T tile[kLanes][kLanes + 1];
     const int nslots = *count;
     const int per_group = (len + kLanes - 1) / kLanes;
+    const int lane = static_cast<int>(threadIdx.x % kLanes);
+    const int r0 = static_cast<int>(threadIdx.x / kLanes);
     forGroupItems(nslots, per_group, [&](int g, int tb) {
         const int e0 = tb * kLanes;
-        for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
-            const int e = e0 + r;
-            const int lane = static_cast<int>(threadIdx.x % kLanes);
-            tile[lane][r] = e < len ? A[(static_cast<size_t>(g) * len + e) * kLanes + lane] : T(0);
+        const BInst* row[kTileRows];
+        tileRowInsts(insts, slots, nslots, g, row);
+#pragma unroll
+        for (int k = 0; k < kTileRows; ++k) {
+            const int e = e0 + r0 + k * (kBlock / kLanes);
+            tile[lane][r0 + k * (kBlock / kLanes)] =
+                e < len ? A[(static_cast<size_t>(g) * len + e) * kLanes + lane] : T(0);
         }
         __syncthreads();
-        for (int r = threadIdx.x / kLanes; r < kLanes; r += kBlock / kLanes) {
-            const int s = g * kLanes + r;
-            const int e = e0 + static_cast<int>(threadIdx.x % kLanes);
-            if (s < nslots && e < len) {
-                const BVecs<T>& vv = vecsOf<T>(insts[slots[s]]);
-                (which == 0 ? vv.x : vv.y)[e] = tile[r][threadIdx.x % kLanes];
+        const int e = e0 + lane;
+#pragma unroll
+        for (int k = 0; k < kTileRows; ++k) {
+            const int r = r0 + k * (kBlock / kLanes);
+            if (row[k] != nullptr && e < len) {
+                const BVecs<T>& vv = vecsOf<T>(*row[k]);
+                (which == 0 ? vv.x : vv.y)[e] = tile[r][lane];
             }
         }
         __syncthreads();
