@@ -194,13 +194,32 @@ __global__ __launch_bounds__(kBlock) void kDualStepLong(
     dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, q, sc, k);
 }
 
+// Row i's dot product so far after chunk `ch`, over the chunk's entries
+// [b, e), for the group's lane 0 (part[i]: the sum after the earlier
+// chunks).  With one lane (G = 1) the sum continues part[i] entry by entry,
+// so the chunked product is the one-launch sum in storage order; with G
+// lanes the chunk's own group sum is added to it, ((p_0 + p_1) + ...) +
+// p_{C-1}.
+template <int G, typename T, typename TX>
+__device__ inline T chunkDot(int b, int e, const int* __restrict__ idx, const T* __restrict__ val,
+                             const TX* __restrict__ x, int lane, int ch, const T* __restrict__ part,
+                             int i) {
+    if constexpr (G == 1) {
+        T acc = ch > 0 ? part[i] : T(0);
+        for (int k = b; k < e; ++k) acc += val[k] * static_cast<T>(x[idx[k]]);
+        return acc;
+    } else {
+        const T acc = groupDotBE<G>(b, e, idx, val, x, lane);
+        return ch > 0 ? part[i] + acc : acc;
+    }
+}
+
 // Dual half step with A x_r in column chunks: chunk c of C holds, row by
 // row, the entries whose columns lie in chunk c (rows in order, the entries
 // of a row in column order; row starts rs_c).  A chunk's slice of x_r fits
 // in L2, so the gathers of one launch hit L2 instead of going to memory.
-// Chunks 0..C-2 leave their row sums in `part` (first writes, the others
-// add); the last adds its own and finishes the row: A x_r = ((p0 + p1) +
-// ...) + p_{C-1}.
+// Chunks 0..C-2 leave their rows' sums so far in `part` (chunkDot); the last
+// finishes the row.
 template <int G, typename T, bool STORE_HAT>
 __global__ __launch_bounds__(kBlock) void kDualChunk(
     int m, const int* __restrict__ rs_c, const int* __restrict__ ci_c, const T* __restrict__ val_c,
@@ -211,16 +230,10 @@ __global__ __launch_bounds__(kBlock) void kDualChunk(
     const long long gid = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
     const int i = static_cast<int>(gid / G), lane = static_cast<int>(gid % G);
     if (i >= m) return;
-    const T acc = groupDotBE<G>(rs_c[i], rs_c[i + 1], ci_c, val_c, xr, lane);
+    const T acc = chunkDot<G>(rs_c[i], rs_c[i + 1], ci_c, val_c, xr, lane, c, part, i);
     if (lane != 0) return;
-    if (c == 0 && C > 1) {
-        part[i] = acc;
-    } else if (c < C - 1) {
-        part[i] = part[i] + acc;
-    } else {
-        const T axr = C > 1 ? part[i] + acc : acc;
-        dualEpilogue<T, STORE_HAT>(i, axr, y, y0, yh, sigma_base, rl, ru, q, sc, k);
-    }
+    if (c < C - 1) part[i] = acc;
+    else dualEpilogue<T, STORE_HAT>(i, acc, y, y0, yh, sigma_base, rl, ru, q, sc, k);
 }
 
 // ---- evaluation (src/lp/pdlp.cpp:1076-1202) --------------------------------

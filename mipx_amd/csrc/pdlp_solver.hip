@@ -426,7 +426,7 @@ void SingleSolver::setup(const double* warm_x, const double* warm_y) {
         }
         if (cc > 0 && cc < n_ && !longRows && m_ > 0) {
             nch_ = static_cast<int>((n_ + cc - 1) / cc);
-            gch_ = pickGroup(nnz_ / nch_, m_, 0);
+            gch_ = pickGroup(nnz_ / nch_, m_, o_.spmv_group);
             bytes += padBytes(static_cast<size_t>(nch_) * (m + 1), 4) + 2 * padBytes(nnz, 4) +
                      padBytes(nnz, 8) + padBytes(m, 8);
             if (mixed_) bytes += padBytes(nnz, 4) + padBytes(m, 4);

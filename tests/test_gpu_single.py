@@ -365,18 +365,23 @@ def test_chunked_dual_spmv(precision):
 
 def test_chunked_dual_one_lane_follows_oracle():
     """With one lane per row the chunked A x_r continues each row's running
-    sum chunk after chunk, the reference's order: the FP64 solve follows the
-    oracle's trajectory (iterations within one evaluation block, restarts
-    within one), as the one-launch SpMV does."""
+    sum chunk after chunk, the reference's order: the chunked solve is bit
+    for bit the one-launch solve, and both follow the oracle's trajectory
+    (iterations within one evaluation block, restarts within one)."""
     lp = lpgen.c0_plain(seed=0xC0, m=2000, n=5000).lp
     ro, _, _ = ob.solve(lp, **TIGHT)
+    runs = {}
     for chunk in (1300, -1):
         r, x, y = native.solve(lp, spmv_group=1, spmv_chunk_cols=chunk, **TIGHT)
+        runs[chunk] = (r, x, y)
         assert r["status"] == ro["status"] == 0, (chunk, r)
         assert abs(r["iterations"] - ro["iterations"]) <= 200, (chunk, r["iterations"], ro)
         assert abs(r["restarts"] - ro["restarts"]) <= 1, (chunk, r, ro)
         assert abs(r["objective"] - ro["objective"]) <= 1e-6 * (1 + abs(ro["objective"]))
         certify(lp, r, x, y)
+    (ra, xa, ya), (rb, xb, yb) = runs[1300], runs[-1]
+    assert ra["iterations"] == rb["iterations"] and ra["objective"] == rb["objective"]
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
 
 
 def test_preconditioner_refresh_single_path():
