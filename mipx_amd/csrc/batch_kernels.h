@@ -126,18 +126,6 @@ struct BOwner {
 };
 
 // ---- block helpers -----------------------------------------------------------
-__device__ inline double blockSum1(double v) {
-    double a[1] = {v};
-    const bool mx[1] = {false};
-    double o[1];
-    blockReduce<1>(a, mx, o);
-    __shared__ double bc;
-    if (threadIdx.x == 0) bc = o[0];
-    __syncthreads();
-    const double r = bc;
-    __syncthreads();
-    return r;
-}
 
 template <int G, typename T, typename TX>
 __device__ inline T bDot(const int* __restrict__ starts, const int* __restrict__ idx,
@@ -148,7 +136,46 @@ __device__ inline T bDot(const int* __restrict__ starts, const int* __restrict__
     return groupSum<G>(acc);
 }
 
-constexpr int kNumEval = static_cast<int>(kNumColFields) + static_cast<int>(kNumRowFields);
+// Certificate terms (src/lp/pdlp.cpp:119-284) gathered by the evaluation on
+// the rounds whose decision may check the certificates, in original units
+// before the rays are normalised: sums of squares for the norms, the sums of
+// the objective terms, and the largest value of the quantities that the
+// checks compare against tol times the norm (negated where the check is
+// a lower one).  Column terms use A'y^, row terms A x^, both already formed
+// by the evaluation.
+enum CertColField {
+    kCxsq = 0,  // |x_o|^2
+    kCfBad,     // Farkas: max of -(A'y_o)_j (finite lower bound), (A'y_o)_j (finite upper)
+    kCcol,      // column part of the Farkas sum
+    kCrBad,     // primal ray: max of -x_o (finite lower bound), x_o (finite upper)
+    kCobj,      // c_o'x_o
+    kNumCertCol
+};
+enum CertRowField {
+    kRysq = 0,  // |y_o|^2
+    kRfBad,     // Farkas: max of -y_o (only an upper bound), y_o (only a lower bound)
+    kRrow,      // row part of the Farkas sum
+    kRrBad,     // primal ray: max of |A x_o| (both bounds), A x_o (upper only), -A x_o (lower only)
+    kNumCertRow
+};
+constexpr int kNumEvalCore = static_cast<int>(kNumColFields) + static_cast<int>(kNumRowFields);
+constexpr int kCertCol0 = kNumEvalCore, kCertRow0 = kNumEvalCore + kNumCertCol;
+constexpr int kNumEval = kNumEvalCore + kNumCertCol + kNumCertRow;
+
+// Whether field f of an evaluation part combines by maximum (else by sum).
+__host__ __device__ constexpr bool evalFieldIsMax(int f) {
+    return f == static_cast<int>(kDmax) || f == static_cast<int>(kNumColFields) + kPmax ||
+           f == kCertCol0 + kCfBad || f == kCertCol0 + kCrBad || f == kCertRow0 + kRfBad ||
+           f == kCertRow0 + kRrBad;
+}
+
+// The certificates are checked at the evaluation that ends a round with the
+// iteration total a multiple of `every` (ten evaluation blocks), outside
+// the polish phase.
+__device__ inline bool certDue(const BCtrl& c, int every) {
+    const int after = c.total + c.round_iters;
+    return c.phase != kPhasePolish && every > 0 && after >= every && after % every == 0;
+}
 // Columns (rows) per lane group and pass in the batched evaluation.
 constexpr int kEvalCW = 4;
 // An instance's evaluation is split into parts of about kEvalChunk matrix
@@ -190,10 +217,16 @@ __device__ inline T* evalStage() {
 // pass, then x^ for the row pass, and gathered from there.  With thousands
 // of instances in flight their gathered vectors do not stay in L2; the
 // values, and so the fields, are the same.
+//
+// cert: also the certificate terms (CertColField, CertRowField) after the
+// kNumEvalCore fields; otherwise those fields are left unwritten.
 template <int GC, int GR, typename T, bool kStage = false>
 __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCtrl& c,
-                                   double* dst, int jb, int je, int ib, int ie) {
+                                   double* dst, int jb, int je, int ib, int ie, bool cert) {
     const int tid = threadIdx.x;
+    const double ninf = -INFINITY;
+    double cc[kNumCertCol] = {0.0, ninf, 0.0, ninf, 0.0};
+    double crr[kNumCertRow] = {0.0, ninf, 0.0, ninf};
     T* stage = kStage ? evalStage<T>() : nullptr;
     if constexpr (kStage) {
         __syncthreads();
@@ -275,8 +308,46 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
                 if (!isfinite(ub)) gp = maxRef(gp, 0.0);
                 if (gp > 0.0 && isfinite(lb)) cf[kDcol] += gp * lb;
                 else if (gp < 0.0 && isfinite(ub)) cf[kDcol] += gp * ub;
+                if (cert) {
+                    // Farkas column terms on y_o = sgn R y^ / os: (A'y_o)_j =
+                    // sgn (A~'y^)_j / (C_j os); primal-ray terms on x_o =
+                    // C x^ / cs (pdlp.cpp:119-284).
+                    const double lo = in.lo_o[j[w]], hi = in.hi_o[j[w]];
+                    const double atr = c.sgn * aty[w] / csj[w] / c.os;
+                    const double xo = xj[w] * csj[w] * (1.0 / c.cs);
+                    cc[kCxsq] += xo * xo;
+                    if (isfinite(lo)) {
+                        cc[kCfBad] = maxRef(cc[kCfBad], -atr);
+                        cc[kCrBad] = maxRef(cc[kCrBad], -xo);
+                    }
+                    if (isfinite(hi)) {
+                        cc[kCfBad] = maxRef(cc[kCfBad], atr);
+                        cc[kCrBad] = maxRef(cc[kCrBad], xo);
+                    }
+                    if (atr > 0.0 && isfinite(lo)) cc[kCcol] += atr * lo;
+                    else if (atr < 0.0 && isfinite(hi)) cc[kCcol] += atr * hi;
+                    cc[kCobj] += in.obj_o[j[w]] * xo;
+                }
             }
         }
+    }
+    // Block sums of the column fields now (each field reduces on its own,
+    // so reducing them before the row pass changes no bits).
+    {
+        bool mx[kNumColFields];
+        double out[kNumColFields];
+        for (int f = 0; f < kNumColFields; ++f) mx[f] = f == kDmax;
+        blockReduce<kNumColFields>(cf, mx, out);
+        if (tid == 0)
+            for (int f = 0; f < kNumColFields; ++f) dst[f] = out[f];
+    }
+    if (cert) {
+        bool mx[kNumCertCol];
+        double out[kNumCertCol];
+        for (int f = 0; f < kNumCertCol; ++f) mx[f] = evalFieldIsMax(kCertCol0 + f);
+        blockReduce<kNumCertCol>(cc, mx, out);
+        if (tid == 0)
+            for (int f = 0; f < kNumCertCol; ++f) dst[kCertCol0 + f] = out[f];
     }
     if constexpr (kStage) {
         __syncthreads();
@@ -331,22 +402,61 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
                 if (!isfinite(lo)) yp = maxRef(yp, 0.0);
                 if (yp > 0.0 && isfinite(hi)) rf[kDrow] -= yp * hi;
                 else if (yp < 0.0 && isfinite(lo)) rf[kDrow] -= yp * lo;
+                if (cert) {
+                    const double rlo = in.rl_o[i[w]], rhi = in.ru_o[i[w]];
+                    const bool fl = isfinite(rlo), fh = isfinite(rhi);
+                    const double yo = c.sgn * yv[w] * rsi[w] * (1.0 / c.os);
+                    crr[kRysq] += yo * yo;
+                    if (fh && !fl) crr[kRfBad] = maxRef(crr[kRfBad], -yo);
+                    if (fl && !fh) crr[kRfBad] = maxRef(crr[kRfBad], yo);
+                    if (rlo == rhi) crr[kRrow] += yo * rlo;
+                    else if (yo > 0.0 && fh) crr[kRrow] += yo * rhi;
+                    else if (yo < 0.0 && fl) crr[kRrow] += yo * rlo;
+                    const double ar = ax[w] / rsi[w] / c.cs;
+                    if (fl && fh) crr[kRrBad] = maxRef(crr[kRrBad], fabs(ar));
+                    else if (fh) crr[kRrBad] = maxRef(crr[kRrBad], ar);
+                    else if (fl) crr[kRrBad] = maxRef(crr[kRrBad], -ar);
+                }
             }
         }
     }
-    double all[kNumEval], out[kNumEval];
-    bool mx[kNumEval];
-    for (int f = 0; f < kNumColFields; ++f) {
-        all[f] = cf[f];
-        mx[f] = f == kDmax;
+    {
+        bool mx[kNumRowFields];
+        double out[kNumRowFields];
+        for (int f = 0; f < kNumRowFields; ++f) mx[f] = f == kPmax;
+        blockReduce<kNumRowFields>(rf, mx, out);
+        if (tid == 0)
+            for (int f = 0; f < kNumRowFields; ++f) dst[kNumColFields + f] = out[f];
     }
-    for (int f = 0; f < kNumRowFields; ++f) {
-        all[kNumColFields + f] = rf[f];
-        mx[kNumColFields + f] = f == kPmax;
+    if (cert) {
+        bool mx[kNumCertRow];
+        double out[kNumCertRow];
+        for (int f = 0; f < kNumCertRow; ++f) mx[f] = evalFieldIsMax(kCertRow0 + f);
+        blockReduce<kNumCertRow>(crr, mx, out);
+        if (tid == 0)
+            for (int f = 0; f < kNumCertRow; ++f) dst[kCertRow0 + f] = out[f];
     }
-    blockReduce<kNumEval>(all, mx, out);
-    if (tid == 0)
-        for (int f = 0; f < kNumEval; ++f) dst[f] = out[f];
+}
+
+// The certificate verdict from an instance's combined certificate terms
+// (the checks of src/lp/pdlp.cpp:119-284 with the normalisation by the ray
+// norm moved onto the tolerance): 1 = infeasible (Farkas ray), 2 =
+// unbounded (primal ray), 0 = neither.
+__device__ inline int certVerdict(const double* f, int sense_max) {
+    const double* cc = f + kCertCol0;
+    const double* cr = f + kCertRow0;
+    const double tol = 1e-6;
+    const double ny = sqrt(cr[kRysq]);
+    if (ny >= 1e-15) {
+        const bool bad = cr[kRfBad] > tol * ny || cc[kCfBad] > tol * ny;
+        if (!bad && cr[kRrow] / ny - cc[kCcol] / ny < -tol) return 1;
+    }
+    const double nx = sqrt(cc[kCxsq]);
+    if (nx < 1e-15) return 0;
+    const bool bad = cc[kCrBad] > tol * nx || cr[kRrBad] > tol * nx;
+    const double sg = sense_max ? -1.0 : 1.0;
+    if (!bad && sg * (cc[kCobj] / nx) < -tol) return 2;
+    return 0;
 }
 
 // The termination quantities from the evaluation sums f (column fields, then
@@ -369,98 +479,6 @@ __device__ inline BEval bEvalFinish(const double* f, const BCtrl& c) {
     e.dobj = c.offset + c.sgn * (mc[kDcol] + mr[kDrow]) * unscale;
     e.gap = fabs(e.pobj - e.dobj) / (1.0 + fabs(e.pobj) + fabs(e.dobj));
     return e;
-}
-
-// Farkas and primal-ray certificates (src/lp/pdlp.cpp:119-284) evaluated on
-// the device in the scaled space: with y = sgn y^ R / os and x = C x^ / cs,
-// A'y = sgn C^-1 (A~' y^) / os and A x = R^-1 (A~ x^) / cs.  Returns
-// 1 = infeasible, 2 = unbounded, 0 = neither.  Uses the original row / column
-// bounds recovered from the scaled ones.
-template <int GC, int GR, typename T>
-__device__ inline int bCertificates(const BInst& in, const BVecs<T>& v, const BCtrl& c,
-                                    const double* lo_o, const double* hi_o, const double* rl_o,
-                                    const double* ru_o, const double* obj_o, int sense_max) {
-    const int tid = threadIdx.x;
-    // --- Farkas on y_orig ---
-    double ysq = 0.0;
-    for (int i = tid; i < in.m; i += kBlock) {
-        const double yo = c.sgn * static_cast<double>(v.yh[i]) * in.row_scale[i] * (1.0 / c.os);
-        ysq += yo * yo;
-    }
-    const double ny = sqrt(blockSum1(ysq));
-    const double tol = 1e-6;
-    int verdict = 0;
-    if (ny >= 1e-15) {
-        double bad = 0.0, rowpart = 0.0;
-        for (int i = tid; i < in.m; i += kBlock) {
-            const double r = c.sgn * static_cast<double>(v.yh[i]) * in.row_scale[i] * (1.0 / c.os) / ny;
-            const bool lo = isfinite(rl_o[i]), hi = isfinite(ru_o[i]);
-            if (hi && !lo && r < -tol) bad = 1.0;
-            if (lo && !hi && r > tol) bad = 1.0;
-            if (rl_o[i] == ru_o[i]) rowpart += r * rl_o[i];
-            else if (r > 0.0 && hi) rowpart += r * ru_o[i];
-            else if (r < 0.0 && lo) rowpart += r * rl_o[i];
-        }
-        double colpart = 0.0;
-        for (int base = 0; base < in.n; base += kBlock / GC) {
-            const int j = base + tid / GC, lane = tid % GC;
-            if (j < in.n) {
-                double acc = 0.0;
-                for (int q = in.cs[j] + lane; q < in.cs[j + 1]; q += GC)
-                    acc += in.valt[q] * static_cast<double>(v.yh[in.ri[q]]);
-                acc = groupSum<GC>(acc);
-                if (lane == 0) {
-                    const double atr = c.sgn * acc / in.col_scale[j] / c.os / ny;
-                    if (isfinite(lo_o[j]) && atr < -tol) bad = 1.0;
-                    if (isfinite(hi_o[j]) && atr > tol) bad = 1.0;
-                    if (atr > 0.0 && isfinite(lo_o[j])) colpart += atr * lo_o[j];
-                    else if (atr < 0.0 && isfinite(hi_o[j])) colpart += atr * hi_o[j];
-                }
-            }
-        }
-        const double sbad = blockSum1(bad), srow = blockSum1(rowpart), scol = blockSum1(colpart);
-        if (sbad == 0.0 && srow - scol < -tol) verdict = 1;
-    }
-    if (verdict) return verdict;
-    // --- primal ray on x_orig ---
-    double xsq = 0.0;
-    for (int j = tid; j < in.n; j += kBlock) {
-        const double xo = static_cast<double>(v.xh[j]) * in.col_scale[j] * (1.0 / c.cs);
-        xsq += xo * xo;
-    }
-    const double nx = sqrt(blockSum1(xsq));
-    if (nx < 1e-15) return 0;
-    double bad = 0.0, cr = 0.0;
-    for (int j = tid; j < in.n; j += kBlock) {
-        const double r = static_cast<double>(v.xh[j]) * in.col_scale[j] * (1.0 / c.cs) / nx;
-        if (isfinite(lo_o[j]) && r < -tol) bad = 1.0;
-        if (isfinite(hi_o[j]) && r > tol) bad = 1.0;
-        cr += obj_o[j] * r;
-    }
-    for (int base = 0; base < in.m; base += kBlock / GR) {
-        const int i = base + tid / GR, lane = tid % GR;
-        if (i < in.m) {
-            double acc = 0.0;
-            for (int q = in.rs[i] + lane; q < in.rs[i + 1]; q += GR)
-                acc += in.val[q] * static_cast<double>(v.xh[in.ci[q]]);
-            acc = groupSum<GR>(acc);
-            if (lane == 0) {
-                const double ar = acc / in.row_scale[i] / c.cs / nx;
-                const bool lo = isfinite(rl_o[i]), hi = isfinite(ru_o[i]);
-                if (lo && hi) {
-                    if (fabs(ar) > tol) bad = 1.0;
-                } else if (hi) {
-                    if (ar > tol) bad = 1.0;
-                } else if (lo) {
-                    if (ar < -tol) bad = 1.0;
-                }
-            }
-        }
-    }
-    const double sbad = blockSum1(bad), scr = blockSum1(cr);
-    const double sg = sense_max ? -1.0 : 1.0;
-    if (sbad == 0.0 && sg * scr < -tol) return 2;
-    return 0;
 }
 
 // Re-centring of an instance's refinement (one workgroup): the FP64 base

@@ -451,12 +451,11 @@ __global__ void kTileFill(const BInst* __restrict__ insts, int K, const int* __r
 // termination / certificate / restart / primal-weight decisions of the
 // reference's loop (src/lp/pdlp.cpp:2228-2469) and the polish phase switch
 // (2477-2676); then sets the instance's iteration count for the next round.
-template <int GC, int GR, typename T>
+template <typename T>
 __device__ void evalControl(const BInst& in, const BOptions& o, const double* parts) {
     BCtrl& c = *in.ctrl;
     const int tid = threadIdx.x;
     constexpr bool kFp32 = sizeof(T) == 4;
-    const BVecs<T>& v = pickVecs<T>(in);
     __shared__ int s_act, s_phase, s_ran;
     if (tid == 0) {
         s_phase = c.phase;
@@ -485,15 +484,18 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
     // of) that field over the parts in order.  A large LP has up to
     // kEvalMaxParts parts; one thread loading them field by field was a
     // chain of P * kNumEval dependent global loads.
+    // The certificate terms are there (and combined) only on the rounds
+    // whose decision may check the certificates.
     __shared__ double fs[kNumEval];
     __shared__ double staged[kEvalMaxParts * kNumEval];
     const int P = evalParts(in);
-    for (int t = tid; t < P * kNumEval; t += kBlock) staged[t] = parts[t];
+    const int nf = certDue(c, 10 * o.N) ? kNumEval : kNumEvalCore;
+    for (int t = tid; t < P * kNumEval; t += kBlock)
+        if (t % kNumEval < nf) staged[t] = parts[t];
     __syncthreads();
-    if (tid < kNumEval) {
+    if (tid < nf) {
         const int f = tid;
-        const bool mx = f == static_cast<int>(kDmax) ||
-                        f == static_cast<int>(kNumColFields) + static_cast<int>(kPmax);
+        const bool mx = evalFieldIsMax(f);
         double a = staged[f];
         for (int p = 1; p < P; ++p) {
             const double x = staged[p * kNumEval + f];
@@ -579,8 +581,7 @@ __device__ void evalControl(const BInst& in, const BOptions& o, const double* pa
     __syncthreads();
     int act = s_act;
     if (act == kActCert) {
-        const int verdict = bCertificates<GC, GR, T>(in, v, c, in.lo_o, in.hi_o, in.rl_o, in.ru_o,
-                                                     in.obj_o, in.sense_max);
+        const int verdict = certVerdict(fs, in.sense_max);
         if (tid == 0) {
             if (verdict) {
                 c.phase = kPhaseDone;
@@ -761,13 +762,15 @@ __global__ __launch_bounds__(1024) void kEvalScan(const BInst* __restrict__ inst
 
 // Part p of P: columns and rows in P equal slices.
 template <int GC, int GR, typename T, bool kStage>
-__device__ inline void evalPart(const BInst& in, int p, int P, double* dst) {
+__device__ inline void evalPart(const BInst& in, int p, int P, double* dst, int certEvery) {
     const long long cspan = (in.n + P - 1) / P, rspan = (in.m + P - 1) / P;
     const int jb = static_cast<int>(min(static_cast<long long>(in.n), p * cspan));
     const int je = static_cast<int>(min(static_cast<long long>(in.n), jb + cspan));
     const int ib = static_cast<int>(min(static_cast<long long>(in.m), p * rspan));
     const int ie = static_cast<int>(min(static_cast<long long>(in.m), ib + rspan));
-    bEvalFields<GC, GR, T, kStage>(in, pickVecs<T>(in), *in.ctrl, dst, jb, je, ib, ie);
+    const BCtrl& c = *in.ctrl;
+    bEvalFields<GC, GR, T, kStage>(in, pickVecs<T>(in), c, dst, jb, je, ib, ie,
+                                   certDue(c, certEvery));
 }
 
 // Evaluations done in one part whose gathered vectors fit the LDS stage
@@ -784,7 +787,8 @@ __device__ inline bool evalStaged(const BInst& in) {
 template <typename T, bool kStage>
 __global__ __launch_bounds__(kBlock) void kEvalParts(const BInst* __restrict__ insts, int K,
                                                      const int* __restrict__ off,
-                                                     double* __restrict__ parts) {
+                                                     double* __restrict__ parts,
+                                                     int certEvery) {
     const int total = off[K];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = K - 1;
@@ -798,14 +802,14 @@ __global__ __launch_bounds__(kBlock) void kEvalParts(const BInst* __restrict__ i
         const int p = t - off[lo], P = evalParts(in);
         double* dst = parts + (static_cast<size_t>(lo) * kEvalMaxParts + p) * kNumEval;
         switch (in.preset) {
-            case 0: evalPart<1, 1, T, kStage>(in, p, P, dst); break;
-            case 1: evalPart<2, 2, T, kStage>(in, p, P, dst); break;
-            case 2: evalPart<2, 4, T, kStage>(in, p, P, dst); break;
-            case 3: evalPart<4, 4, T, kStage>(in, p, P, dst); break;
-            case 4: evalPart<4, 8, T, kStage>(in, p, P, dst); break;
-            case 5: evalPart<8, 8, T, kStage>(in, p, P, dst); break;
-            case 6: evalPart<8, 16, T, kStage>(in, p, P, dst); break;
-            default: evalPart<16, 16, T, kStage>(in, p, P, dst); break;
+            case 0: evalPart<1, 1, T, kStage>(in, p, P, dst, certEvery); break;
+            case 1: evalPart<2, 2, T, kStage>(in, p, P, dst, certEvery); break;
+            case 2: evalPart<2, 4, T, kStage>(in, p, P, dst, certEvery); break;
+            case 3: evalPart<4, 4, T, kStage>(in, p, P, dst, certEvery); break;
+            case 4: evalPart<4, 8, T, kStage>(in, p, P, dst, certEvery); break;
+            case 5: evalPart<8, 8, T, kStage>(in, p, P, dst, certEvery); break;
+            case 6: evalPart<8, 16, T, kStage>(in, p, P, dst, certEvery); break;
+            default: evalPart<16, 16, T, kStage>(in, p, P, dst, certEvery); break;
         }
     }
 }
@@ -814,18 +818,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void kEvalControl(const BInst* __restrict__ insts, int K,
                                                         BOptions o, const double* __restrict__ parts) {
     if (static_cast<int>(blockIdx.x) >= K) return;
-    const BInst& in = insts[blockIdx.x];
-    const double* mine = parts + static_cast<size_t>(blockIdx.x) * kEvalMaxParts * kNumEval;
-    switch (in.preset) {
-        case 0: evalControl<1, 1, T>(in, o, mine); break;
-        case 1: evalControl<2, 2, T>(in, o, mine); break;
-        case 2: evalControl<2, 4, T>(in, o, mine); break;
-        case 3: evalControl<4, 4, T>(in, o, mine); break;
-        case 4: evalControl<4, 8, T>(in, o, mine); break;
-        case 5: evalControl<8, 8, T>(in, o, mine); break;
-        case 6: evalControl<8, 16, T>(in, o, mine); break;
-        default: evalControl<16, 16, T>(in, o, mine); break;
-    }
+    evalControl<T>(insts[blockIdx.x], o,
+                   parts + static_cast<size_t>(blockIdx.x) * kEvalMaxParts * kNumEval);
 }
 
 // Calls f(instance, jb, je, ib, ie) for every part of this round's
@@ -895,8 +889,8 @@ void enqueueEval(hipStream_t st, int grid, const BInst* di, int K, const BOption
     if constexpr (std::is_same_v<T, double>) {
         if (bo.refine) kRefineMaterializeParts<<<grid, kBlock, 0, st>>>(di, K, off);
     }
-    kEvalParts<T, true><<<grid, kBlock, 0, st>>>(di, K, off, parts);
-    kEvalParts<T, false><<<grid, kBlock, 0, st>>>(di, K, off, parts);
+    kEvalParts<T, true><<<grid, kBlock, 0, st>>>(di, K, off, parts, 10 * bo.N);
+    kEvalParts<T, false><<<grid, kBlock, 0, st>>>(di, K, off, parts, 10 * bo.N);
     kEvalControl<T><<<K, kBlock, 0, st>>>(di, K, bo, parts);
     kControlCopy<T><<<grid, kBlock, 0, st>>>(di, K, off);
     if (bo.refine) kControlRecenter<<<grid, kBlock, 0, st>>>(di, K, off);
