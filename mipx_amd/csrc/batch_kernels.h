@@ -601,7 +601,9 @@ __device__ inline void pendingCopySlice(const BInst& in, int code, int jb, int j
 }
 
 // The FP64 evaluation point of a refining instance over one column / row
-// slice: element-wise, as refineMaterialize.
+// slice, element-wise: base plus correction; a column whose correction sits
+// on its shifted bound takes the FP64 bound itself, and a ranged row whose
+// correction cancels the base multiplier takes zero.
 __device__ inline void refineMaterializeSlice(const BInst& in, int jb, int je, int ib, int ie) {
     for (int j = jb + static_cast<int>(threadIdx.x); j < je; j += kBlock) {
         const float d = in.f.xh[j];
