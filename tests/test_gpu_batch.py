@@ -133,6 +133,42 @@ def test_batch_detects_infeasible_and_unbounded():
     assert abs(res[2]["objective"] + 4.0) <= 1e-4
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_certificates_match_oracle(precision):
+    """The infeasibility and unboundedness certificates (tests/test_pdlp.cpp:
+    347-496, src/lp/pdlp.cpp:119-284) on the batched paths, whose evaluation
+    parts gather the certificates' terms and whose control kernel compares
+    them: every closed-form LP with rows and columns in one packed batch of
+    independent LPs, and bounds variants of one matrix on the lane-per-
+    instance path (x1 - x2 <= 0, min -x1: as given unbounded, x2 <= 3 and
+    x1 <= 2 bounded, x1 >= 5 with x2 <= 3 infeasible through the bounds
+    alone, which the reference's Farkas test does not certify: it stops at
+    the iteration limit, on the device as in the oracle), each with the
+    oracle's status and, where optimal, its objective."""
+    from lp_cases import closed_form_cases
+    cases = [(c, lp) for c, lp in closed_form_cases()
+             if c["status"] in ("Optimal", "Infeasible", "Unbounded") and c["m"] > 0 and c["n"] > 0]
+    lps = [lp for _, lp in cases]
+    res, xs, ys = native.solve_batch(lps, precision=precision, max_iter=100000)
+    for (c, lp), r in zip(cases, res):
+        ro, _, _ = ob.solve(lp, max_iter=100000, verbose=0)
+        assert r["status"] == ro["status"], (c["name"], r, ro)
+        if ro["status"] == 0:
+            assert abs(r["objective"] - c["objective"]) <= 1e-4 * (1 + abs(c["objective"])), (c, r)
+    base = {c["name"]: lp for c, lp in cases}["unbounded_constrained"]
+    inf = np.inf
+    lo = np.array([[0, 0], [0, 0], [5, 0], [0, 0]], dtype=float)
+    hi = np.array([[inf, inf], [inf, 3], [inf, 3], [2, inf]], dtype=float)
+    res, X, Y = native.solve_batch_shared(base, lo, hi, precision=precision, max_iter=20000)
+    want = [(2, None), (0, -3.0), (3, None), (0, -2.0)]
+    for i, (st, obj) in enumerate(want):
+        one = base.with_col_bounds(lo[i], hi[i])
+        ro, _, _ = ob.solve(one, max_iter=20000, verbose=0)
+        assert res[i]["status"] == ro["status"] == st, (i, res[i], ro)
+        if obj is not None:
+            assert abs(res[i]["objective"] - obj) <= 1e-4, (i, res[i])
+
+
 @pytest.mark.parametrize("which", ["c1_small", "c2_small"])
 def test_batched_setup_vs_oracle(which):
     """The batched engines' cohort setup: element-wise scaling bit-identical
