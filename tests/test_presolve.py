@@ -113,3 +113,58 @@ def test_closed_forms_through_presolve():
         if ro["status"] == 0:
             x, _ = pre.postsolve(xr, yr)
             assert abs(lp.obj @ x + lp.obj_offset - ro["objective"]) <= 1e-4, c["name"]
+
+
+def _random_feasible_lp(rng):
+    """A small LP that x0 satisfies, with the structures presolve removes:
+    fixed columns, empty and singleton rows, empty columns; every other
+    column boxed, so the LP is bounded."""
+    m, n = int(rng.integers(2, 9)), int(rng.integers(2, 11))
+    a = np.zeros((m, n))
+    for i in range(m):
+        kind = rng.integers(0, 4)
+        if kind == 1:
+            a[i, rng.integers(0, n)] = rng.uniform(0.5, 3.0) * rng.choice([-1.0, 1.0])
+        elif kind >= 2:
+            mask = rng.random(n) < 0.5
+            a[i, mask] = rng.uniform(-3.0, 3.0, mask.sum())
+    lo = -rng.uniform(0.0, 3.0, n)
+    hi = rng.uniform(0.0, 3.0, n)
+    fixed = rng.random(n) < 0.25
+    v = rng.uniform(-2.0, 2.0, n)
+    lo[fixed] = hi[fixed] = v[fixed]
+    x0 = np.clip(rng.uniform(-3.0, 3.0, n), lo, hi)
+    ax = a @ x0
+    rl, ru = ax - rng.uniform(0.0, 1.0, m), ax + rng.uniform(0.0, 1.0, m)
+    kinds = rng.integers(0, 4, m)
+    rl[kinds == 0] = ru[kinds == 0] = ax[kinds == 0]
+    rl[kinds == 1] = -np.inf
+    ru[kinds == 2] = np.inf
+    return from_dense(a, rng.uniform(-2.0, 2.0, n), lo, hi, rl, ru)
+
+
+def test_random_lps_presolve_sound():
+    """Random small LPs that are feasible by construction: presolve never
+    declares one infeasible, and the oracle's solution of the reduced LP,
+    postsolved, has the original LP's exact optimal objective (scipy HiGHS)
+    and passes the long-double KKT check on the original LP (primal and
+    dual).  The original LPs themselves often stall in the oracle, as in the
+    reference, when a fixed column has a negative reduced cost."""
+    rng = np.random.default_rng(0x9E5)
+    removed = 0
+    for case in range(60):
+        lp = _random_feasible_lp(rng)
+        pre = Presolved(lp)
+        assert pre.stats["infeasible"] == 0, case
+        removed += pre.stats["vars_removed"] + pre.stats["rows_removed"]
+        rr, xr, yr = ob.solve(pre.reduced, max_iter=500000, **TIGHT)
+        assert rr["status"] == 0, (case, rr)
+        x, y = pre.postsolve(xr, yr)
+        obj = lp.obj @ x + lp.obj_offset
+        assert abs(obj - rr["objective"]) <= 1e-9 * (1 + abs(rr["objective"])), case
+        opt = highs_objective(lp)
+        assert abs(obj - opt) <= 1e-5 * (1 + abs(opt)), (case, obj, opt)
+        k = ob.kkt_check(lp, x, y)
+        assert k["rel_primal"] <= 1e-5 and k["rel_dual"] <= 1e-5 and k["rel_gap"] <= 1e-5, (case, k)
+        assert k["bound_viol"] == 0.0, (case, k)
+    assert removed > 100
