@@ -228,3 +228,62 @@ def write_free_mps(lp, path):
     lines.append("ENDATA")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def work_unit_workload():
+    """The seeded solves of the device work-units gate
+    (tests/test_gpu_work_units_gate.py, baseline written by
+    tests/golden/make_device_work_units.py): name -> (kind, payload).
+    kind 'single' (one LP), 'batch' (independent LPs), 'shared' (an LP and
+    bounds variants), 'server' (fresh LPs and variants of a base)."""
+    from mipx_amd import lpgen
+    singles = {f"c0_{seed}_{m}x{n}": lpgen.c0_plain(seed=seed, m=m, n=n).lp
+               for seed, m, n in ((101, 300, 750), (102, 500, 1250), (103, 200, 500),
+                                  (0xC0, 2000, 5000))}
+    batch = [lpgen.c3_instance(i, m).lp for i, m in enumerate((200, 600, 1500, 3000))]
+    g = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    count = 64
+    lo = np.empty((count, 800))
+    hi = np.empty((count, 800))
+    for i in range(count):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i))
+    fresh = [lpgen.c3_instance(40 + i, m).lp for i, m in enumerate((400, 2500))]
+    return {"singles": singles, "batch": batch, "shared": (g.lp, lo, hi),
+            "server": (fresh, g.lp, lo[:8], hi[:8])}
+
+
+def run_work_unit_workload(native, tight):
+    """Solves the gate's workload on the device; returns name -> dict(status,
+    iterations, work_units)."""
+    w = work_unit_workload()
+    out = {}
+
+    def rec(name, r):
+        out[name] = dict(status=int(r["status"]), iterations=int(r["iterations"]),
+                         work_units=float(r["work_units"]))
+
+    for prec, tag in ((0, "fp64"), (1, "mixed")):
+        for name, lp in w["singles"].items():
+            rec(f"single/{tag}/{name}", native.solve(lp, precision=prec, **tight)[0])
+        res, _, _ = native.solve_batch(w["batch"], precision=prec, **tight)
+        for i, r in enumerate(res):
+            rec(f"batch/{tag}/{i}", r)
+        lp, lo, hi = w["shared"]
+        res, _, _ = native.solve_batch_shared(lp, lo, hi, precision=prec, **tight)
+        for i, r in enumerate(res):
+            rec(f"shared/{tag}/{i}", r)
+    fresh, base_lp, lo, hi = w["server"]
+    srv = native.Server(hbm_budget_bytes=4e9, max_slots=64, precision=1, **tight)
+    base = srv.add_base(base_lp)
+    tickets = [(f"server/mixed/fresh{i}", srv.submit(lp)) for i, lp in enumerate(fresh)]
+    tickets += [(f"server/mixed/variant{i}", srv.submit_bounds(base, lo[i], hi[i]))
+                for i in range(lo.shape[0])]
+    got = {}
+    while len(got) < len(tickets):
+        srv.step()
+        for t in srv.poll():
+            got[t] = srv.result(t)
+    srv.close()
+    for name, t in tickets:
+        rec(name, got[t][0])
+    return out
