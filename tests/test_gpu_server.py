@@ -94,6 +94,52 @@ def test_server_results_identical_to_solo_batch(precision):
             certify(lp, r, x, y, tag=f"{kind}{i}")
 
 
+def test_server_two_bases_identical_to_solo_batch():
+    """Bounds variants of two registered bases (different matrices, each
+    iterated in its own lane-per-instance groups) and a fresh LP, submitted
+    interleaved and in two waves: every result is bit-identical to the same
+    request solved alone, and certified."""
+    opts = dict(precision=1, max_iter=200000, **TIGHT)
+    ga = lpgen.c2_shared(seed=0xC2, m=300, n=800)
+    gb = lpgen.c2_shared(seed=0xC5, m=200, n=500)
+    ba = [ga.branch_bounds(lpgen.c2_instance_seed(i)) for i in range(5)]
+    bb = [gb.branch_bounds(lpgen.c2_instance_seed(i, base=0xC5)) for i in range(5)]
+    fresh = lpgen.c3_instance(7, 150).lp
+    srv = native.Server(max_slots=16, **opts)
+    base_a, base_b = srv.add_base(ga.lp), srv.add_base(gb.lp)
+    kinds = {}
+
+    def sub(which, idx):
+        ts = []
+        for i in idx:
+            t = (srv.submit_bounds(base_a, *ba[i]) if which == "a"
+                 else srv.submit_bounds(base_b, *bb[i]))
+            kinds[t] = (which, i)
+            ts.append(t)
+        return ts
+
+    tickets = sub("a", [0, 1]) + sub("b", [0, 1, 2])
+    t = srv.submit(fresh)
+    kinds[t] = ("fresh", 0)
+    tickets.append(t)
+    got = _drain(srv, tickets, submit_at={5: lambda: sub("b", [3, 4]) + sub("a", [2, 3, 4])})
+    srv.close()
+    assert len(got) == 11
+    for t, (kind, i) in kinds.items():
+        if kind == "fresh":
+            lp = fresh
+            res, xs, ys = native.solve_batch([lp], **opts)
+            alone = (res[0], xs[0], ys[0])
+        else:
+            g, b = (ga, ba[i]) if kind == "a" else (gb, bb[i])
+            lp = g.lp.with_col_bounds(*b)
+            alone = _alone_shared(g, b, opts)
+        _same(got[t], alone, f"{kind}{i}")
+        r, x, y = got[t]
+        if r["status"] == 0:
+            certify(lp, r, x, y, tag=f"{kind}{i}")
+
+
 def _alone_shared(g, b, opts):
     res, xs, ys = native.solve_batch_shared(g.lp, b[0][None], b[1][None], **opts)
     return res[0], xs[0], ys[0]
