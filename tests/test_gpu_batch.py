@@ -169,6 +169,39 @@ def test_batch_certificates_match_oracle(precision):
             assert abs(res[i]["objective"] - obj) <= 1e-4, (i, res[i])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_staged_evaluation_boundary(precision):
+    """The evaluation copies an instance's y^, y0 and x^ into LDS when 2m
+    and n are at most 8,192 (and the whole evaluation is one part): LPs
+    exactly at that size and one row or column past it, in one packed
+    batch and as bounds variants of a shared matrix, each Optimal and
+    certified by the long-double checker, with the single-LP path's
+    objective."""
+    lps = [lpgen.c0_plain(seed=61, m=4096, n=8192, per_col=4).lp,   # staged, at the limit
+           lpgen.c0_plain(seed=62, m=4097, n=8192, per_col=4).lp,   # 2m one past
+           lpgen.c0_plain(seed=63, m=2000, n=8193, per_col=4).lp]   # n one past
+    opts = dict(precision=precision, max_iter=300000, **TIGHT)
+    res, xs, ys = native.solve_batch(lps, **opts)
+    for i, (lp, r, x, y) in enumerate(zip(lps, res, xs, ys)):
+        assert r["status"] == 0, (i, r)
+        certify(lp, r, x, y, tag=f"packed{i}")
+        one, _, _ = native.solve(lp, **opts)
+        assert abs(one["objective"] - r["objective"]) <= 5e-6 * (1 + abs(one["objective"])), (i, r, one)
+    g = lpgen.c2_shared(seed=0xC6, m=4096, n=8192)
+    lo = np.empty((6, 8192))
+    hi = np.empty((6, 8192))
+    for i in range(6):
+        lo[i], hi[i] = g.branch_bounds(lpgen.c2_instance_seed(i, base=0xC6))
+    res, X, Y = native.solve_batch_shared(g.lp, lo, hi, **opts)
+    for i in range(6):
+        lp = g.lp.with_col_bounds(lo[i], hi[i])
+        if res[i]["status"] == 0:
+            certify(lp, res[i], X[i], Y[i], tag=f"shared{i}")
+        else:
+            ro, _, _ = ob.solve(lp, max_iter=300000, **TIGHT)
+            assert res[i]["status"] == ro["status"], (i, res[i], ro)
+
+
 @pytest.mark.parametrize("which", ["c1_small", "c2_small"])
 def test_batched_setup_vs_oracle(which):
     """The batched engines' cohort setup: element-wise scaling bit-identical
