@@ -244,7 +244,23 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
     constexpr int PC = kBlock / GC, PR = kBlock / GR;
     const int lc = tid % GC, gc = tid / GC;
     double cf[kNumColFields] = {0, 0, 0, 0, 0, 0};
+    // Column starts of the coming pass, loaded while the current pass's
+    // entries are in flight (the pass then needs one round trip less);
+    // not in the FP32 staged kernel, where the extra registers would cost a
+    // wave per SIMD.
+    constexpr bool kAhead = !(kStage && sizeof(T) == 4);
+    int nb[kEvalCW], ne[kEvalCW];
+    auto colStarts = [&](int b) {
+#pragma unroll
+        for (int w = 0; w < kEvalCW; ++w) {
+            const int jj = b + gc + w * PC;
+            nb[w] = jj < je ? in.cs[jj] : 0;
+            ne[w] = jj < je ? in.cs[jj + 1] : 0;
+        }
+    };
+    if constexpr (kAhead) colStarts(jb);
     for (int base = jb; base < je; base += kEvalCW * PC) {
+        if constexpr (!kAhead) colStarts(base);
         int j[kEvalCW], qb[kEvalCW], qe[kEvalCW];
         double aty[kEvalCW], atdy[kEvalCW], xj[kEvalCW], x0j[kEvalCW];
         // The column's own operands, loaded with its first loads (lane 0 of
@@ -255,8 +271,8 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
         for (int w = 0; w < kEvalCW; ++w) {
             j[w] = base + gc + w * PC;
             const bool ok = j[w] < je;
-            qb[w] = ok ? in.cs[j[w]] + lc : 0;
-            qe[w] = ok ? in.cs[j[w] + 1] : 0;
+            qb[w] = ok ? nb[w] + lc : 0;
+            qe[w] = ok ? ne[w] : 0;
             xj[w] = ok ? static_cast<double>(v.xh[j[w]]) : 0.0;
             x0j[w] = ok ? static_cast<double>(v.x0[j[w]]) : 0.0;
             const bool own = ok && lc == 0;
@@ -268,6 +284,7 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
             most = cnt > most ? cnt : most;
             aty[w] = atdy[w] = 0.0;
         }
+        if constexpr (kAhead) colStarts(base + kEvalCW * PC);
         for (int t = 0; t < most; ++t) {
 #pragma unroll
             for (int w = 0; w < kEvalCW; ++w) {
@@ -356,7 +373,17 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
     }
     const int lr = tid % GR, gr = tid / GR;
     double rf[kNumRowFields] = {0, 0, 0, 0};
+    auto rowStarts = [&](int b) {
+#pragma unroll
+        for (int w = 0; w < kEvalCW; ++w) {
+            const int ii = b + gr + w * PR;
+            nb[w] = ii < ie ? in.rs[ii] : 0;
+            ne[w] = ii < ie ? in.rs[ii + 1] : 0;
+        }
+    };
+    if constexpr (kAhead) rowStarts(ib);
     for (int base = ib; base < ie; base += kEvalCW * PR) {
+        if constexpr (!kAhead) rowStarts(base);
         int i[kEvalCW], qb[kEvalCW], qe[kEvalCW];
         double ax[kEvalCW], yv[kEvalCW], y0v[kEvalCW], loi[kEvalCW], hii[kEvalCW], rsi[kEvalCW];
         int most = 0;
@@ -364,8 +391,8 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
         for (int w = 0; w < kEvalCW; ++w) {
             i[w] = base + gr + w * PR;
             const bool ok = i[w] < ie;
-            qb[w] = ok ? in.rs[i[w]] + lr : 0;
-            qe[w] = ok ? in.rs[i[w] + 1] : 0;
+            qb[w] = ok ? nb[w] + lr : 0;
+            qe[w] = ok ? ne[w] : 0;
             yv[w] = ok ? static_cast<double>(v.yh[i[w]]) : 0.0;
             y0v[w] = ok ? static_cast<double>(v.y0[i[w]]) : 0.0;
             const bool own = ok && lr == 0;
@@ -376,6 +403,7 @@ __device__ inline void bEvalFields(const BInst& in, const BVecs<T>& v, const BCt
             most = cnt > most ? cnt : most;
             ax[w] = 0.0;
         }
+        if constexpr (kAhead) rowStarts(base + kEvalCW * PR);
         for (int t = 0; t < most; ++t) {
 #pragma unroll
             for (int w = 0; w < kEvalCW; ++w) {
