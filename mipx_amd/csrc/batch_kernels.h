@@ -553,22 +553,69 @@ __device__ inline void refineRecenter(const BInst& in) {
 __device__ inline void refineRecenterSlice(const BInst& in, int jb, int je, int ib, int ie) {
     float* lf = const_cast<float*>(in.f.l);
     float* uf = const_cast<float*>(in.f.u);
-    for (int j = jb + static_cast<int>(threadIdx.x); j < je; j += kBlock) {
-        double g = 0.0;
-        for (int k = in.cs[j]; k < in.cs[j + 1]; ++k) g += in.valt[k] * in.d.y[in.ri[k]];
-        const double xb = in.d.x[j];
-        in.rc[j] = static_cast<float>(in.c[j] + g);
-        lf[j] = static_cast<float>(in.l[j] - xb);
-        uf[j] = static_cast<float>(in.u[j] - xb);
-        in.f.x[j] = in.f.x0[j] = in.f.xh[j] = 0.0f;
+    // A thread takes its columns (rows) kEvalCW at a time, their entry
+    // chains in flight together; each sum still runs over its own entries
+    // in storage order.
+    constexpr int W = kEvalCW;
+    const int tid = static_cast<int>(threadIdx.x);
+    for (int j0 = jb + tid; j0 < je; j0 += W * kBlock) {
+        int cb[W], ce[W];
+        double g[W];
+        int most = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int j = j0 + w * kBlock;
+            cb[w] = j < je ? in.cs[j] : 0;
+            ce[w] = j < je ? in.cs[j + 1] : 0;
+            most = ce[w] - cb[w] > most ? ce[w] - cb[w] : most;
+            g[w] = 0.0;
+        }
+        for (int t = 0; t < most; ++t) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const int k = cb[w] + t;
+                if (k < ce[w]) g[w] += in.valt[k] * in.d.y[in.ri[k]];
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int j = j0 + w * kBlock;
+            if (j >= je) break;
+            const double xb = in.d.x[j];
+            in.rc[j] = static_cast<float>(in.c[j] + g[w]);
+            lf[j] = static_cast<float>(in.l[j] - xb);
+            uf[j] = static_cast<float>(in.u[j] - xb);
+            in.f.x[j] = in.f.x0[j] = in.f.xh[j] = 0.0f;
+        }
     }
-    for (int i = ib + static_cast<int>(threadIdx.x); i < ie; i += kBlock) {
-        double r = 0.0;
-        for (int k = in.rs[i]; k < in.rs[i + 1]; ++k) r += in.val[k] * in.d.x[in.ci[k]];
-        in.rrl[i] = static_cast<float>(in.rl[i] - r);
-        in.rru[i] = static_cast<float>(in.ru[i] - r);
-        in.rq[i] = static_cast<float>(in.d.y[i]);
-        in.f.y[i] = in.f.y0[i] = in.f.yh[i] = 0.0f;
+    for (int i0 = ib + tid; i0 < ie; i0 += W * kBlock) {
+        int cb[W], ce[W];
+        double r[W];
+        int most = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int i = i0 + w * kBlock;
+            cb[w] = i < ie ? in.rs[i] : 0;
+            ce[w] = i < ie ? in.rs[i + 1] : 0;
+            most = ce[w] - cb[w] > most ? ce[w] - cb[w] : most;
+            r[w] = 0.0;
+        }
+        for (int t = 0; t < most; ++t) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const int k = cb[w] + t;
+                if (k < ce[w]) r[w] += in.val[k] * in.d.x[in.ci[k]];
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int i = i0 + w * kBlock;
+            if (i >= ie) break;
+            in.rrl[i] = static_cast<float>(in.rl[i] - r[w]);
+            in.rru[i] = static_cast<float>(in.ru[i] - r[w]);
+            in.rq[i] = static_cast<float>(in.d.y[i]);
+            in.f.y[i] = in.f.y0[i] = in.f.yh[i] = 0.0f;
+        }
     }
 }
 
